@@ -1,0 +1,154 @@
+/*
+ * srx.h -- C ABI of libsrx, the MI355X EVP-FFT crystal-plasticity solver.
+ *
+ * The reference (mesoOSU/SRX) has no plugin or FFI interface: its hot path is
+ * reached through the process entry `trial <deck>` (evp.c:7-31), which calls
+ * three void(void) functions over process globals.  libsrx replaces that
+ * entry with a context-based C ABI.  Each declaration below names the
+ * reference interface it replaces.
+ *
+ * Conventions: plain pointers and sizes; every call returns 0 or an error
+ * code (never aborts the process); the reference's own PError/exit codes are
+ * kept (4, 7, 28, 35, 114, 116, 230, 445, 604, 720, 1110, 1139) and new codes
+ * start at 2000.  One srx_ctx per host thread; contexts are independent.
+ * Field arrays use the reference's voxel order (x slowest, index
+ * (x*N1 + y)*N2 + z, evp.h:60) and its per-voxel layout (3x3 row-major
+ * tensors, 12 values per slip-system array).
+ */
+#ifndef SRX_H
+#define SRX_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct srx_ctx srx_ctx;
+
+/* Precision of the FFT + Green-operator passes. */
+enum { SRX_PREC_FP64 = 0, SRX_PREC_MIXED = 1 };
+
+/* Error codes added by libsrx (the reference aborts or fails silently). */
+enum {
+    SRX_OK = 0,
+    SRX_E_FILE = 2001,          /* deck, slip-mode or mesh file cannot be opened */
+    SRX_E_GRID = 2002,          /* CellDim not supported (each extent must be 2^a 3^b 5^c, <= 1024) */
+    SRX_E_HIP = 2003,           /* HIP runtime error */
+    SRX_E_SYSTEMS = 2004,       /* more than 12 slip systems in a mode */
+    SRX_E_NOMEM = 2005,         /* allocation failed */
+    SRX_E_DONE = 2006,          /* srx_step after the last increment */
+    SRX_E_CREEP_STOP = 2007,    /* creep run reached e_vm > 0.3 (evolution.c:1020-1022) */
+    SRX_E_SIZE = 2008,          /* caller buffer too small */
+    SRX_E_FIELD = 2009,         /* unknown field id */
+    SRX_E_ARG = 2010,           /* invalid argument */
+    SRX_E_NAN = 3001,           /* NaN in the constitutive update (the reference asserts, constitutive.c:660,683,707) */
+    SRX_E_UPDATE_FLAG = 3002    /* Update_Flag 0 leaves Schm_gr uninitialised at istep 0 (init.c:815): rejected */
+};
+
+typedef struct {
+    int    precision;        /* SRX_PREC_FP64 (default) or SRX_PREC_MIXED */
+    int    fixed_iters;      /* >0: exactly this many outer iterations per increment (parity runs) */
+    double err_override;     /* >0: replaces the deck's Err */
+    int    itermax_override; /* >0: replaces the deck's IterMax */
+    int    nsteps_override;  /* >0: replaces the deck's N_steps */
+    int    tau0_mode;        /* 0: lattice resistance = tau0xf of the phase; 1: tau0_value */
+    double tau0_value;
+    int    svm_mode;         /* 0: reproduce evolution.c:648 (out-of-bounds term := svm_stray); 1: standard */
+    double svm_stray;
+    int    write_files;      /* 1: write vm.out, dd.out, err.out, drx.out into the work directory */
+    int    write_fields;     /* 1: also the .iout field dumps at PrintControl steps */
+    int    nthreads;         /* host threads for setup (0: default) */
+    int    verbose;          /* 1: per-iteration errors on stdout */
+    int    warm_start;       /* reserved */
+    int    reserved[8];
+} srx_options;
+
+/* Everything the reference prints or writes per increment, plus counters it
+   does not keep (iterations, NR statistics). */
+typedef struct {
+    int    istep;
+    int    iters;               /* outer fixed-point iterations run */
+    int    converged;           /* max(|Err_s|,|Err_e|) <= Err at exit */
+    int    nr_iters_max;        /* largest per-voxel NR iteration count in the increment */
+    long   nr_iters_total;      /* sum of per-voxel NR iterations over the increment */
+    double err_s, err_e;        /* normalised errors at exit (evolution.c:1246-1252) */
+    double time_tot, e_vm, evmp, d_vm, dvmp, s_vm, s_vm1;   /* vm.out columns */
+    double rho_F_avg, rho_P_avg, rho_s_avg, rho_m_avg, rho_g_avg;
+    double rho_dipole_avg, rho_forest_avg, rho_inplane_avg; /* AvgDislocation */
+    double sig_avg[9], sig_avg1[9], eps_avg[9], edot_avg[9];
+    double disgrad_avg_actual[9], velgrad_avg[9];
+    double reserved[16];
+} srx_step_report;
+
+/* Field ids for srx_get_field / srx_set_field (values per voxel in brackets). */
+enum {
+    SRX_F_SIG = 0,        /* [9] Cauchy stress Sig (evp.h:429) */
+    SRX_F_EPS = 1,        /* [9] plastic strain Eps */
+    SRX_F_EDOT = 2,       /* [9] plastic strain rate Edot */
+    SRX_F_DISGRAD = 3,    /* [9] displacement gradient DisGrad */
+    SRX_F_VELGRAD = 4,    /* [9] velocity gradient VelGrad of the last increment */
+    SRX_F_TRANMAT = 5,    /* [9] crystal->sample matrix TranMat_xt2sa */
+    SRX_F_RHO_S = 6,      /* [12] SSD density */
+    SRX_F_RHO_M = 7,      /* [12] mobile density */
+    SRX_F_RHO_F = 8,      /* [12] forest density of the last constitutive call */
+    SRX_F_RHO_P = 9,      /* [12] parallel density of the last constitutive call */
+    SRX_F_GAMDOT = 10,    /* [12] slip rates */
+    SRX_F_GAM_ACUM = 11,  /* [12] accumulated slip */
+    SRX_F_RHO_DIPOLE = 12, SRX_F_RHO_FOREST = 13, SRX_F_RHO_INPLANE = 14,   /* [12] */
+    SRX_F_NR_ITERS = 15,  /* [1] NR iterations of the last outer iteration */
+    SRX_F_FLUCT = 16,     /* [9] accumulated strain fluctuation (evolution.c:1198) */
+    SRX_F_TRIAL_GAM_ACUM = 17, /* [12] */
+    SRX_F_RHO_TOT = 18,   /* [1] rho_tot used by the last hardening update */
+    SRX_F_PHASE = 19, SRX_F_GRAIN = 20   /* [1] as doubles */
+};
+
+/* Replaces GetNameList + SetupJob (io.c:96-164, init.c:728-1216) and
+   OpenFiles (io.c:213-228).  `workdir` is the directory relative paths in the
+   deck are resolved against and outputs are written to (the reference uses
+   the CWD); NULL means the deck's directory.  `opt` may be NULL. */
+int srx_open(const char *deck_path, const char *workdir, const srx_options *opt, srx_ctx **out);
+
+/* One loading increment: evolution.c:726-1520 (one pass of the istep loop). */
+int srx_step(srx_ctx *ctx, srx_step_report *rep);
+
+/* All remaining increments (the istep loop of Evolution, evolution.c:667-1551). */
+int srx_run(srx_ctx *ctx);
+
+/* Copy a field to a caller buffer of n doubles (see the field ids). */
+int srx_get_field(srx_ctx *ctx, int field_id, double *host, size_t n);
+
+/* Overwrite a state field from a caller buffer (SIG, EPS, DISGRAD, TRANMAT,
+   RHO_S, RHO_M, GAM_ACUM); used by tests to start from a given state. */
+int srx_set_field(srx_ctx *ctx, int field_id, const double *host, size_t n);
+
+/* Grid extents CellDim[3]. */
+int srx_dims(srx_ctx *ctx, int *n3);
+
+/* Replaces DestroyJob + CloseFiles (init.c:1218-1416, io.c:230-245). */
+void srx_close(srx_ctx *ctx);
+
+/* Text for an error code. */
+const char *srx_strerror(int code);
+
+void srx_default_options(srx_options *opt);
+
+/* Per-kernel timing of the last srx_step, from HIP events on the solver's
+   stream: names[i] and total milliseconds ms[i] for i < *count. */
+int srx_kernel_times(srx_ctx *ctx, const char **names, double *ms, int *calls, int *count);
+
+/* ---- single-kernel entry points (for the parity tests) ---------------- */
+
+/* Forward 3D FFT of `ncomp` real fields (each n0*n1*n2, x slowest) to the
+   half spectrum along z: out[c][kx][ky][kz] complex (re,im), kz < n2/2+1.
+   Unnormalised, sign -1, the DFT the reference gets from fftwnd_mpi
+   (evolution.c:1163).  precision: SRX_PREC_FP64 or SRX_PREC_MIXED. */
+int srx_fft_forward(const double *in, int ncomp, int n0, int n1, int n2, int precision, double *out);
+
+/* Inverse of srx_fft_forward for real output: out = IFFT(in) * scale. */
+int srx_fft_inverse(const double *in, int ncomp, int n0, int n1, int n2, int precision, double scale, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
