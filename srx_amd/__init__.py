@@ -1,0 +1,213 @@
+"""srx_amd -- MI355X EVP-FFT crystal-plasticity solver (ctypes front end of libsrx.so).
+
+The product is the C ABI in include/srx.h, implemented by the in-tree
+``srx_amd/libsrx.so`` (HIP kernels for gfx950 + C++ host driver).  This module
+loads that library and mirrors its entry points for Python callers; it has no
+fallback: if the library is missing, importing ``Solver`` raises.
+
+Reference interface mirrored (evp.c:7-31):
+    GetNameList + SetupJob  -> Solver(deck)          (srx_open)
+    one istep of Evolution  -> Solver.step()         (srx_step)
+    Evolution               -> Solver.run()          (srx_run)
+    DestroyJob              -> Solver.close()        (srx_close)
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsrx.so")
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+
+
+class Options(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int), ("fixed_iters", C.c_int), ("err_override", C.c_double),
+        ("itermax_override", C.c_int), ("nsteps_override", C.c_int), ("tau0_mode", C.c_int),
+        ("tau0_value", C.c_double), ("svm_mode", C.c_int), ("svm_stray", C.c_double),
+        ("write_files", C.c_int), ("write_fields", C.c_int), ("nthreads", C.c_int),
+        ("verbose", C.c_int), ("warm_start", C.c_int), ("reserved", C.c_int * 8),
+    ]
+
+
+class StepReport(C.Structure):
+    _fields_ = [
+        ("istep", C.c_int), ("iters", C.c_int), ("converged", C.c_int), ("nr_iters_max", C.c_int),
+        ("nr_iters_total", C.c_long), ("err_s", C.c_double), ("err_e", C.c_double),
+        ("time_tot", C.c_double), ("e_vm", C.c_double), ("evmp", C.c_double), ("d_vm", C.c_double),
+        ("dvmp", C.c_double), ("s_vm", C.c_double), ("s_vm1", C.c_double),
+        ("rho_F_avg", C.c_double), ("rho_P_avg", C.c_double), ("rho_s_avg", C.c_double),
+        ("rho_m_avg", C.c_double), ("rho_g_avg", C.c_double), ("rho_dipole_avg", C.c_double),
+        ("rho_forest_avg", C.c_double), ("rho_inplane_avg", C.c_double),
+        ("sig_avg", C.c_double * 9), ("sig_avg1", C.c_double * 9), ("eps_avg", C.c_double * 9),
+        ("edot_avg", C.c_double * 9), ("disgrad_avg_actual", C.c_double * 9),
+        ("velgrad_avg", C.c_double * 9), ("reserved", C.c_double * 16),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            v = getattr(self, name)
+            out[name] = list(v) if hasattr(v, "__len__") else v
+        return out
+
+
+FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velgrad": (4, 9),
+          "tranmat": (5, 9), "rho_s": (6, 12), "rho_m": (7, 12), "rho_F": (8, 12), "rho_P": (9, 12),
+          "gamdot": (10, 12), "gam_acum": (11, 12), "rho_dipole": (12, 12), "rho_forest": (13, 12),
+          "rho_inplane": (14, 12), "nr_iters": (15, 1), "fluct": (16, 9), "trial_gam_acum": (17, 12),
+          "phase": (19, 1), "grain": (20, 1)}
+
+EXPORTS = ["srx_open", "srx_step", "srx_run", "srx_get_field", "srx_set_field", "srx_dims", "srx_close",
+           "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse"]
+
+_lib = None
+
+
+def lib():
+    """Load libsrx.so (built by srx_amd/build.py).  Raises if it is absent."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError("libsrx.so is not built (run `python srx_amd/build.py`); "
+                               "there is no fallback path")
+        L = C.CDLL(LIB_PATH)
+        P = C.POINTER
+        L.srx_default_options.argtypes = [P(Options)]
+        L.srx_open.argtypes = [C.c_char_p, C.c_char_p, P(Options), P(C.c_void_p)]
+        L.srx_step.argtypes = [C.c_void_p, P(StepReport)]
+        L.srx_run.argtypes = [C.c_void_p]
+        L.srx_get_field.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_size_t]
+        L.srx_set_field.argtypes = [C.c_void_p, C.c_int, P(C.c_double), C.c_size_t]
+        L.srx_dims.argtypes = [C.c_void_p, P(C.c_int)]
+        L.srx_close.argtypes = [C.c_void_p]
+        L.srx_strerror.argtypes = [C.c_int]
+        L.srx_strerror.restype = C.c_char_p
+        L.srx_kernel_times.argtypes = [C.c_void_p, P(C.c_char_p), P(C.c_double), P(C.c_int), P(C.c_int)]
+        L.srx_fft_forward.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double)]
+        L.srx_fft_inverse.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                      P(C.c_double)]
+        _lib = L
+    return _lib
+
+
+def _dptr(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class SrxError(RuntimeError):
+    def __init__(self, code: int, where: str):
+        self.code = code
+        msg = lib().srx_strerror(code).decode()
+        super().__init__("%s: %s (code %d)" % (where, msg, code))
+
+
+def options(**kw) -> Options:
+    o = Options()
+    lib().srx_default_options(C.byref(o))
+    for k, v in kw.items():
+        if k == "reserved":
+            for i, x in enumerate(v):
+                o.reserved[i] = x
+        else:
+            setattr(o, k, v)
+    return o
+
+
+class Solver:
+    """One solver context (srx_open ... srx_close)."""
+
+    def __init__(self, deck: str, workdir: str | None = None, **opt):
+        self.L = lib()
+        self.h = C.c_void_p()
+        self.opt = options(**opt)
+        wd = workdir or os.path.dirname(os.path.abspath(deck))
+        rc = self.L.srx_open(deck.encode(), wd.encode(), C.byref(self.opt), C.byref(self.h))
+        if rc:
+            raise SrxError(rc, "srx_open")
+        n = (C.c_int * 3)()
+        self.L.srx_dims(self.h, n)
+        self.n = tuple(n)
+        self.nxyz = n[0] * n[1] * n[2]
+
+    def step(self) -> dict:
+        r = StepReport()
+        rc = self.L.srx_step(self.h, C.byref(r))
+        if rc:
+            raise SrxError(rc, "srx_step")
+        return r.as_dict()
+
+    def run(self) -> None:
+        rc = self.L.srx_run(self.h)
+        if rc:
+            raise SrxError(rc, "srx_run")
+
+    def field(self, name: str) -> np.ndarray:
+        fid, nc = FIELDS[name]
+        out = np.zeros(self.nxyz * nc)
+        rc = self.L.srx_get_field(self.h, fid, _dptr(out), out.size)
+        if rc:
+            raise SrxError(rc, "srx_get_field")
+        return out.reshape(self.nxyz, nc) if nc > 1 else out
+
+    def set_field(self, name: str, a: np.ndarray) -> None:
+        fid, nc = FIELDS[name]
+        a = np.ascontiguousarray(a, dtype=np.float64).ravel()
+        rc = self.L.srx_set_field(self.h, fid, _dptr(a), a.size)
+        if rc:
+            raise SrxError(rc, "srx_set_field")
+
+    def kernel_times(self) -> dict:
+        n = C.c_int(16)
+        names = (C.c_char_p * 16)()
+        ms = (C.c_double * 16)()
+        calls = (C.c_int * 16)()
+        self.L.srx_kernel_times(self.h, names, ms, calls, C.byref(n))
+        return {names[i].decode(): (ms[i], calls[i]) for i in range(n.value)}
+
+    def close(self) -> None:
+        if self.h:
+            self.L.srx_close(self.h)
+            self.h = C.c_void_p()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def fft_forward(x: np.ndarray, precision: int = PREC_FP64) -> np.ndarray:
+    """Half-spectrum forward FFT of real fields x[c, n0, n1, n2] on the GPU.
+    Returns complex [c, n0, n1, n2//2+1] (numpy.fft.rfftn convention)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    nc, n0, n1, n2 = x.shape
+    out = np.zeros(nc * n0 * n1 * (n2 // 2 + 1) * 2)
+    rc = lib().srx_fft_forward(_dptr(x), nc, n0, n1, n2, precision, _dptr(out))
+    if rc:
+        raise SrxError(rc, "srx_fft_forward")
+    return out.view(np.complex128).reshape(nc, n0, n1, n2 // 2 + 1)
+
+
+def fft_inverse(X: np.ndarray, n2: int, precision: int = PREC_FP64, scale: float = 1.0) -> np.ndarray:
+    """Real inverse of fft_forward: returns scale * irfftn-without-normalisation."""
+    X = np.ascontiguousarray(X, dtype=np.complex128)
+    nc, n0, n1, h = X.shape
+    xin = X.view(np.float64).ravel().copy()
+    out = np.zeros(nc * n0 * n1 * n2)
+    rc = lib().srx_fft_inverse(_dptr(xin), nc, n0, n1, n2, precision, scale, _dptr(out))
+    if rc:
+        raise SrxError(rc, "srx_fft_inverse")
+    return out.reshape(nc, n0, n1, n2)

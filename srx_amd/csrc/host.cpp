@@ -1,0 +1,1263 @@
+// host.cpp -- libsrx host side: input readers, setup, increment driver, C ABI.
+//
+// Replaces the reference's step driver (evp.c main -> GetNameList, SetupJob,
+// Evolution, DestroyJob).  The grid stays resident on one GPU; the host only
+// holds scalars between kernels (the 20 reduced values of each outer
+// iteration) and does the macro-stress / mixed-boundary-condition algebra of
+// get_smacro (evolution.c:590-664) on them.
+#include "srx_dev.hpp"
+#include "../../include/srx.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <string>
+#include <vector>
+
+namespace srx {
+void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *specB, int ncomp, double *out,
+                                  const int n[3], const FftPlan *pl, void *const *tw, double divisor);
+}
+
+using namespace srx;
+
+namespace {
+
+// ---------------------------------------------------------------------------
+// constants of evp.h:84-94
+// ---------------------------------------------------------------------------
+constexpr double PI_REF = 3.14159265359;
+constexpr double EV_IN_J = 1.60217733e-19;
+constexpr double J_IN_EV = (1.0 / EV_IN_J);
+constexpr double MY_J_IN_EV = 0.62415063631;
+constexpr double BOLZ = 1.380658e-23;
+constexpr double R2 = 0.70710678118654744, R3 = 0.57735026918962584, R6 = 0.40824829046386304;
+
+// B basis (kinematics.c:35-53)
+struct Basis {
+    double b[6][3][3];
+    Basis() {
+        std::memset(b, 0, sizeof(b));
+        b[0][0][0] = -R2; b[0][1][1] = R2;
+        b[1][0][0] = -R6; b[1][1][1] = -R6; b[1][2][2] = 2. * R6;
+        b[2][1][2] = R2; b[2][2][1] = R2;
+        b[3][0][2] = R2; b[3][2][0] = R2;
+        b[4][0][1] = R2; b[4][1][0] = R2;
+        b[5][0][0] = R3; b[5][1][1] = R3; b[5][2][2] = R3;
+    }
+};
+const Basis BB;
+
+void t9_to_b6(const double t[9], double v[6]) {
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) s += t[3 * a + b] * BB.b[i][a][b];
+        v[i] = s;
+    }
+}
+void b6_to_t9(const double v[6], double t[9], int dim = 6) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double s = 0.0;
+            for (int i = 0; i < dim; ++i) s += v[i] * BB.b[i][a][b];
+            t[3 * a + b] = s;
+        }
+}
+// 6x6 B-basis matrix -> 4th-order tensor (chg_basis opt 3)
+void c66_to_c4(const double c[36], double t[81]) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c3 = 0; c3 < 3; ++c3)
+                for (int d = 0; d < 3; ++d) {
+                    double s = 0.0;
+                    for (int i = 0; i < 6; ++i)
+                        for (int j = 0; j < 6; ++j) s += c[6 * i + j] * BB.b[i][a][b] * BB.b[j][c3][d];
+                    t[((a * 3 + b) * 3 + c3) * 3 + d] = s;
+                }
+}
+
+// Q6 rotation of B vectors (crystal -> sample for a = TranMat_xt2sa); 6x6 with hydro 1
+void q6_from(const double a[9], double Q[36]) {
+    std::memset(Q, 0, 36 * sizeof(double));
+    for (int be = 0; be < 5; ++be) {
+        double m[9];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double s = 0.0;
+                for (int k = 0; k < 3; ++k)
+                    for (int l = 0; l < 3; ++l) s += a[3 * i + k] * BB.b[be][k][l] * a[3 * j + l];
+                m[3 * i + j] = s;
+            }
+        double v[6];
+        t9_to_b6(m, v);
+        for (int al = 0; al < 5; ++al) Q[6 * al + be] = v[al];
+    }
+    Q[35] = 1.0;
+}
+
+// Gauss-Jordan inverse with partial pivoting (n <= 6)
+bool invert(double *A, int n) {
+    double M[6][12];
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < 2 * n; ++j) M[i][j] = j < n ? A[n * i + j] : (double)(j - n == i);
+    for (int c = 0; c < n; ++c) {
+        int p = c;
+        for (int r = c + 1; r < n; ++r)
+            if (std::fabs(M[r][c]) > std::fabs(M[p][c])) p = r;
+        if (M[p][c] == 0.0) return false;
+        if (p != c)
+            for (int j = 0; j < 2 * n; ++j) std::swap(M[p][j], M[c][j]);
+        const double r = 1.0 / M[c][c];
+        for (int j = 0; j < 2 * n; ++j) M[c][j] *= r;
+        for (int i = 0; i < n; ++i) {
+            if (i == c) continue;
+            const double f = M[i][c];
+            if (f != 0.0)
+                for (int j = 0; j < 2 * n; ++j) M[i][j] -= f * M[c][j];
+        }
+    }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < n; ++j) A[n * i + j] = M[i][j + n];
+    return true;
+}
+
+double von_mises(const double t[9]) {   // kinematics.c:465-483
+    const double tr = t[0] + t[4] + t[8];
+    double vm = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double d = (t[3 * i + j] + t[3 * j + i]) / 2. - (double)(i == j) * tr / 3.0;
+            vm += d * d;
+        }
+    return std::sqrt(2. / 3. * vm);
+}
+
+void voigt_to_full(const double *v, double m[9]) {   // kinematics.c:5-12
+    m[0] = v[0]; m[1] = v[5]; m[2] = v[4];
+    m[3] = v[5]; m[4] = v[1]; m[5] = v[3];
+    m[6] = v[4]; m[7] = v[3]; m[8] = v[2];
+}
+
+// EulerToTransMatrix (kinematics.c:187-240), opt 2: angles (rad) -> sample->crystal matrix
+void euler_to_matrix(double t1, double ph, double t2, double a[9]) {
+    const double ct1 = std::cos(t1), st1 = std::sin(t1), cph = std::cos(ph), sph = std::sin(ph);
+    const double ct2 = std::cos(t2), st2 = std::sin(t2);
+    a[0] = ct1 * ct2 - st1 * st2 * cph;
+    a[1] = st1 * ct2 + ct1 * st2 * cph;
+    a[2] = st2 * sph;
+    a[3] = -1.0 * ct1 * st2 - st1 * ct2 * cph;
+    a[4] = -1.0 * st1 * st2 + ct1 * ct2 * cph;
+    a[5] = ct2 * sph;
+    a[6] = st1 * sph;
+    a[7] = -1.0 * ct1 * sph;
+    a[8] = cph;
+}
+// opt 1: matrix -> angles in degrees
+void matrix_to_euler(const double a[9], double &t1, double &ph, double &t2) {
+    double p = std::acos(a[8]), x1, x2;
+    if (std::fabs(a[8]) > 0.9999) {
+        x2 = 0.;
+        x1 = std::atan2(a[1], a[0]);
+    } else {
+        const double sph = std::sin(p);
+        x1 = std::atan2(a[6] / sph, -1.0 * a[7] / sph);
+        x2 = std::atan2(a[2] / sph, a[5] / sph);
+    }
+    ph = p * 180. / PI_REF;
+    t1 = x1 * 180. / PI_REF;
+    t2 = x2 * 180. / PI_REF;
+}
+
+// ---------------------------------------------------------------------------
+// input deck: namelist semantics of io.c:96-164
+// ---------------------------------------------------------------------------
+struct Deck {
+    int CellDim[3] = {0, 0, 0}, N_phases = 0, Type_phases[2] = {0, 0}, N_steps = 0, Update_Flag = 0, Hard_Flag = 0;
+    int Tex_Flag = 0, IterMax = 0, PrintControl[2] = {0, 0}, VelGrad_BC_Flag[6] = {0}, Stress_BC_Flag[6] = {0};
+    int ElastBC = 0, nucleus_radius = 0, CellDim_pf[3] = {0}, RandSeed = 0, Nucl_Static_Flag = 0;
+    double TimeStep = 0, Err = 0, VelGrad_BC[6] = {0}, Stress_BC[6] = {0}, ElastConst_PhaseI[4] = {0};
+    double ElastConst_PhaseII[4] = {0};
+    char Slip_PhaseI[100] = {0}, Slip_PhaseII[100] = {0}, initial_ms[100] = {0};
+    double a0[2] = {0}, L0 = 0, GND_ScaleRatio = 0, T__K = 0, Q_slip[2] = {0}, Q_bulk[2] = {0};
+    double C_1[2] = {0}, C_2[2] = {0}, C_3[2] = {0}, C_4[2] = {0}, C_5[2] = {0}, C_6[2] = {0}, C_7[2] = {0};
+    double C_8[2] = {0};
+    double Selfinter0[2] = {0}, Coplanar0[2] = {0}, CrossSlip0[2] = {0}, GlissileJunction0[2] = {0};
+    double HirthLock0[2] = {0}, LomerCottrellLock0[2] = {0};
+    double rho_SSD_initial = 0, D_0 = 0, E_gb = 0, M_gb = 0, Scale_Fdeform = 0, k_c = 0, alpha = 0, zeta_kc = 0;
+    double age_drx = 0, ssdScaler_DRX = 0, gndScaler_DRX = 0, sigScaler_DRX = 0, tau_DRX = 0, DeltaQ_DRX = 0;
+    double DeltaTau_DRX = 0, pf_length_scale = 0, pf_time_step = 0;
+};
+
+struct Key {
+    const char *name;
+    void *ptr;
+    char type;   // 'i', 'r', 's'
+    int len;
+};
+
+int read_deck(const char *path, Deck &d) {
+    std::vector<Key> keys = {
+        {"CellDim", d.CellDim, 'i', 3}, {"N_phases", &d.N_phases, 'i', 1}, {"Type_phases", d.Type_phases, 'i', 2},
+        {"TimeStep", &d.TimeStep, 'r', 1}, {"N_steps", &d.N_steps, 'i', 1}, {"Update_Flag", &d.Update_Flag, 'i', 1},
+        {"Hard_Flag", &d.Hard_Flag, 'i', 1}, {"Tex_Flag", &d.Tex_Flag, 'i', 1}, {"IterMax", &d.IterMax, 'i', 1},
+        {"PrintControl", d.PrintControl, 'i', 2}, {"Err", &d.Err, 'r', 1},
+        {"VelGrad_BC_Flag", d.VelGrad_BC_Flag, 'i', 6}, {"VelGrad_BC", d.VelGrad_BC, 'r', 6},
+        {"Stress_BC_Flag", d.Stress_BC_Flag, 'i', 6}, {"Stress_BC", d.Stress_BC, 'r', 6},
+        {"ElastBC", &d.ElastBC, 'i', 1}, {"ElastConst_PhaseI", d.ElastConst_PhaseI, 'r', 4},
+        {"ElastConst_PhaseII", d.ElastConst_PhaseII, 'r', 4}, {"Slip_PhaseI", d.Slip_PhaseI, 's', 1},
+        {"Slip_PhaseII", d.Slip_PhaseII, 's', 1}, {"initial_ms", d.initial_ms, 's', 1}, {"a0", d.a0, 'r', 2},
+        {"L0", &d.L0, 'r', 1}, {"GND_ScaleRatio", &d.GND_ScaleRatio, 'r', 1}, {"T__K", &d.T__K, 'r', 1},
+        {"Q_slip", d.Q_slip, 'r', 2}, {"Q_bulk", d.Q_bulk, 'r', 2}, {"C_1", d.C_1, 'r', 2}, {"C_2", d.C_2, 'r', 2},
+        {"C_3", d.C_3, 'r', 2}, {"C_4", d.C_4, 'r', 2}, {"C_5", d.C_5, 'r', 2}, {"C_6", d.C_6, 'r', 2},
+        {"C_7", d.C_7, 'r', 2}, {"C_8", d.C_8, 'r', 2}, {"Selfinter0", d.Selfinter0, 'r', 2},
+        {"Coplanar0", d.Coplanar0, 'r', 2}, {"CrossSlip0", d.CrossSlip0, 'r', 2},
+        {"GlissileJunction0", d.GlissileJunction0, 'r', 2}, {"HirthLock0", d.HirthLock0, 'r', 2},
+        {"LomerCottrellLock0", d.LomerCottrellLock0, 'r', 2}, {"rho_SSD_initial", &d.rho_SSD_initial, 'r', 1},
+        {"D_0", &d.D_0, 'r', 1}, {"nucleus_radius", &d.nucleus_radius, 'i', 1}, {"CellDim_pf", d.CellDim_pf, 'i', 3},
+        {"RandSeed", &d.RandSeed, 'i', 1}, {"E_gb", &d.E_gb, 'r', 1}, {"M_gb", &d.M_gb, 'r', 1},
+        {"Scale_Fdeform", &d.Scale_Fdeform, 'r', 1}, {"k_c", &d.k_c, 'r', 1},
+        {"Nucl_Static_Flag", &d.Nucl_Static_Flag, 'i', 1}, {"alpha", &d.alpha, 'r', 1},
+        {"zeta_kc", &d.zeta_kc, 'r', 1}, {"age_drx", &d.age_drx, 'r', 1}, {"ssdScaler_DRX", &d.ssdScaler_DRX, 'r', 1},
+        {"gndScaler_DRX", &d.gndScaler_DRX, 'r', 1}, {"sigScaler_DRX", &d.sigScaler_DRX, 'r', 1},
+        {"tau_DRX", &d.tau_DRX, 'r', 1}, {"DeltaQ_DRX", &d.DeltaQ_DRX, 'r', 1},
+        {"DeltaTau_DRX", &d.DeltaTau_DRX, 'r', 1}, {"pf_length_scale", &d.pf_length_scale, 'r', 1},
+        {"pf_time_step", &d.pf_time_step, 'r', 1},
+    };
+    std::vector<char> seen(keys.size(), 0);
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return SRX_E_FILE;
+    char buff[200];
+    while (std::fgets(buff, 200, fp)) {
+        if (std::feof(fp)) break;                 // a last line without a newline is not read
+        if (buff[0] == '#' || buff[0] == '\n') continue;
+        char *save = nullptr;
+        char *tok = strtok_r(buff, " \t\n", &save);
+        if (!tok) break;                          // a blank-but-whitespace line ends the deck
+        for (size_t k = 0; k < keys.size(); ++k) {
+            if (std::strcmp(tok, keys[k].name) != 0) continue;
+            if (!seen[k]) {                       // later duplicates are ignored
+                seen[k] = 1;
+                for (int j = 0; j < keys[k].len; ++j) {
+                    char *t = strtok_r(nullptr, ", \t\n", &save);
+                    if (!t) break;
+                    if (keys[k].type == 'i') static_cast<int *>(keys[k].ptr)[j] = (int)std::atol(t);
+                    else if (keys[k].type == 'r') static_cast<double *>(keys[k].ptr)[j] = std::atof(t);
+                    else { std::snprintf(static_cast<char *>(keys[k].ptr), 100, "%s", t); }
+                }
+            }
+            break;
+        }
+    }
+    std::fclose(fp);
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// slip-mode file: PlasticInit (init.c:52-231)
+// ---------------------------------------------------------------------------
+struct PhaseHost {
+    int xtal = 0, nsys = 0;
+    double c11 = 0, c12 = 0, c44 = 0, nu = 0, G = 0;
+    double bn[NSYS][3] = {{0}}, bb[NSYS][3] = {{0}}, p[NSYS][5] = {{0}};
+    double tau0xf = 0, b = 0;
+};
+
+int read_sx(const char *path, PhaseHost &P) {
+    FILE *fp = std::fopen(path, "r");
+    if (!fp) return SRX_E_FILE;
+    char buf[80];
+    auto gl = [&]() { if (!std::fgets(buf, 80, fp)) buf[0] = 0; };
+    int nmodes_max = 0, nmodes = 0, imode[16] = {0};
+    for (int i = 0; i < 7; ++i) gl();
+    gl(); std::sscanf(buf, "%d", &nmodes_max);
+    gl(); gl();
+    gl(); std::sscanf(buf, "%d", &nmodes);
+    nmodes = std::min(nmodes, 16);
+    gl(); gl();
+    for (int i = 0; i < nmodes; ++i) { gl(); std::sscanf(buf, "%d", &imode[i]); }
+    gl();
+    P.nsys = 0;
+    for (int m = 0; m < nmodes_max; ++m) {
+        int mx = -1, nsmx = 0, isect = 0;
+        double nrsx, gamd0x, twshx, tau0xf = 0, tau0xb, tau1x, thet0 = 0, thet1 = 0, hs, hl;
+        gl(); std::sscanf(buf, "%*s %*s %d %*s", &mx);
+        bool match = false;
+        for (int j = 0; j < nmodes; ++j)
+            if (mx == imode[j]) { match = true; break; }
+        gl(); gl(); std::sscanf(buf, "%d %lf %lf %lf %d", &nsmx, &nrsx, &gamd0x, &twshx, &isect);
+        gl(); gl(); std::sscanf(buf, "%lf %lf %lf %lf %lf", &tau0xf, &tau0xb, &tau1x, &thet0, &thet1);
+        gl(); gl(); std::sscanf(buf, "%lf %lf", &hs, &hl);
+        gl();
+        if (match) {
+            if (nsmx > NSYS) { std::fclose(fp); return SRX_E_SYSTEMS; }
+            P.nsys = nsmx;
+            if (thet0 < thet1) { std::fclose(fp); return 230; }
+            P.tau0xf = tau0xf;
+            for (int j = 0; j < nsmx; ++j) {
+                gl();
+                std::sscanf(buf, "%lf %lf %lf %lf %lf %lf", &P.bn[j][0], &P.bn[j][1], &P.bn[j][2], &P.bb[j][0],
+                            &P.bb[j][1], &P.bb[j][2]);
+            }
+        } else {
+            for (int j = 0; j < nsmx; ++j) gl();
+        }
+    }
+    std::fclose(fp);
+    for (int j = 0; j < P.nsys; ++j) {
+        double nn = std::sqrt(std::pow(P.bn[j][0], 2.0) + std::pow(P.bn[j][1], 2.0) + std::pow(P.bn[j][2], 2.0));
+        for (int a = 0; a < 3; ++a) P.bn[j][a] /= nn;
+        nn = std::sqrt(std::pow(P.bb[j][0], 2.0) + std::pow(P.bb[j][1], 2.0) + std::pow(P.bb[j][2], 2.0));
+        for (int a = 0; a < 3; ++a) P.bb[j][a] /= nn;
+        const double dot = P.bn[j][0] * P.bb[j][0] + P.bn[j][1] * P.bb[j][1] + P.bn[j][2] * P.bb[j][2];
+        if (dot > 1E-3) return 7;
+        double s[9], v[6];
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) s[3 * a + b] = (P.bn[j][a] * P.bb[j][b] + P.bn[j][b] * P.bb[j][a]) / 2.0;
+        t9_to_b6(s, v);
+        for (int a = 0; a < 5; ++a) P.p[j][a] = v[a];
+    }
+    return 0;
+}
+
+// Arsenlis-Parks table (init.c:594-703): S self, C coplanar, X cross slip,
+// G glissile, H Hirth, L Lomer; the upper triangle row by row.
+const char *const ISM[12] = {"SCCHGLXGGHLG", "SCGXGGHLLHG", "SLGHGLHGGX", "SCCHLGXGG", "SCLHGGHL", "SGGXGLH",
+                             "SCCHGL",       "SCGXG",       "SLGH",       "SCC",       "SC",       "S"};
+void interaction(double self, double cop, double cross, double gli, double hir, double lom, double chi[NSYS][NSYS]) {
+    for (int i = 0; i < 12; ++i)
+        for (int j = i; j < 12; ++j) {
+            const char c = ISM[i][j - i];
+            const double v = c == 'S' ? self : c == 'C' ? cop : c == 'X' ? cross : c == 'G' ? gli : c == 'H' ? hir : lom;
+            chi[i][j] = chi[j][i] = v;
+        }
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// the solver context
+// ---------------------------------------------------------------------------
+struct srx_ctx {
+    srx_options opt;
+    Deck dk;
+    std::string workdir;
+    int n[3] = {0, 0, 0};
+    long nvox = 0;
+    double wgt = 0, wgt_ph1 = 0;
+    PhaseHost ph[NPH];
+    DevConst hc;
+    DevConst *dc = nullptr;
+    Fields f;
+    void *specB = nullptr, *specC = nullptr;
+    void *tw[3] = {nullptr, nullptr, nullptr};
+    FftPlan pl[3];
+    double *partials = nullptr, *red = nullptr, *hred = nullptr;
+    int nr_blocks = 0, end_blocks = 0;
+    hipStream_t stream = nullptr;
+    // macro state
+    double Udot[9], Udot_s[9], Udot_a[9], Scauchy[9];
+    double DisGradAvg[9], DisGradAvg_t[9], dDisGradAvg[9], acum[9], uniform[9];
+    double SigAvg[9], SigAvg1[9];
+    double C066[36], S066[36];
+    double e_vm = 0, d_vm = 0, s_vm = 0, s_vm1 = 0, TimeTot = 0, err = 0, Err_e = 0, Err_s = 0;
+    double sig_sum6[6] = {0, 0, 0, 0, 0, 0};
+    int itermax = 0, nsteps = 0, istep = 0, creep = 0;
+    std::vector<int> grain;
+    FILE *fvm = nullptr, *fdd = nullptr, *ferr = nullptr, *fdrx = nullptr;
+    // timing
+    bool timing = false;
+    std::vector<hipEvent_t> evpool;
+    struct KT { const char *name; double ms; int calls; };
+    KT kt[8];
+    int nkt = 0;
+};
+
+namespace {
+
+#define HIPCK(x)                                                                                         \
+    do {                                                                                                 \
+        hipError_t e_ = (x);                                                                             \
+        if (e_ != hipSuccess) {                                                                          \
+            std::fprintf(stderr, "srx: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
+            return SRX_E_HIP;                                                                            \
+        }                                                                                                \
+    } while (0)
+
+std::string resolve(const std::string &dir, const char *name) {
+    if (name[0] == '/' || dir.empty()) return name;
+    return dir + "/" + name;
+}
+
+// largest LDS tile check: every pass must fit its butterflies and 64 KB of LDS
+bool grid_supported(const FftPlan pl[3], const int n[3]) {
+    for (int ax = 0; ax < 3; ++ax) {
+        const FftPlan &p = pl[ax];
+        int mr = 1 << 30;
+        for (int i = 0; i < p.nst; ++i) mr = std::min(mr, p.rad[i]);
+        if (p.nst == 0) mr = 1;
+        const int slots = ax == 0 ? 9 : 1;
+        const int maxb = ax == 0 ? 6 : 4;
+        if ((long)slots * p.n / std::max(mr, 1) > (long)maxb * 256) return false;
+        if ((long)slots * p.n * 16 > 65536) return false;
+    }
+    return n[0] >= 1 && n[1] >= 1 && n[2] >= 1;
+}
+
+template <typename T>
+int upload_twiddles(int n, void **dst) {
+    std::vector<T> h(2 * std::max(n, 1));
+    for (int m = 0; m < n; ++m) {
+        const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
+        h[2 * m] = (T)std::cos(ang);
+        h[2 * m + 1] = (T)std::sin(ang);
+    }
+    HIPCK(hipMalloc(dst, h.size() * sizeof(T)));
+    HIPCK(hipMemcpy(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    return 0;
+}
+
+int setup(srx_ctx *c) {
+    Deck &d = c->dk;
+    for (int i = 0; i < 3; ++i) c->n[i] = d.CellDim[i];
+    for (int i = 0; i < 3; ++i)
+        if (!make_plan(c->n[i], c->pl[i])) return SRX_E_GRID;
+    if (!grid_supported(c->pl, c->n)) return SRX_E_GRID;
+    c->nvox = (long)c->n[0] * c->n[1] * c->n[2];
+    c->wgt = 1. / c->nvox;
+    const double kT = (d.T__K * BOLZ * J_IN_EV);
+    if (d.Update_Flag != 1) return SRX_E_UPDATE_FLAG;
+
+    // phases (init.c:815-856)
+    std::memset(&c->hc, 0, sizeof(c->hc));
+    for (int p = 0; p < NPH; ++p) c->ph[p].xtal = (p < d.N_phases) && d.Type_phases[p] == 0;
+    if (c->ph[0].xtal) {   // Cu constants at T__K override the deck (init.c:822-826)
+        d.ElastConst_PhaseI[0] = (1.818 - 0.0004045 * d.T__K) * 100 * 1000;
+        d.ElastConst_PhaseI[1] = (1.285 - 0.0002 * d.T__K) * 100 * 1000;
+        d.ElastConst_PhaseI[2] = (0.8396 - 0.0002711 * d.T__K) * 100 * 1000;
+    }
+    for (int p = 0; p < NPH; ++p) {
+        PhaseHost &P = c->ph[p];
+        DevPhase &D = c->hc.ph[p];
+        D.xtal = P.xtal;
+        // interaction matrices exist for every phase (init.c:1100-1116)
+        double chi[NSYS][NSYS], chi1[NSYS][NSYS];
+        interaction(d.Selfinter0[p], d.Coplanar0[p], 0.0, 0.0, 0.0, 0.0, chi);
+        interaction(0.0, 0.0, d.CrossSlip0[p], d.GlissileJunction0[p], d.HirthLock0[p], d.LomerCottrellLock0[p], chi1);
+        if (!P.xtal) continue;
+        const double *E = p == 0 ? d.ElastConst_PhaseI : d.ElastConst_PhaseII;
+        P.c11 = E[0]; P.c12 = E[1]; P.c44 = E[2]; P.nu = E[3];
+        P.G = 0.1 * (E[0] - E[1] + 3 * E[2]) + (0.5) * (5 * (E[0] - E[1]) * E[2]) / (4 * E[2] + 3 * (E[0] - E[1]));
+        const std::string sx = resolve(c->workdir, p == 0 ? d.Slip_PhaseI : d.Slip_PhaseII);
+        int rc = read_sx(sx.c_str(), P);
+        if (rc) return rc;
+        P.b = d.a0[p] / std::sqrt(2.0);
+        D.nsys = P.nsys;
+        for (int s = 0; s < NSYS; ++s) {
+            for (int k = 0; k < 5; ++k) D.p[s][k] = s < P.nsys ? P.p[s][k] : 0.0;
+            for (int k = 0; k < 3; ++k) { D.bn[s][k] = P.bn[s][k]; D.bb[s][k] = P.bb[s][k]; }
+            for (int j = 0; j < NSYS; ++j) {
+                const bool in = s < P.nsys && j < P.nsys;
+                D.chi[s][j] = in ? chi[s][j] : 0.0;
+                D.chi1[s][j] = in ? chi1[s][j] : 0.0;
+            }
+        }
+        const double cd[6] = {P.c11 - P.c12, P.c11 - P.c12, 2 * P.c44, 2 * P.c44, 2 * P.c44, P.c11 + 2 * P.c12};
+        for (int k = 0; k < 6; ++k) D.sinv[k] = 1.0 / cd[k];
+        const double c1 = d.C_1[p], c2 = d.C_2[p], c3 = d.C_3[p];
+        D.c1Gb = c1 * P.G * P.b;
+        D.kTf = kT / MY_J_IN_EV;
+        D.c2 = c2; D.c3 = c3; D.b = P.b;
+        D.tau0 = c->opt.tau0_mode == 1 ? c->opt.tau0_value : P.tau0xf;
+        D.rhom_k = 2.0 * kT / c1 / c2 / c3 / P.G / (std::pow(P.b, 3.0));
+        D.expmq = std::exp(-1.0 * (d.Q_slip[p] * MY_J_IN_EV / kT));
+        D.G = P.G;
+        D.c4 = d.C_4[p]; D.c5 = d.C_5[p]; D.c6 = d.C_6[p];
+        D.expmqb = std::exp(-1.0 * (d.Q_bulk[p] * MY_J_IN_EV / kT));
+    }
+    c->hc.dt = d.TimeStep;
+    c->hc.nu = d.ElastConst_PhaseI[3];
+    c->hc.T = d.T__K;
+    c->hc.D0 = d.D_0;
+    c->hc.elast_bc = d.ElastBC;
+    for (int i = 0; i < 3; ++i) c->hc.n[i] = c->n[i];
+
+    // microstructure (init.c:233-396)
+    const long N = c->nvox;
+    std::vector<double> tran(9 * N, 0.0), sig(6 * N, 0.0);
+    std::vector<int> phase(N, 0);
+    c->grain.assign(N, 0);
+    {
+        const std::string ms = resolve(c->workdir, d.initial_ms);
+        FILE *fp = std::fopen(ms.c_str(), "r");
+        if (!fp) return SRX_E_FILE;
+        char buf[80];
+        long nph1 = 0;
+        for (long v = 0; v < N; ++v) {
+            double t1 = 0, th = 0, om = 0;
+            int ii = 0, jj = 0, kk = 0, jgr = 0, jph = 0;
+            if (!std::fgets(buf, 80, fp)) { std::fclose(fp); return 1139; }
+            std::sscanf(buf, "%lf %lf %lf %d %d %d %d %d", &t1, &th, &om, &ii, &jj, &kk, &jgr, &jph);
+            const long idx = ((long)(ii - 1) * c->n[1] + jj - 1) * c->n[2] + kk - 1;
+            if (idx != v || jph < 1 || jph > NPH) { std::fclose(fp); return 1139; }
+            if (jph == 1) ++nph1;
+            c->grain[v] = jgr;
+            phase[v] = jph;
+            if (!c->ph[jph - 1].xtal) continue;
+            t1 *= PI_REF / 180.; th *= PI_REF / 180.; om *= PI_REF / 180.;
+            double s2x[9];
+            euler_to_matrix(t1, th, om, s2x);
+            for (int a = 0; a < 3; ++a)
+                for (int b = 0; b < 3; ++b) tran[9 * v + 3 * a + b] = s2x[3 * b + a];
+        }
+        std::fclose(fp);
+        c->wgt_ph1 = (1.0 * (double)nph1) / N;
+    }
+    // C066 = <C_gr> over the grid; C_gr = Q diag Q^T in the B basis.  Chunked sums
+    // keep the result independent of the thread count.
+    {
+        const long CH = 4096;
+        const long nch = (N + CH - 1) / CH;
+        std::vector<double> part(36 * nch, 0.0);
+#pragma omp parallel for schedule(static)
+        for (long ch = 0; ch < nch; ++ch) {
+            double acc[36] = {0};
+            for (long v = ch * CH; v < std::min(N, (ch + 1) * CH); ++v) {
+                const int jph = phase[v];
+                if (!c->ph[jph - 1].xtal) continue;
+                double Q[36];
+                q6_from(&tran[9 * v], Q);
+                const double *sinv = c->hc.ph[jph - 1].sinv;
+                for (int i = 0; i < 6; ++i)
+                    for (int j = 0; j < 6; ++j) {
+                        double s = 0.0;
+                        for (int k = 0; k < 6; ++k) s += Q[6 * i + k] * (1.0 / sinv[k]) * Q[6 * j + k];
+                        acc[6 * i + j] += s;
+                    }
+            }
+            std::memcpy(&part[36 * ch], acc, sizeof(acc));
+        }
+        double C[36] = {0};
+        for (long ch = 0; ch < nch; ++ch)
+            for (int k = 0; k < 36; ++k) C[k] += part[36 * ch + k];
+        for (int k = 0; k < 36; ++k) c->C066[k] = C[k] * c->wgt;
+        // exact block structure: the hydrostatic axis of a cubic crystal is isotropic
+        for (int k = 0; k < 5; ++k) c->C066[6 * k + 5] = c->C066[30 + k] = 0.0;
+        std::memcpy(c->S066, c->C066, sizeof(c->C066));
+        if (!invert(c->S066, 6)) return 35;
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) c->hc.S066[i][j] = c->S066[6 * i + j];
+        c66_to_c4(c->C066, c->hc.C0);
+        c66_to_c4(c->S066, c->hc.S0);
+    }
+
+    // boundary conditions (init.c:865-901)
+    {
+        if (d.VelGrad_BC_Flag[0] + d.VelGrad_BC_Flag[1] + d.VelGrad_BC_Flag[2] == 2) return 116;
+        double vg[6];
+        for (int i = 0; i < 6; ++i) vg[i] = d.VelGrad_BC[i] * d.VelGrad_BC_Flag[i];
+        voigt_to_full(vg, c->Udot);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                c->Udot_s[3 * i + j] = (c->Udot[3 * i + j] + c->Udot[3 * j + i]) / 2.;
+                c->Udot_a[3 * i + j] = (c->Udot[3 * i + j] - c->Udot[3 * j + i]) / 2.;
+            }
+        double d6[6], dev[9];
+        t9_to_b6(c->Udot_s, d6);
+        b6_to_t9(d6, dev, 5);
+        c->d_vm = 0.0;
+        for (int k = 0; k < 9; ++k) c->d_vm += std::pow(dev[k], 2.0);
+        c->d_vm = std::sqrt(2. / 3. * c->d_vm);
+        for (int i = 0; i < 6; ++i)
+            if ((d.Stress_BC_Flag[i] * d.VelGrad_BC_Flag[i] != 0) || (d.Stress_BC_Flag[i] + d.VelGrad_BC_Flag[i] != 1))
+                return 114;
+        c->creep = 0;
+        for (int i = 0; i < 6; ++i) c->creep += d.Stress_BC_Flag[i];
+        c->creep /= 6;
+        voigt_to_full(d.Stress_BC, c->Scauchy);
+        for (int k = 0; k < 9; ++k) c->hc.Udot_a[k] = c->Udot_a[k];
+    }
+    // InitFields (init.c:444-516): Sig = C_gr : (Udot dt)
+    for (int k = 0; k < 9; ++k) {
+        c->DisGradAvg[k] = c->Udot[k] * d.TimeStep;
+        c->dDisGradAvg[k] = c->acum[k] = c->DisGradAvg_t[k] = c->uniform[k] = 0.0;
+    }
+    {
+        double e9[9], e6[6], sc6[6];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) e9[3 * i + j] = (c->DisGradAvg[3 * i + j] + c->DisGradAvg[3 * j + i]) / 2.0;
+        t9_to_b6(e9, e6);
+        t9_to_b6(c->Scauchy, sc6);
+#pragma omp parallel for schedule(static)
+        for (long v = 0; v < N; ++v) {
+            const int jph = phase[v];
+            if (!c->ph[jph - 1].xtal) continue;
+            double Q[36], t[6], s6[6];
+            q6_from(&tran[9 * v], Q);
+            const double *sinv = c->hc.ph[jph - 1].sinv;
+            for (int k = 0; k < 6; ++k) {
+                double a = 0.0;
+                for (int i = 0; i < 6; ++i) a += Q[6 * i + k] * e6[i];
+                t[k] = a / sinv[k];
+            }
+            for (int i = 0; i < 6; ++i) {
+                double a = 0.0;
+                for (int k = 0; k < 6; ++k) a += Q[6 * i + k] * t[k];
+                s6[i] = a + (c->creep == 1 ? sc6[i] : 0.0);
+            }
+            for (int i = 0; i < 6; ++i) sig[i * N + v] = s6[i];
+        }
+        for (long v = 0; v < N; ++v)
+            for (int i = 0; i < 6; ++i) c->sig_sum6[i] += sig[i * N + v];
+    }
+    c->e_vm = c->d_vm * d.TimeStep;
+    c->err = c->opt.err_override > 0 ? c->opt.err_override : d.Err;
+    c->itermax = c->opt.itermax_override > 0 ? c->opt.itermax_override : d.IterMax;
+    c->nsteps = c->opt.nsteps_override > 0 ? c->opt.nsteps_override : d.N_steps;
+
+    // device allocation
+    HIPCK(hipStreamCreate(&c->stream));
+    HIPCK(hipMalloc(&c->dc, sizeof(DevConst)));
+    HIPCK(hipMemcpy(c->dc, &c->hc, sizeof(DevConst), hipMemcpyHostToDevice));
+    auto dalloc = [&](double **p, int comps, double init) -> int {
+        HIPCK(hipMalloc(p, sizeof(double) * comps * N));
+        if (init == 0.0) {
+            HIPCK(hipMemset(*p, 0, sizeof(double) * comps * N));
+        } else {
+            std::vector<double> h((size_t)comps * N, init);
+            HIPCK(hipMemcpy(*p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
+        }
+        return 0;
+    };
+    Fields &F = c->f;
+    int rc = 0;
+    if ((rc = dalloc(&F.sig, 6, 0.0)) || (rc = dalloc(&F.sig_pre, 5, 0.0)) || (rc = dalloc(&F.eps, 5, 0.0)) ||
+        (rc = dalloc(&F.edot, 5, 0.0)) || (rc = dalloc(&F.dg, 9, 0.0)) || (rc = dalloc(&F.dg0, 9, 0.0)) ||
+        (rc = dalloc(&F.tran, 9, 0.0)) || (rc = dalloc(&F.tran0, 9, 0.0)) ||
+        (rc = dalloc(&F.rho_s, 12, d.rho_SSD_initial)) || (rc = dalloc(&F.rho_m, 12, 0.0)) ||
+        (rc = dalloc(&F.rho_m_old, 12, 0.0)) || (rc = dalloc(&F.rho_F, 12, 0.0)) || (rc = dalloc(&F.rho_P, 12, 0.0)) ||
+        (rc = dalloc(&F.gamdot, 12, 0.0)) || (rc = dalloc(&F.gam_acum, 12, 0.0)) ||
+        (rc = dalloc(&F.rho_dipole, 12, 0.0)) || (rc = dalloc(&F.rho_forest, 12, 0.0)) ||
+        (rc = dalloc(&F.rho_inplane, 12, 0.0)))
+        return rc;
+    HIPCK(hipMalloc(&F.phase, sizeof(int) * N));
+    HIPCK(hipMalloc(&F.nr_it, sizeof(int) * N));
+    HIPCK(hipMemset(F.nr_it, 0, sizeof(int) * N));
+    HIPCK(hipMemcpy(F.phase, phase.data(), sizeof(int) * N, hipMemcpyHostToDevice));
+    HIPCK(hipMemcpy(F.sig, sig.data(), sizeof(double) * 6 * N, hipMemcpyHostToDevice));
+    {
+        std::vector<double> soa(9 * N);
+        for (long v = 0; v < N; ++v)
+            for (int k = 0; k < 9; ++k) soa[k * N + v] = tran[9 * v + k];
+        HIPCK(hipMemcpy(F.tran, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+        HIPCK(hipMemcpy(F.tran0, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+    }
+    const size_t H = (size_t)(c->n[2] / 2 + 1);
+    const size_t spec_elems = 9 * H * c->n[0] * c->n[1];
+    const size_t csz = c->opt.precision == SRX_PREC_MIXED ? 2 * sizeof(float) : 2 * sizeof(double);
+    HIPCK(hipMalloc(&c->specB, spec_elems * csz));
+    HIPCK(hipMalloc(&c->specC, spec_elems * csz));
+    for (int ax = 0; ax < 3; ++ax) {
+        int r = c->opt.precision == SRX_PREC_MIXED ? upload_twiddles<float>(c->n[ax], &c->tw[ax])
+                                                   : upload_twiddles<double>(c->n[ax], &c->tw[ax]);
+        if (r) return r;
+    }
+    c->nr_blocks = nr_block_count((int)N);
+    c->end_blocks = end_block_count((int)N);
+    const int pb = std::max(c->nr_blocks * NR_NPART, c->end_blocks * END_NPART);
+    HIPCK(hipMalloc(&c->partials, sizeof(double) * pb));
+    HIPCK(hipMalloc(&c->red, sizeof(double) * 64));
+    HIPCK(hipHostMalloc((void **)&c->hred, sizeof(double) * 64, hipHostMallocDefault));
+    return 0;
+}
+
+void open_files(srx_ctx *c) {
+    if (!c->opt.write_files) return;
+    auto op = [&](const char *name) { return std::fopen(resolve(c->workdir, name).c_str(), "w+"); };
+    c->fvm = op("vm.out");
+    c->fdd = op("dd.out");
+    c->fdrx = op("drx.out");
+    c->ferr = op("err.out");
+    // headers of InitFields (init.c:497-505)
+    if (c->fvm) std::fprintf(c->fvm, "TIME\tEVM\tEVMP\tDVM\tDVMP\tSVM\tSVMSVM_1\n");
+    if (c->fdd) std::fprintf(c->fdd, "TIME\tEVM\tDD_F\tDD_P\tDD_s\tDD_m\tDD_g\n");
+    if (c->fdrx) std::fprintf(c->fdrx, "TIME\tEVM\tNEW_IDs\n");
+    if (c->ferr) std::fprintf(c->ferr, "IT\tERRE\tERRS\tSVM\n");
+}
+
+// get_smacro (evolution.c:590-664) from the reduced sums
+void get_smacro(srx_ctx *c, const double sum6[6], const double sum6_ph1[6]) {
+    static const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
+    double t[9], t1[9];
+    b6_to_t9(sum6, t);
+    b6_to_t9(sum6_ph1, t1);
+    for (int k = 0; k < 9; ++k) {
+        c->SigAvg[k] = t[k] * c->wgt;
+        c->SigAvg1[k] = t1[k] * (c->wgt / c->wgt_ph1);
+    }
+    for (int i = 0; i < 6; ++i) {
+        const int ii = IJV[0][i], jj = IJV[1][i];
+        double acc = 0.0;
+        if (c->dk.VelGrad_BC_Flag[i] == 0) {
+            for (int k = 0; k < 6; ++k) {
+                const int kk = IJV[0][k], ll = IJV[1][k];
+                acc += c->hc.S0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * c->dk.Stress_BC_Flag[k] *
+                       (c->Scauchy[3 * kk + ll] - c->SigAvg[3 * kk + ll]);
+            }
+        }
+        c->dDisGradAvg[3 * ii + jj] = acc;
+    }
+    for (int k = 0; k < 9; ++k) c->acum[k] += c->dDisGradAvg[k];
+    double sav6[6], sda[9];
+    t9_to_b6(c->SigAvg, sav6);
+    b6_to_t9(sav6, sda, 5);
+    const double tr = c->opt.svm_mode == 1 ? (sda[0] + sda[4] + sda[8]) : (sda[4] + sda[8] + c->opt.svm_stray);
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double x = sda[3 * i + j];
+            s += i != j ? x * x : (x - tr / 3.0) * (x - tr / 3.0);
+        }
+    c->s_vm = std::sqrt(3. / 2. * s);
+    t9_to_b6(c->SigAvg1, sav6);
+    b6_to_t9(sav6, sda, 5);
+    s = 0.0;
+    for (int k = 0; k < 9; ++k) s += sda[k] * sda[k];
+    c->s_vm1 = std::sqrt(3. / 2. * s);
+}
+
+// event-timed launch helper
+struct Timer {
+    srx_ctx *c;
+    int slot;
+    hipEvent_t a = nullptr, b = nullptr;
+    Timer(srx_ctx *cc, int s) : c(cc), slot(s) {
+        if (!c->timing) return;
+        hipEventCreate(&a);
+        hipEventCreate(&b);
+        hipEventRecord(a, c->stream);
+    }
+    ~Timer() {
+        if (!c->timing) return;
+        hipEventRecord(b, c->stream);
+        c->evpool.push_back(a);
+        c->evpool.push_back(b);
+        c->evpool.push_back((hipEvent_t)(intptr_t)slot);
+    }
+};
+
+const char *KT_NAMES[8] = {"fwd_z", "fwd_y", "green_x", "inv_y", "inv_z", "nr", "end", "reduce"};
+
+void collect_timing(srx_ctx *c) {
+    if (!c->timing) return;
+    hipStreamSynchronize(c->stream);
+    for (size_t i = 0; i + 2 < c->evpool.size() + 1 && i + 2 < c->evpool.size() + 2; i += 3) {
+        if (i + 2 >= c->evpool.size()) break;
+        float ms = 0;
+        hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]);
+        const int slot = (int)(intptr_t)c->evpool[i + 2];
+        c->kt[slot].ms += ms;
+        c->kt[slot].calls += 1;
+        hipEventDestroy(c->evpool[i]);
+        hipEventDestroy(c->evpool[i + 1]);
+    }
+    c->evpool.clear();
+}
+
+int sync_read(srx_ctx *c, int nvals) {
+    HIPCK(hipMemcpyAsync(c->hred, c->red, sizeof(double) * nvals, hipMemcpyDeviceToHost, c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+void srx_default_options(srx_options *o) {
+    std::memset(o, 0, sizeof(*o));
+}
+
+const char *srx_strerror(int code) {
+    switch (code) {
+    case 0: return "success";
+    case 4: return "chg_basis: wrong option";
+    case 7: return "slip system is not orthogonal";
+    case 28: return "only rank 0 may write the global files";
+    case 35: return "singular 6x6 matrix";
+    case 114: return "check boundary conditions on strain rate and stress";
+    case 116: return "cannot enforce only two deviatoric components (VelGrad_BC_Flag)";
+    case 230: return "initial hardening rate lower than the final one";
+    case 445: return "wrong option for Ten4thTransform";
+    case 604: return "wrong option for EulerToTransMatrix";
+    case 720: return "wrong type of dislocations to write";
+    case 1110: return "interaction strength matrix supports FCC only";
+    case 1139: return "error in reading the microstructure file (inconsistent index system)";
+    case SRX_E_FILE: return "file not found";
+    case SRX_E_GRID: return "unsupported grid size";
+    case SRX_E_HIP: return "HIP runtime error";
+    case SRX_E_SYSTEMS: return "more than 12 slip systems";
+    case SRX_E_NOMEM: return "out of memory";
+    case SRX_E_DONE: return "all increments done";
+    case SRX_E_CREEP_STOP: return "creep run stopped at e_vm > 0.3";
+    case SRX_E_SIZE: return "buffer too small";
+    case SRX_E_FIELD: return "unknown field id";
+    case SRX_E_ARG: return "invalid argument";
+    case SRX_E_NAN: return "NaN in the constitutive update";
+    case SRX_E_UPDATE_FLAG: return "Update_Flag must be 1";
+    default: return "unknown error";
+    }
+}
+
+int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_ctx **out) {
+    if (!deck || !out) return SRX_E_ARG;
+    srx_ctx *c = new srx_ctx();
+    if (opt) c->opt = *opt; else srx_default_options(&c->opt);
+    if (workdir) c->workdir = workdir;
+    else {
+        std::string d(deck);
+        const size_t p = d.find_last_of('/');
+        c->workdir = p == std::string::npos ? std::string(".") : d.substr(0, p);
+    }
+    c->timing = c->opt.reserved[0] != 0;
+    int rc = read_deck(deck, c->dk);
+    if (!rc) rc = setup(c);
+    if (rc) {
+        srx_close(c);
+        return rc;
+    }
+    open_files(c);
+    *out = c;
+    return 0;
+}
+
+int srx_dims(srx_ctx *c, int *n3) {
+    if (!c || !n3) return SRX_E_ARG;
+    for (int i = 0; i < 3; ++i) n3[i] = c->n[i];
+    return 0;
+}
+
+int srx_step(srx_ctx *c, srx_step_report *rep) {
+    if (!c) return SRX_E_ARG;
+    if (c->istep >= c->nsteps) return SRX_E_DONE;
+    if (c->creep == 1 && c->e_vm > 0.3) return SRX_E_CREEP_STOP;
+    const Deck &d = c->dk;
+    const double dt = d.TimeStep;
+    const int prec = c->opt.precision;
+    const int N = (int)c->nvox;
+    srx_step_report r;
+    std::memset(&r, 0, sizeof(r));
+    if (c->ferr && c->nsteps != 1) std::fprintf(c->ferr, "STEP = %d\n", c->istep);
+    if (c->timing) {
+        for (int k = 0; k < 8; ++k) c->kt[k] = {KT_NAMES[k], 0.0, 0};
+        c->nkt = 8;
+    }
+
+    // evolution.c:1115-1137
+    double add[9], set[9];
+    for (int k = 0; k < 9; ++k) {
+        add[k] = c->Udot[k] * dt;
+        set[k] = c->DisGradAvg[k] + c->acum[k];
+    }
+    launch_prologue(c->stream, c->f, N, add, c->creep, set);
+    if (c->creep == 1) for (int k = 0; k < 9; ++k) c->uniform[k] = set[k];
+    else for (int k = 0; k < 9; ++k) c->uniform[k] += add[k];
+    for (int k = 0; k < 9; ++k) c->dDisGradAvg[k] = c->acum[k] = 0.0;
+    c->Err_e = 2. * c->err;
+    c->Err_s = 2. * c->err;
+    const int rotated = c->istep > 2 ? 1 : 0;   // update_orient first runs at the end of istep 2
+    int iter = 0, nrmax = 0;
+    long nrtot = 0;
+    const double divisor = (double)c->nvox;
+    for (;;) {
+        if (c->opt.fixed_iters > 0) {
+            if (iter >= c->opt.fixed_iters) break;
+        } else if (!(iter < c->itermax && std::fmax(std::fabs(c->Err_s), std::fabs(c->Err_e)) > c->err)) {
+            break;
+        }
+        ++iter;
+        double mean6[6], dcs[6];
+        for (int k = 0; k < 6; ++k) {
+            mean6[k] = c->sig_sum6[k] / c->nvox;
+            dcs[k] = prec == SRX_PREC_MIXED ? mean6[k] * c->nvox : 0.0;
+        }
+        { Timer t(c, 0); launch_fft_fwd_z(c->stream, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], mean6); }
+        { Timer t(c, 1); launch_fft_y(c->stream, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1]); }
+        { Timer t(c, 2); launch_green_x(c->stream, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, dcs); }
+        { Timer t(c, 3); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1]); }
+        { Timer t(c, 4); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], c->dDisGradAvg, divisor); }
+        for (int k = 0; k < 9; ++k) c->uniform[k] += c->dDisGradAvg[k];
+        { Timer t(c, 5); launch_nr(c->stream, c->dc, c->f, N, iter == 1, rotated, c->partials, c->nr_blocks); }
+        { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red); }
+        int rc = sync_read(c, NR_NPART);
+        if (rc) return rc;
+        const double *h = c->hred;
+        if (h[16] > 0) return SRX_E_NAN;
+        for (int k = 0; k < 6; ++k) c->sig_sum6[k] = h[2 + k];
+        nrtot += (long)h[14];
+        nrmax = std::max(nrmax, (int)h[15]);
+        c->Err_e = h[1] * c->wgt;
+        c->Err_s = h[0] * c->wgt;
+        get_smacro(c, h + 2, h + 8);
+        if (std::fabs(c->e_vm) < 1E-5) c->Err_e = 0.0;
+        else c->Err_e /= c->e_vm;
+        c->Err_s /= c->s_vm;
+        if (c->opt.verbose) std::printf("iter %d Err_e %e Err_s %e\n", iter, c->Err_e, c->Err_s);
+    }
+    r.converged = !(std::fmax(std::fabs(c->Err_s), std::fabs(c->Err_e)) > c->err);
+    // evolution.c:1268-1290
+    double act[9], vga[9];
+    for (int k = 0; k < 9; ++k) {
+        act[k] = c->DisGradAvg[k] + c->acum[k];
+        vga[k] = (act[k] - c->DisGradAvg_t[k]) / dt;
+    }
+    c->e_vm = von_mises(act);
+    c->d_vm = von_mises(vga);
+    c->TimeTot += dt;
+    std::memcpy(c->DisGradAvg_t, act, sizeof(act));
+    std::memcpy(r.disgrad_avg_actual, act, sizeof(act));
+    std::memcpy(r.velgrad_avg, vga, sizeof(vga));
+    // Edot/Eps, update_orient (istep>1), DislocationEvolution, averages
+    const int do_orient = (d.Update_Flag == 1 && c->istep > 1) ? 1 : 0;
+    { Timer t(c, 6); launch_end(c->stream, c->dc, c->f, N, iter == 1 ? 1 : 0, rotated, do_orient, 1, c->istep, c->partials); }
+    { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->end_blocks, END_NPART, c->red); }
+    int rc = sync_read(c, END_NPART);
+    if (rc) return rc;
+    const double *h = c->hred;
+    if (h[21] > 0) return SRX_E_NAN;
+    {
+        double e6[6], d6[6], et[9], dtt[9];
+        for (int k = 0; k < 6; ++k) { e6[k] = h[k]; d6[k] = h[6 + k]; }
+        b6_to_t9(e6, et);
+        b6_to_t9(d6, dtt);
+        double evmp = 0.0, dvmp = 0.0;
+        for (int k = 0; k < 9; ++k) {
+            r.eps_avg[k] = et[k] * c->wgt;
+            r.edot_avg[k] = dtt[k] * c->wgt;
+            evmp += r.eps_avg[k] * r.eps_avg[k];
+            dvmp += r.edot_avg[k] * r.edot_avg[k];
+        }
+        r.evmp = std::sqrt(2. / 3. * evmp);
+        r.dvmp = std::sqrt(2. / 3. * dvmp);
+    }
+    const double inv = 1.0 / c->nvox;
+    r.rho_F_avg = h[12] * inv;
+    r.rho_P_avg = h[13] * inv;
+    r.rho_s_avg = h[14] * inv;
+    r.rho_m_avg = h[15] * inv;
+    r.rho_g_avg = 0.0;
+    r.rho_dipole_avg = h[16] * inv;
+    r.rho_forest_avg = h[17] * inv;
+    r.rho_inplane_avg = h[20] * inv;
+    if (c->fdd) {
+        std::fprintf(c->fdd, "%e\t%e\t%e\t%e\t%e\t%e\t%e\n", c->TimeTot, c->e_vm, r.rho_s_avg, r.rho_m_avg,
+                     r.rho_dipole_avg, r.rho_inplane_avg, r.rho_forest_avg);
+        std::fflush(c->fdd);
+    }
+    r.istep = c->istep;
+    r.iters = iter;
+    r.nr_iters_max = nrmax;
+    r.nr_iters_total = nrtot;
+    r.err_s = c->Err_s;
+    r.err_e = c->Err_e;
+    r.time_tot = c->TimeTot;
+    r.e_vm = c->e_vm;
+    r.d_vm = c->d_vm;
+    r.s_vm = c->s_vm;
+    r.s_vm1 = c->s_vm1;
+    std::memcpy(r.sig_avg, c->SigAvg, sizeof(r.sig_avg));
+    std::memcpy(r.sig_avg1, c->SigAvg1, sizeof(r.sig_avg1));
+    if (c->fvm) {
+        std::fprintf(c->fvm, "%e\t%e\t%e\t%e\t%e\t%e\t%e\n", c->TimeTot, c->e_vm, r.evmp, c->d_vm, r.dvmp, c->s_vm,
+                     c->s_vm1);
+        std::fflush(c->fvm);
+    }
+    if (c->ferr) std::fflush(c->ferr);
+    // evolution.c:1513-1515
+    for (int k = 0; k < 9; ++k) c->DisGradAvg[k] = c->DisGradAvg_t[k] + c->Udot[k] * dt;
+    collect_timing(c);
+    c->istep++;
+    if (rep) *rep = r;
+    return 0;
+}
+
+int srx_run(srx_ctx *c) {
+    if (!c) return SRX_E_ARG;
+    while (c->istep < c->nsteps) {
+        const int rc = srx_step(c, nullptr);
+        if (rc == SRX_E_CREEP_STOP) break;
+        if (rc) return rc;
+    }
+    return 0;
+}
+
+int srx_kernel_times(srx_ctx *c, const char **names, double *ms, int *calls, int *count) {
+    if (!c || !count) return SRX_E_ARG;
+    const int n = std::min(*count, c->nkt);
+    for (int i = 0; i < n; ++i) {
+        if (names) names[i] = c->kt[i].name;
+        if (ms) ms[i] = c->kt[i].ms;
+        if (calls) calls[i] = c->kt[i].calls;
+    }
+    *count = n;
+    return 0;
+}
+
+static int field_info(int id, int &nc) {
+    switch (id) {
+    case SRX_F_SIG: case SRX_F_EPS: case SRX_F_EDOT: case SRX_F_DISGRAD: case SRX_F_VELGRAD: case SRX_F_TRANMAT:
+    case SRX_F_FLUCT: nc = 9; return 0;
+    case SRX_F_RHO_S: case SRX_F_RHO_M: case SRX_F_RHO_F: case SRX_F_RHO_P: case SRX_F_GAMDOT: case SRX_F_GAM_ACUM:
+    case SRX_F_RHO_DIPOLE: case SRX_F_RHO_FOREST: case SRX_F_RHO_INPLANE: case SRX_F_TRIAL_GAM_ACUM: nc = 12; return 0;
+    case SRX_F_NR_ITERS: case SRX_F_PHASE: case SRX_F_GRAIN: nc = 1; return 0;
+    default: return SRX_E_FIELD;
+    }
+}
+
+int srx_get_field(srx_ctx *c, int id, double *out, size_t n) {
+    if (!c || !out) return SRX_E_ARG;
+    int nc = 0;
+    if (field_info(id, nc)) return SRX_E_FIELD;
+    const long N = c->nvox;
+    if (n < (size_t)nc * N) return SRX_E_SIZE;
+    HIPCK(hipStreamSynchronize(c->stream));
+    auto fetch = [&](const void *src, size_t bytes, void *dst) -> int {
+        HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        return 0;
+    };
+    std::vector<double> tmp;
+    int rc = 0;
+    if (id == SRX_F_SIG || id == SRX_F_EPS || id == SRX_F_EDOT) {
+        const int m = id == SRX_F_SIG ? 6 : 5;
+        const double *src = id == SRX_F_SIG ? c->f.sig : id == SRX_F_EPS ? c->f.eps : c->f.edot;
+        tmp.resize((size_t)m * N);
+        if ((rc = fetch(src, sizeof(double) * m * N, tmp.data()))) return rc;
+        for (long v = 0; v < N; ++v) {
+            double b6[6] = {0, 0, 0, 0, 0, 0};
+            for (int k = 0; k < m; ++k) b6[k] = tmp[k * N + v];
+            b6_to_t9(b6, out + 9 * v, 6);
+        }
+        return 0;
+    }
+    if (id == SRX_F_DISGRAD || id == SRX_F_TRANMAT || id == SRX_F_VELGRAD || id == SRX_F_FLUCT) {
+        tmp.resize((size_t)9 * N);
+        if ((rc = fetch(id == SRX_F_TRANMAT ? c->f.tran : c->f.dg, sizeof(double) * 9 * N, tmp.data()))) return rc;
+        std::vector<double> t0;
+        if (id == SRX_F_VELGRAD) {
+            t0.resize((size_t)9 * N);
+            if ((rc = fetch(c->f.dg0, sizeof(double) * 9 * N, t0.data()))) return rc;
+        }
+        for (long v = 0; v < N; ++v)
+            for (int k = 0; k < 9; ++k) {
+                double x = tmp[k * N + v];
+                if (id == SRX_F_VELGRAD) x = (x - t0[k * N + v]) / c->dk.TimeStep;
+                if (id == SRX_F_FLUCT) x -= c->uniform[k];
+                out[9 * v + k] = x;
+            }
+        return 0;
+    }
+    if (nc == 12) {
+        const double *src = id == SRX_F_RHO_S ? c->f.rho_s : id == SRX_F_RHO_M ? c->f.rho_m : id == SRX_F_RHO_F ? c->f.rho_F
+                          : id == SRX_F_RHO_P ? c->f.rho_P : id == SRX_F_GAMDOT ? c->f.gamdot
+                          : id == SRX_F_RHO_DIPOLE ? c->f.rho_dipole : id == SRX_F_RHO_FOREST ? c->f.rho_forest
+                          : id == SRX_F_RHO_INPLANE ? c->f.rho_inplane : c->f.gam_acum;
+        tmp.resize((size_t)12 * N);
+        if ((rc = fetch(src, sizeof(double) * 12 * N, tmp.data()))) return rc;
+        for (long v = 0; v < N; ++v)
+            for (int k = 0; k < 12; ++k) out[12 * v + k] = tmp[k * N + v];
+        return 0;
+    }
+    if (id == SRX_F_NR_ITERS || id == SRX_F_PHASE) {
+        std::vector<int> t(N);
+        if ((rc = fetch(id == SRX_F_NR_ITERS ? c->f.nr_it : c->f.phase, sizeof(int) * N, t.data()))) return rc;
+        for (long v = 0; v < N; ++v) out[v] = t[v];
+        return 0;
+    }
+    if (id == SRX_F_GRAIN) {
+        for (long v = 0; v < N; ++v) out[v] = c->grain[v];
+        return 0;
+    }
+    return SRX_E_FIELD;
+}
+
+int srx_set_field(srx_ctx *c, int id, const double *in, size_t n) {
+    if (!c || !in) return SRX_E_ARG;
+    int nc = 0;
+    if (field_info(id, nc)) return SRX_E_FIELD;
+    const long N = c->nvox;
+    if (n < (size_t)nc * N) return SRX_E_SIZE;
+    HIPCK(hipStreamSynchronize(c->stream));
+    std::vector<double> tmp;
+    if (id == SRX_F_SIG || id == SRX_F_EPS) {
+        const int m = id == SRX_F_SIG ? 6 : 5;
+        tmp.resize((size_t)m * N);
+        for (long v = 0; v < N; ++v) {
+            double b6[6];
+            t9_to_b6(in + 9 * v, b6);
+            for (int k = 0; k < m; ++k) tmp[k * N + v] = b6[k];
+        }
+        HIPCK(hipMemcpy(id == SRX_F_SIG ? c->f.sig : c->f.eps, tmp.data(), sizeof(double) * m * N, hipMemcpyHostToDevice));
+        if (id == SRX_F_SIG) {
+            for (int k = 0; k < 6; ++k) {
+                double s = 0;
+                for (long v = 0; v < N; ++v) s += tmp[k * N + v];
+                c->sig_sum6[k] = s;
+            }
+        }
+        return 0;
+    }
+    if (id == SRX_F_DISGRAD || id == SRX_F_TRANMAT) {
+        tmp.resize((size_t)9 * N);
+        for (long v = 0; v < N; ++v)
+            for (int k = 0; k < 9; ++k) tmp[k * N + v] = in[9 * v + k];
+        HIPCK(hipMemcpy(id == SRX_F_DISGRAD ? c->f.dg : c->f.tran, tmp.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+        return 0;
+    }
+    if (id == SRX_F_RHO_S || id == SRX_F_RHO_M || id == SRX_F_GAM_ACUM) {
+        tmp.resize((size_t)12 * N);
+        for (long v = 0; v < N; ++v)
+            for (int k = 0; k < 12; ++k) tmp[k * N + v] = in[12 * v + k];
+        double *dst = id == SRX_F_RHO_S ? c->f.rho_s : id == SRX_F_RHO_M ? c->f.rho_m : c->f.gam_acum;
+        HIPCK(hipMemcpy(dst, tmp.data(), sizeof(double) * 12 * N, hipMemcpyHostToDevice));
+        return 0;
+    }
+    return SRX_E_FIELD;
+}
+
+void srx_close(srx_ctx *c) {
+    if (!c) return;
+    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->fvm) std::fclose(c->fvm);
+    if (c->fdd) std::fclose(c->fdd);
+    if (c->fdrx) std::fclose(c->fdrx);
+    if (c->ferr) std::fclose(c->ferr);
+    double *ptrs[] = {c->f.sig, c->f.sig_pre, c->f.eps, c->f.edot, c->f.dg, c->f.dg0, c->f.tran, c->f.tran0,
+                      c->f.rho_s, c->f.rho_m, c->f.rho_m_old, c->f.rho_F, c->f.rho_P, c->f.gamdot, c->f.gam_acum,
+                      c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->partials, c->red};
+    for (double *p : ptrs)
+        if (p) hipFree(p);
+    if (c->f.phase) hipFree(c->f.phase);
+    if (c->f.nr_it) hipFree(c->f.nr_it);
+    if (c->specB) hipFree(c->specB);
+    if (c->specC) hipFree(c->specC);
+    for (int i = 0; i < 3; ++i)
+        if (c->tw[i]) hipFree(c->tw[i]);
+    if (c->dc) hipFree(c->dc);
+    if (c->hred) hipHostFree(c->hred);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+// ---- single-kernel entry points -------------------------------------------
+static int fft_test_common(int ncomp, int n0, int n1, int n2, int prec, FftPlan pl[3], void *tw[3], void **sB,
+                           void **sC, size_t &specn) {
+    const int n[3] = {n0, n1, n2};
+    for (int i = 0; i < 3; ++i)
+        if (!make_plan(n[i], pl[i])) return SRX_E_GRID;
+    if (!grid_supported(pl, n) || ncomp < 1 || ncomp > 9) return SRX_E_GRID;
+    for (int ax = 0; ax < 3; ++ax) {
+        int r = prec == SRX_PREC_MIXED ? upload_twiddles<float>(n[ax], &tw[ax]) : upload_twiddles<double>(n[ax], &tw[ax]);
+        if (r) return r;
+    }
+    const size_t H = n2 / 2 + 1;
+    specn = (size_t)ncomp * H * n0 * n1;
+    const size_t csz = prec == SRX_PREC_MIXED ? 8 : 16;
+    HIPCK(hipMalloc(sB, specn * csz));
+    HIPCK(hipMalloc(sC, specn * csz));
+    return 0;
+}
+
+int srx_fft_forward(const double *in, int ncomp, int n0, int n1, int n2, int prec, double *out) {
+    FftPlan pl[3];
+    void *tw[3] = {nullptr, nullptr, nullptr}, *sB = nullptr, *sC = nullptr;
+    size_t specn = 0;
+    int rc = fft_test_common(ncomp, n0, n1, n2, prec, pl, tw, &sB, &sC, specn);
+    const size_t N = (size_t)n0 * n1 * n2;
+    double *din = nullptr;
+    const int n[3] = {n0, n1, n2};
+    if (!rc) {
+        HIPCK(hipMalloc(&din, sizeof(double) * ncomp * N));
+        HIPCK(hipMemcpy(din, in, sizeof(double) * ncomp * N, hipMemcpyHostToDevice));
+        launch_fft_real_forward_test(nullptr, prec, din, ncomp, sB, sC, n, pl, tw);
+        HIPCK(hipDeviceSynchronize());
+        // layout C [c][kz][y][x] -> out [c][x][y][kz]
+        const size_t H = n2 / 2 + 1;
+        const size_t csz = prec == SRX_PREC_MIXED ? 8 : 16;
+        std::vector<char> h(specn * csz);
+        HIPCK(hipMemcpy(h.data(), sC, specn * csz, hipMemcpyDeviceToHost));
+        for (int c = 0; c < ncomp; ++c)
+            for (size_t kz = 0; kz < H; ++kz)
+                for (int y = 0; y < n1; ++y)
+                    for (int x = 0; x < n0; ++x) {
+                        const size_t src = ((c * H + kz) * n1 + y) * n0 + x;
+                        const size_t dst = (((size_t)c * n0 + x) * n1 + y) * H + kz;
+                        if (prec == SRX_PREC_MIXED) {
+                            const float *f = reinterpret_cast<const float *>(h.data()) + 2 * src;
+                            out[2 * dst] = f[0];
+                            out[2 * dst + 1] = f[1];
+                        } else {
+                            const double *f = reinterpret_cast<const double *>(h.data()) + 2 * src;
+                            out[2 * dst] = f[0];
+                            out[2 * dst + 1] = f[1];
+                        }
+                    }
+    }
+    if (din) hipFree(din);
+    if (sB) hipFree(sB);
+    if (sC) hipFree(sC);
+    for (int i = 0; i < 3; ++i)
+        if (tw[i]) hipFree(tw[i]);
+    return rc;
+}
+
+int srx_fft_inverse(const double *in, int ncomp, int n0, int n1, int n2, int prec, double scale, double *out) {
+    FftPlan pl[3];
+    void *tw[3] = {nullptr, nullptr, nullptr}, *sB = nullptr, *sC = nullptr;
+    size_t specn = 0;
+    int rc = fft_test_common(ncomp, n0, n1, n2, prec, pl, tw, &sB, &sC, specn);
+    const size_t N = (size_t)n0 * n1 * n2;
+    double *dout = nullptr;
+    const int n[3] = {n0, n1, n2};
+    if (!rc) {
+        const size_t H = n2 / 2 + 1;
+        const size_t csz = prec == SRX_PREC_MIXED ? 8 : 16;
+        std::vector<char> h(specn * csz);
+        for (int c = 0; c < ncomp; ++c)
+            for (size_t kz = 0; kz < H; ++kz)
+                for (int y = 0; y < n1; ++y)
+                    for (int x = 0; x < n0; ++x) {
+                        const size_t dst = ((c * H + kz) * n1 + y) * n0 + x;
+                        const size_t src = (((size_t)c * n0 + x) * n1 + y) * H + kz;
+                        if (prec == SRX_PREC_MIXED) {
+                            float *f = reinterpret_cast<float *>(h.data()) + 2 * dst;
+                            f[0] = (float)in[2 * src];
+                            f[1] = (float)in[2 * src + 1];
+                        } else {
+                            double *f = reinterpret_cast<double *>(h.data()) + 2 * dst;
+                            f[0] = in[2 * src];
+                            f[1] = in[2 * src + 1];
+                        }
+                    }
+        HIPCK(hipMemcpy(sC, h.data(), specn * csz, hipMemcpyHostToDevice));
+        HIPCK(hipMalloc(&dout, sizeof(double) * ncomp * N));
+        launch_fft_real_inverse_test(nullptr, prec, sC, sB, ncomp, dout, n, pl, tw, 1.0 / scale);
+        HIPCK(hipDeviceSynchronize());
+        HIPCK(hipMemcpy(out, dout, sizeof(double) * ncomp * N, hipMemcpyDeviceToHost));
+    }
+    if (dout) hipFree(dout);
+    if (sB) hipFree(sB);
+    if (sC) hipFree(sC);
+    for (int i = 0; i < 3; ++i)
+        if (tw[i]) hipFree(tw[i]);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,1374 @@
+// kernels.hip -- CDNA4 (gfx950) kernels of the EVP-FFT increment loop.
+//
+// FFT passes (evolution.c:1157-1201 replaced): LDS-resident Stockham
+// radix-2/3/4/5 transforms over batches of pencils, 256 threads per block.
+//   fwd_z   real [c][x][y][z]      -> half spectrum B [c][kz][x][y]   (two real comps per complex FFT)
+//   fft_y   B <-> C [c][kz][y][x]  (the y transform doubles as the pencil transpose)
+//   green_x C(6 stress comps) -> forward x -> Green operator -> inverse x -> C(9 comps), in place
+//   inv_z   B(9 comps) -> c2r along z, fused with the FP64 DisGrad update (Eq. 15)
+// Per-voxel kernels (one voxel per lane, FP64):
+//   nr      the constitutive Newton solve of UpdateStress (evolution.c:157-427)
+//   end     Edot/Eps update, update_orient, DislocationEvolution (evolution.c:1268-1362)
+// Block partial sums are reduced by reduce_partials in a fixed order, so every
+// sum is bitwise repeatable from run to run.
+#include "srx_dev.hpp"
+
+#include <hip/hip_runtime.h>
+#include <cmath>
+#include <cstdio>
+
+namespace srx {
+
+// ---------------------------------------------------------------------------
+// complex helpers
+// ---------------------------------------------------------------------------
+template <typename T>
+struct cx {
+    T x, y;
+};
+template <typename T>
+__device__ __forceinline__ cx<T> operator+(cx<T> a, cx<T> b) { return {a.x + b.x, a.y + b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> operator-(cx<T> a, cx<T> b) { return {a.x - b.x, a.y - b.y}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cmul(cx<T> a, cx<T> b) { return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x}; }
+template <typename T>
+__device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s}; }
+// multiply by -i*sign ... helpers: i*a = (-a.y, a.x)
+template <typename T>
+__device__ __forceinline__ cx<T> mul_i(cx<T> a) { return {-a.y, a.x}; }
+
+// ---------------------------------------------------------------------------
+// radix butterflies; sign = -1 forward (e^{-2 pi i/R}), +1 inverse
+// ---------------------------------------------------------------------------
+template <typename T, int R>
+struct Dft;
+
+template <typename T>
+struct Dft<T, 2> {
+    __device__ __forceinline__ static void run(cx<T> *v, int) {
+        cx<T> a = v[0], b = v[1];
+        v[0] = a + b;
+        v[1] = a - b;
+    }
+};
+template <typename T>
+struct Dft<T, 3> {
+    __device__ __forceinline__ static void run(cx<T> *v, int sign) {
+        const T c = T(-0.5), s = T(0.86602540378443864676) * T(sign);
+        cx<T> t1 = v[1] + v[2], t2 = v[1] - v[2];
+        cx<T> a = v[0];
+        cx<T> m = {a.x + c * t1.x, a.y + c * t1.y};
+        cx<T> r = {-s * t2.y, s * t2.x};   // i*s*t2
+        v[0] = a + t1;
+        v[1] = m + r;
+        v[2] = m - r;
+    }
+};
+template <typename T>
+struct Dft<T, 4> {
+    __device__ __forceinline__ static void run(cx<T> *v, int sign) {
+        cx<T> t0 = v[0] + v[2], t1 = v[0] - v[2], t2 = v[1] + v[3], d = v[1] - v[3];
+        // t3 = sign*i*(b-d): forward -> -i(b-d)
+        cx<T> t3 = sign < 0 ? cx<T>{d.y, -d.x} : cx<T>{-d.y, d.x};
+        v[0] = t0 + t2;
+        v[2] = t0 - t2;
+        v[1] = t1 + t3;
+        v[3] = t1 - t3;
+    }
+};
+template <typename T>
+struct Dft<T, 5> {
+    __device__ __forceinline__ static void run(cx<T> *v, int sign) {
+        const T c1 = T(0.30901699437494742410), c2 = T(-0.80901699437494742410);
+        const T s1 = T(0.95105651629515357212) * T(sign), s2 = T(0.58778525229247312917) * T(sign);
+        cx<T> a = v[0];
+        cx<T> p1 = v[1] + v[4], m1 = v[1] - v[4], p2 = v[2] + v[3], m2 = v[2] - v[3];
+        cx<T> r1 = {a.x + c1 * p1.x + c2 * p2.x, a.y + c1 * p1.y + c2 * p2.y};
+        cx<T> r2 = {a.x + c2 * p1.x + c1 * p2.x, a.y + c2 * p1.y + c1 * p2.y};
+        // i*(s1*m1 + s2*m2) and i*(s2*m1 - s1*m2)
+        cx<T> q1 = {s1 * m1.x + s2 * m2.x, s1 * m1.y + s2 * m2.y};
+        cx<T> q2 = {s2 * m1.x - s1 * m2.x, s2 * m1.y - s1 * m2.y};
+        v[0] = a + p1 + p2;
+        v[1] = r1 + mul_i(q1);
+        v[4] = r1 - mul_i(q1);
+        v[2] = r2 + mul_i(q2);
+        v[3] = r2 - mul_i(q2);
+    }
+};
+
+constexpr int FFT_BLOCK = 256;
+
+// One Stockham stage (radix R, sub-transform span L) over `npen` pencils of
+// length N stored pencil-major in LDS.  Every butterfly of the stage is held
+// in registers between the load and the store, so the stage runs in place.
+template <typename T, int R, int MAXB>
+__device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int L, const cx<T> *__restrict__ tw,
+                                               int sign) {
+    const int nb = N / R;
+    const int total = npen * nb;
+    const int tstep = N / (L * R);
+    cx<T> v[MAXB][R];
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        const int t = threadIdx.x + m * FFT_BLOCK;
+        if (t < total) {
+            const int p = t / nb, j = t - p * nb;
+            const cx<T> *src = buf + p * N + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        const int t = threadIdx.x + m * FFT_BLOCK;
+        if (t < total) {
+            const int p = t / nb, j = t - p * nb;
+            const int k = j % L;
+            if (k) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cx<T> w = tw[r * k * tstep];
+                    if (sign > 0) w.y = -w.y;
+                    v[m][r] = cmul(v[m][r], w);
+                }
+            }
+            Dft<T, R>::run(v[m], sign);
+            cx<T> *dst = buf + p * N + (j / L) * L * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
+        }
+    }
+    __syncthreads();
+}
+
+// Full 1-D transform of npen LDS-resident pencils.  Caller guarantees
+// npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.
+template <typename T, int MAXB>
+__device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign) {
+    int L = 1;
+    for (int st = 0; st < pl.nst; ++st) {
+        const int R = pl.rad[st];
+        switch (R) {
+        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign); break;
+        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign); break;
+        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign); break;
+        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign); break;
+        }
+        L *= R;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// forward z: real comps -> half spectrum, layout B [c][kz][x][y]
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(FFT_BLOCK) k_fwd_z(const double *__restrict__ in, cx<T> *__restrict__ specB,
+                                                     int ncomp, int n0, int n1, int n2, FftPlan pz,
+                                                     const cx<T> *__restrict__ tw, int TY, double m0, double m1,
+                                                     double m2, double m3, double m4, double m5) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int x = blockIdx.x;
+    const int y0 = blockIdx.y * TY;
+    const int ty = min(TY, n1 - y0);
+    const int H = n2 / 2 + 1;
+    const size_t N = (size_t)n0 * n1 * n2;
+    const size_t HS = (size_t)H * n0 * n1;
+    const double mean[6] = {m0, m1, m2, m3, m4, m5};
+    for (int ca = 0; ca < ncomp; ca += 2) {
+        const int cb = ca + 1;
+        const bool two = cb < ncomp;
+        const double ma = ca < 6 ? mean[ca] : 0.0, mb = (two && cb < 6) ? mean[cb] : 0.0;
+        for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
+            const int yy = i / n2, z = i - yy * n2;
+            const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
+            const double va = in[ca * N + g] - ma;
+            const double vb = two ? in[cb * N + g] - mb : 0.0;
+            buf[yy * n2 + z] = cx<T>{T(va), T(vb)};
+        }
+        __syncthreads();
+        lds_fft<T, 4>(buf, ty, pz, tw, -1);
+        for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
+            const int kz = i / ty, yy = i - kz * ty;
+            const cx<T> z1 = buf[yy * n2 + kz];
+            const cx<T> z2 = buf[yy * n2 + (kz == 0 ? 0 : n2 - kz)];
+            const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
+            specB[ca * HS + o] = cx<T>{T(0.5) * (z1.x + z2.x), T(0.5) * (z1.y - z2.y)};
+            if (two) specB[cb * HS + o] = cx<T>{T(0.5) * (z1.y + z2.y), T(-0.5) * (z1.x - z2.x)};
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// y transform with transpose.  b_to_c: B [c][kz][x][y] -> C [c][kz][y][x]
+// (forward, sign -1); otherwise C -> B (inverse, sign +1).
+// grid: (ceil(n0/TX), H, ncomp)
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
+                                                     int n0, int n1, int n2, int b_to_c, FftPlan py,
+                                                     const cx<T> *__restrict__ tw, int TX) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int H = n2 / 2 + 1;
+    const int x0 = blockIdx.x * TX;
+    const int tx = min(TX, n0 - x0);
+    const int kz = blockIdx.y;
+    const int c = blockIdx.z;
+    const size_t plane = (size_t)n0 * n1;
+    const size_t base = ((size_t)c * H + kz) * plane;
+    if (b_to_c) {
+        for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
+            const int xx = i / n1, y = i - xx * n1;
+            buf[xx * n1 + y] = in[base + (size_t)(x0 + xx) * n1 + y];
+        }
+    } else {
+        for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
+            const int y = i / tx, xx = i - y * tx;
+            buf[xx * n1 + y] = in[base + (size_t)y * n0 + x0 + xx];
+        }
+    }
+    __syncthreads();
+    lds_fft<T, 4>(buf, tx, py, tw, b_to_c ? -1 : +1);
+    if (b_to_c) {
+        for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
+            const int y = i / tx, xx = i - y * tx;
+            out[base + (size_t)y * n0 + x0 + xx] = buf[xx * n1 + y];
+        }
+    } else {
+        for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
+            const int xx = i / n1, y = i - xx * n1;
+            out[base + (size_t)(x0 + xx) * n1 + y] = buf[xx * n1 + y];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Green operator at one k point (init.c:542-592 with evolution.c:1169-1183):
+//   Omega = (C0_ijkl n_j n_l)^-1,  du_ij = -Omega_ik (sig_kl n_l) n_j
+// sig: 6 B-basis comps (complex); du: 9 comps (row-major tensor).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void unit_k(int idx, int n, double &t) {
+    double v = (double)idx * (double)(1.0 / n);
+    v -= round(v);
+    t = v * 6.28318530718;
+}
+
+__device__ void green_w(const double *__restrict__ C0, const double nv[3], const double sr[9], const double si[9],
+                        double wr[3], double wi[3]) {
+    double A[3][3];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            double acc = 0.0;
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int l = 0; l < 3; ++l) acc += C0[((i * 3 + j) * 3 + k) * 3 + l] * nv[j] * nv[l];
+            A[i][k] = acc;
+        }
+    // M3Inverse (init.c:518-540), cofactors over the determinant
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) -
+                       A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0]) +
+                       A[0][2] * (A[1][0] * A[2][1] - A[2][0] * A[1][1]);
+    double om[3][3];
+    om[0][0] = A[1][1] * A[2][2] - A[1][2] * A[2][1];
+    om[0][1] = A[0][2] * A[2][1] - A[0][1] * A[2][2];
+    om[0][2] = A[0][1] * A[1][2] - A[0][2] * A[1][1];
+    om[1][0] = A[1][2] * A[2][0] - A[1][0] * A[2][2];
+    om[1][1] = A[0][0] * A[2][2] - A[0][2] * A[2][0];
+    om[1][2] = A[0][2] * A[1][0] - A[0][0] * A[1][2];
+    om[2][0] = A[1][0] * A[2][1] - A[1][1] * A[2][0];
+    om[2][1] = A[0][1] * A[2][0] - A[0][0] * A[2][1];
+    om[2][2] = A[0][0] * A[1][1] - A[0][1] * A[1][0];
+    const double rdet = 1.0 / det;
+    double tr[3], ti[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        tr[k] = sr[3 * k] * nv[0] + sr[3 * k + 1] * nv[1] + sr[3 * k + 2] * nv[2];
+        ti[k] = si[3 * k] * nv[0] + si[3 * k + 1] * nv[1] + si[3 * k + 2] * nv[2];
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        wr[i] = (om[i][0] * tr[0] + om[i][1] * tr[1] + om[i][2] * tr[2]) * rdet;
+        wi[i] = (om[i][0] * ti[0] + om[i][1] * ti[1] + om[i][2] * ti[2]) * rdet;
+    }
+}
+
+// B-basis 6-vector -> symmetric tensor (chg_basis opt 1, kinematics.c:55-60)
+#define SRX_R2 0.70710678118654744
+#define SRX_R3 0.57735026918962584
+#define SRX_R6 0.40824829046386304
+
+template <typename V>
+__device__ __forceinline__ void b6_to_t9(const V v[6], V t[9]) {
+    t[0] = v[0] * V(-SRX_R2) + v[1] * V(-SRX_R6) + v[5] * V(SRX_R3);
+    t[4] = v[0] * V(SRX_R2) + v[1] * V(-SRX_R6) + v[5] * V(SRX_R3);
+    t[8] = v[1] * V(2. * SRX_R6) + v[5] * V(SRX_R3);
+    t[5] = t[7] = v[2] * V(SRX_R2);
+    t[2] = t[6] = v[3] * V(SRX_R2);
+    t[1] = t[3] = v[4] * V(SRX_R2);
+}
+
+// x pass with the Green operator.  grid: (ceil(n1/TY), H).  LDS: 9 slots of TY*n0.
+// mode 1: forward x of 6 comps, Green, inverse x of 9 comps (the solver pass)
+// mode 0: forward x only (ncomp comps, testing)   mode 2: inverse x only (testing)
+template <typename T, int MAXB>
+__global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec, int n0, int n1, int n2, FftPlan px,
+                                                       const cx<T> *__restrict__ tw, int TY, int mode, int ncomp,
+                                                       const DevConst *__restrict__ cst, double dc0, double dc1,
+                                                       double dc2, double dc3, double dc4, double dc5) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int H = n2 / 2 + 1;
+    const int y0 = blockIdx.x * TY;
+    const int ty = min(TY, n1 - y0);
+    const int kz = blockIdx.y;
+    const size_t plane = (size_t)n0 * n1;
+    const int slot = ty * n0;
+    const int cin = mode == 1 ? 6 : ncomp;
+    const int cout = mode == 1 ? 9 : ncomp;
+    for (int i = threadIdx.x; i < cin * slot; i += FFT_BLOCK) {
+        const int c = i / slot, r = i - c * slot, yy = r / n0, x = r - yy * n0;
+        buf[c * slot + yy * n0 + x] = spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x];
+    }
+    __syncthreads();
+    if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, tw, -1);
+    if (mode == 1) {
+        const double *C0 = cst->C0;
+        double tzv, tyv;
+        unit_k(kz, n2, tzv);
+        const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
+        for (int i = threadIdx.x; i < slot; i += FFT_BLOCK) {
+            const int yy = i / n0, kx = i - yy * n0, ky = y0 + yy;
+            unit_k(ky, n1, tyv);
+            double txv;
+            unit_k(kx, n0, txv);
+            const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
+            const bool nyy = (n1 % 2 == 0) && (ky == n1 / 2);
+            double sr6[6], si6[6];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) {
+                const cx<T> s = buf[c * slot + i];
+                sr6[c] = (double)s.x;
+                si6[c] = (double)s.y;
+            }
+            double sr[9], si[9];
+            b6_to_t9(sr6, sr);
+            b6_to_t9(si6, si);
+            double dr[9], di[9];
+            if (kx == 0 && ky == 0 && kz == 0) {
+                // k = 0: Gamma = -S0 (ElastBC 0) or 0 (init.c:558-569).  dc*: sum of sig over the grid
+                // that the mean-subtracted spectrum lacks (mixed precision), B basis.
+                if (cst->elast_bc == 0) {
+                    const double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
+                    double dct[9];
+                    b6_to_t9(dc6, dct);
+                    for (int a = 0; a < 9; ++a) {
+                        double accr = 0.0, acci = 0.0;
+                        for (int b = 0; b < 9; ++b) {
+                            accr += -cst->S0[a * 9 + b] * (sr[b] + dct[b]);
+                            acci += -cst->S0[a * 9 + b] * si[b];
+                        }
+                        dr[a] = accr;
+                        di[a] = acci;
+                    }
+                } else {
+#pragma unroll
+                    for (int a = 0; a < 9; ++a) dr[a] = di[a] = 0.0;
+                }
+            } else {
+                double nv[3] = {txv, tyv, tzv};
+                {
+                    const double g2 = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+                    const double rg = 1. / g2;
+                    nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
+                }
+                double wr[3], wi[3];
+                green_w(C0, nv, sr, si, wr, wi);
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int b = 0; b < 3; ++b) {
+                        dr[3 * a + b] = -wr[a] * nv[b];
+                        di[3 * a + b] = -wi[a] * nv[b];
+                    }
+                if (nyx || nyy || nyz) {
+                    // Taking Re() of the inverse transform (evolution.c:1197) keeps the Hermitian part
+                    // of du; at a Nyquist index that part averages Gamma(n) with Gamma(n') where n'
+                    // has the Nyquist components negated.
+                    double np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
+                    double wr2[3], wi2[3];
+                    green_w(C0, np, sr, si, wr2, wi2);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b) {
+                            dr[3 * a + b] = 0.5 * (dr[3 * a + b] - wr2[a] * np[b]);
+                            di[3 * a + b] = 0.5 * (di[3 * a + b] - wi2[a] * np[b]);
+                        }
+                }
+            }
+#pragma unroll
+            for (int c = 0; c < 9; ++c) buf[c * slot + i] = cx<T>{T(dr[c]), T(di[c])};
+        }
+        __syncthreads();
+    }
+    if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, tw, +1);
+    for (int i = threadIdx.x; i < cout * slot; i += FFT_BLOCK) {
+        const int c = i / slot, r = i - c * slot, yy = r / n0, x = r - yy * n0;
+        spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x] = buf[c * slot + yy * n0 + x];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// inverse z (c2r) fused with Eq. 15: dg[c] += ddg[c] + Re(IFFT)/Nxyz
+// grid: (n0, ceil(n1/TY)); `dg` may be an output (scale mode) when accumulate=0
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(FFT_BLOCK) k_inv_z(const cx<T> *__restrict__ specB, double *__restrict__ dg,
+                                                     int ncomp, int n0, int n1, int n2, FftPlan pz,
+                                                     const cx<T> *__restrict__ tw, int TY, int accumulate,
+                                                     double divisor, double a0, double a1, double a2, double a3,
+                                                     double a4, double a5, double a6, double a7, double a8) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    const int x = blockIdx.x;
+    const int y0 = blockIdx.y * TY;
+    const int ty = min(TY, n1 - y0);
+    const int H = n2 / 2 + 1;
+    const size_t N = (size_t)n0 * n1 * n2;
+    const size_t HS = (size_t)H * n0 * n1;
+    const double add[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+    for (int ca = 0; ca < ncomp; ca += 2) {
+        const int cb = ca + 1;
+        const bool two = cb < ncomp;
+        for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
+            const int kz = i / ty, yy = i - kz * ty;
+            const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
+            const cx<T> A = specB[ca * HS + o];
+            const cx<T> B = two ? specB[cb * HS + o] : cx<T>{T(0), T(0)};
+            buf[yy * n2 + kz] = cx<T>{A.x - B.y, A.y + B.x};
+            if (kz >= 1 && kz <= n2 - H) buf[yy * n2 + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
+        }
+        __syncthreads();
+        lds_fft<T, 4>(buf, ty, pz, tw, +1);
+        for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
+            const int yy = i / n2, z = i - yy * n2;
+            const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
+            const cx<T> v = buf[yy * n2 + z];
+            if (accumulate) {
+                dg[ca * N + g] += add[ca] + (double)v.x / divisor;
+                if (two) dg[cb * N + g] += add[cb] + (double)v.y / divisor;
+            } else {
+                dg[ca * N + g] = (double)v.x / divisor;
+                if (two) dg[cb * N + g] = (double)v.y / divisor;
+            }
+        }
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// rotations in the B basis
+// ---------------------------------------------------------------------------
+// Q[al][be] = B_al : (a B_be a^T) for the 5 deviatoric basis tensors; maps
+// crystal-frame B-vectors to sample-frame ones when a = TranMat_xt2sa.
+__device__ __forceinline__ void sym_to_b5(double m00, double m11, double m22, double m12, double m02, double m01,
+                                          double *q) {
+    q[0] = SRX_R2 * (m11 - m00);
+    q[1] = SRX_R6 * (2.0 * m22 - m00 - m11);
+    q[2] = 2.0 * SRX_R2 * m12;
+    q[3] = 2.0 * SRX_R2 * m02;
+    q[4] = 2.0 * SRX_R2 * m01;
+}
+
+__device__ __forceinline__ void q5_from(const double a[9], double Q[5][5]) {
+    // columns of a
+    const double c0[3] = {a[0], a[3], a[6]}, c1[3] = {a[1], a[4], a[7]}, c2[3] = {a[2], a[5], a[8]};
+    double col[5];
+#define SRX_OUTER(u, v) u[0] * v[0], u[1] * v[1], u[2] * v[2], u[1] * v[2], u[0] * v[2], u[0] * v[1]
+    {   // beta 0: r2 (-c0c0 + c1c1)
+        const double m00 = SRX_R2 * (c1[0] * c1[0] - c0[0] * c0[0]), m11 = SRX_R2 * (c1[1] * c1[1] - c0[1] * c0[1]);
+        const double m22 = SRX_R2 * (c1[2] * c1[2] - c0[2] * c0[2]), m12 = SRX_R2 * (c1[1] * c1[2] - c0[1] * c0[2]);
+        const double m02 = SRX_R2 * (c1[0] * c1[2] - c0[0] * c0[2]), m01 = SRX_R2 * (c1[0] * c1[1] - c0[0] * c0[1]);
+        sym_to_b5(m00, m11, m22, m12, m02, m01, col);
+        for (int k = 0; k < 5; ++k) Q[k][0] = col[k];
+    }
+    {   // beta 1: r6 (-c0c0 - c1c1 + 2 c2c2)
+        double m[6];
+        for (int t = 0; t < 3; ++t) m[t] = SRX_R6 * (2.0 * c2[t] * c2[t] - c0[t] * c0[t] - c1[t] * c1[t]);
+        m[3] = SRX_R6 * (2.0 * c2[1] * c2[2] - c0[1] * c0[2] - c1[1] * c1[2]);
+        m[4] = SRX_R6 * (2.0 * c2[0] * c2[2] - c0[0] * c0[2] - c1[0] * c1[2]);
+        m[5] = SRX_R6 * (2.0 * c2[0] * c2[1] - c0[0] * c0[1] - c1[0] * c1[1]);
+        sym_to_b5(m[0], m[1], m[2], m[3], m[4], m[5], col);
+        for (int k = 0; k < 5; ++k) Q[k][1] = col[k];
+    }
+    // beta 2,3,4: r2 (u v^T + v u^T)
+    const double *us[3] = {c1, c0, c0}, *vs[3] = {c2, c2, c1};
+    for (int bb = 0; bb < 3; ++bb) {
+        const double *u = us[bb], *v = vs[bb];
+        const double m00 = SRX_R2 * 2.0 * u[0] * v[0], m11 = SRX_R2 * 2.0 * u[1] * v[1], m22 = SRX_R2 * 2.0 * u[2] * v[2];
+        const double m12 = SRX_R2 * (u[1] * v[2] + v[1] * u[2]), m02 = SRX_R2 * (u[0] * v[2] + v[0] * u[2]);
+        const double m01 = SRX_R2 * (u[0] * v[1] + v[0] * u[1]);
+        sym_to_b5(m00, m11, m22, m12, m02, m01, col);
+        for (int k = 0; k < 5; ++k) Q[k][2 + bb] = col[k];
+    }
+#undef SRX_OUTER
+}
+
+// packed symmetric 5x5: index of (i,j), i<=j
+__device__ __forceinline__ constexpr int ps(int i, int j) {
+    return i <= j ? (i * (9 - i)) / 2 + j : (j * (9 - j)) / 2 + i;
+}
+
+// S' = Q2 diag(d) Q2^T, packed
+__device__ __forceinline__ void rot_diag(const double Q2[5][5], const double d[5], double S[15]) {
+    for (int i = 0; i < 5; ++i)
+        for (int j = i; j < 5; ++j) {
+            double acc = 0.0;
+            for (int k = 0; k < 5; ++k) acc += Q2[i][k] * d[k] * Q2[j][k];
+            S[ps(i, j)] = acc;
+        }
+}
+
+// rotation of the compliance: crystal frame of `a` vs crystal frame of `a0`
+__device__ __forceinline__ void sprime(const double a[9], const double a0[9], const double d[5], double S[15]) {
+    // R = a^T a0 maps crystal0 vectors to crystal(a) vectors
+    double R[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) R[3 * i + j] = a[i] * a0[j] + a[3 + i] * a0[3 + j] + a[6 + i] * a0[6 + j];
+    double Q2[5][5];
+    q5_from(R, Q2);
+    rot_diag(Q2, d, S);
+}
+
+// ---------------------------------------------------------------------------
+// per-system constants of StrainRate_Orowan (constitutive.c:556-633)
+// ---------------------------------------------------------------------------
+struct SysConst {
+    double tp[NSYS];    // tau_pass (with the current rho_m)
+    double inv[NSYS];   // 1/(tau_0 + tau_cut)
+    double g0[NSYS];    // gamma_0
+};
+
+// forest (chi1) and parallel (chi) densities of system s (constitutive.c:562-577); the
+// tables are zero beyond nsys, so the fixed-length sums equal the reference's.
+__device__ __forceinline__ void rho_pf(const DevPhase &P, const double rs[NSYS], int s, double &rP, double &rF) {
+    rP = 0.0;
+    rF = 0.0;
+#pragma unroll
+    for (int j = 0; j < NSYS; ++j) {
+        rF += P.chi1[s][j] * (rs[j] + 0.0);
+        rP += P.chi[s][j] * (rs[j] + 0.0);
+    }
+}
+
+// slip rate and its derivative, FORWARD_PLUS_BACKWARD (constitutive.c:634-684):
+//   g  = g0 sinh((|rss|-tp)/(tau0+tau_cut)) sign(rss),  g2 = g0 cosh(..)/(tau0+tau_cut), for |rss| > tp
+__device__ __forceinline__ void slip_rate(double rss, double tp, double inv, double g0, double &g, double &g2) {
+    const double ar = fabs(rss);
+    g = 0.0;
+    g2 = 0.0;
+    if (ar > tp) {
+        const double xx = (ar - tp) * inv;
+        const double e = exp(xx), ei = 1.0 / e;
+        g = g0 * (0.5 * (e - ei));
+        if (!(rss > 0)) g = -g;
+        g2 = g0 * (0.5 * (e + ei)) * inv;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// NR kernel: UpdateStress (evolution.c:157-427) in the crystal frame of the
+// current orientation.  With Q the B-basis rotation, Sc = Q^T S066 Q,
+// S' the grain compliance and D = d(edot)/d(sig):
+//   Rt(sig) = Sc (sig - lam) + eps_p + edot(sig) dt + S' sig - eps_c  ( = S066 R )
+//   K = Sc + S' + D dt  ( = S066 J, symmetric positive definite)
+// so the Newton step K^-1 Rt equals the reference's J^-1 R.
+// ---------------------------------------------------------------------------
+constexpr int NR_BLOCK = 128;
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+    return v;
+}
+
+struct NrResult {
+    double cs, ce;
+    double sig[6];
+    int it;
+    int bad;
+};
+
+__device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__restrict__ cst, const Fields &f,
+                                      int v, size_t N, int first_call, int rotated, NrResult &res) {
+    const double dt = cst->dt;
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+    double Q[5][5];
+    q5_from(a, Q);
+    double lam6[6], ec[6], ep[5];
+#pragma unroll
+    for (int k = 0; k < 6; ++k) lam6[k] = f.sig[k * N + v];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) ep[k] = f.eps[k * N + v];
+    {
+        double d[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d[k] = f.dg[k * N + v];
+        // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
+        const double s00 = d[0], s11 = d[4], s22 = d[8];
+        const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
+        ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
+        ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
+        ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
+        ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
+        ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
+        ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
+    }
+    double signorm = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) signorm += lam6[k] * lam6[k];
+    signorm = sqrt(signorm);
+
+    // crystal frame: lam' = Q^T lam, q' = Q^T (eps_p - eps_c)
+    double lam[5], q[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double al = 0.0, aq = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            al += Q[k][i] * lam6[k];
+            aq += Q[k][i] * (ep[k] - ec[k]);
+        }
+        lam[i] = al;
+        q[i] = aq;
+    }
+    const double lamh = lam6[5], qh = -ec[5];
+    // A = Sc + S', r0 = q - Sc lam
+    double A[15], r0[5];
+    {
+        double T1[5][5];   // S066 Q
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) acc += cst->S066[i][k] * Q[k][j];
+                T1[i][j] = acc;
+            }
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = i; j < 5; ++j) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) acc += Q[k][i] * T1[k][j];
+                A[ps(i, j)] = acc;
+            }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double acc = q[i];
+#pragma unroll
+            for (int j = 0; j < 5; ++j) acc -= A[ps(i, j)] * lam[j];
+            r0[i] = acc;
+        }
+        if (rotated) {
+            double a0[9], Sp[15];
+            for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
+            sprime(a, a0, P.sinv, Sp);
+#pragma unroll
+            for (int k = 0; k < 15; ++k) A[k] += Sp[k];
+        } else {
+#pragma unroll
+            for (int i = 0; i < 5; ++i) A[ps(i, i)] += P.sinv[i];
+        }
+    }
+    const double Shh = cst->S066[5][5];
+    const double Ah = Shh + P.sinv[5];
+    const double r0h = qh - Shh * lamh;
+
+    // per-system constants (constitutive.c:556-633); GND densities are zero
+    SysConst sc;
+    double tpo[NSYS];
+    {
+        const int nsys = P.nsys;
+        double rs[NSYS];
+#pragma unroll
+        for (int s = 0; s < NSYS; ++s) rs[s] = f.rho_s[s * N + v];
+#pragma unroll
+        for (int s = 0; s < NSYS; ++s) {
+            if (s < nsys) {
+                double rP, rF;
+                rho_pf(P, rs, s, rP, rF);
+                const double rmn = P.rhom_k * sqrt(rP * rF) * 160.2177;
+                const double tcut = P.kTf * sqrt(rF) / P.c2 / P.c3 / P.b / P.b * 0.1;
+                sc.g0[s] = 1E-3 * rmn * 1.0E10 * P.c2 * P.b * P.expmq / sqrt(rF);
+                sc.tp[s] = P.c1Gb * sqrt(fabs(rP + rmn)) * 1.0E-3;
+                sc.inv[s] = 1.0 / (P.tau0 + tcut);
+                tpo[s] = sc.tp[s];
+                if (first_call) {
+                    const double rmo = f.rho_m[s * N + v];
+                    tpo[s] = P.c1Gb * sqrt(fabs(rP + rmo)) * 1.0E-3;
+                    f.rho_m_old[s * N + v] = rmo;
+                    f.rho_m[s * N + v] = rmn;
+                }
+            } else {
+                sc.g0[s] = 0.0; sc.tp[s] = 1e300; sc.inv[s] = 0.0; tpo[s] = 1e300;
+            }
+        }
+    }
+
+    double sg[5], sh = lamh, pre[5] = {0, 0, 0, 0, 0}, preh = lamh, ed[5] = {0, 0, 0, 0, 0};
+#pragma unroll
+    for (int i = 0; i < 5; ++i) sg[i] = lam[i];
+    int it = 0;
+    double erral = 10 * 1E-10;
+    while (it < 500 && erral > 1E-10) {
+        ++it;
+        const bool lag = first_call && it == 1;
+        double D[15];
+#pragma unroll
+        for (int k = 0; k < 15; ++k) D[k] = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) ed[k] = 0.0;
+#pragma unroll
+        for (int s = 0; s < NSYS; ++s) {
+            const double *p = P.p[s];
+            const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
+            const double tp = lag ? tpo[s] : sc.tp[s];
+            double g, g2;
+            slip_rate(rss, tp, sc.inv[s], sc.g0[s], g, g2);
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                ed[i] += p[i] * g;
+                const double gp = g2 * p[i];
+#pragma unroll
+                for (int j = i; j < 5; ++j) D[ps(i, j)] += gp * p[j];
+            }
+        }
+        // residual and tangent
+        double R[5], K[15];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double acc = r0[i] + ed[i] * dt;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) acc += A[ps(i, j)] * sg[j];
+            R[i] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) K[k] = A[k] + D[k] * dt;
+        // LDL^T factorisation and solve of K d = R
+        double Ld[5], L[5][5];
+#pragma unroll
+        for (int j = 0; j < 5; ++j) {
+            double dj = K[ps(j, j)];
+#pragma unroll
+            for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k] * Ld[k];
+            Ld[j] = dj;
+            const double rj = 1.0 / dj;
+#pragma unroll
+            for (int i = j + 1; i < 5; ++i) {
+                double acc = K[ps(i, j)];
+#pragma unroll
+                for (int k = 0; k < j; ++k) acc -= L[i][k] * L[j][k] * Ld[k];
+                L[i][j] = acc * rj;
+            }
+        }
+        double y[5];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double acc = R[i];
+#pragma unroll
+            for (int k = 0; k < i; ++k) acc -= L[i][k] * y[k];
+            y[i] = acc;
+        }
+#pragma unroll
+        for (int i = 0; i < 5; ++i) y[i] /= Ld[i];
+        double d5[5];
+#pragma unroll
+        for (int i = 4; i >= 0; --i) {
+            double acc = y[i];
+#pragma unroll
+            for (int k = i + 1; k < 5; ++k) acc -= L[k][i] * d5[k];
+            d5[i] = acc;
+        }
+        const double Rh = Ah * sh + r0h;
+        const double dh = Rh / Ah;
+        double cnr = dh * dh;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            pre[i] = sg[i];
+            sg[i] -= d5[i];
+            cnr += d5[i] * d5[i];
+        }
+        preh = sh;
+        sh -= dh;
+        erral = sqrt(cnr) / signorm;
+    }
+    // outputs in the sample frame; errors of evolution.c:381-391 (rotation invariant)
+    double cs2 = 0.0;
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc += Q[i][k] * sg[k];
+        res.sig[i] = acc;
+        cs2 += (acc - lam6[i]) * (acc - lam6[i]);
+    }
+    res.sig[5] = sh;
+    cs2 += (sh - lamh) * (sh - lamh);
+    res.cs = sqrt(cs2);
+    // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre
+    {
+        double Sp[15];
+        if (rotated) {
+            double a0[9];
+            for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
+            sprime(a, a0, P.sinv, Sp);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 15; ++k) Sp[k] = 0.0;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) Sp[ps(i, i)] = P.sinv[i];
+        }
+        double ce2 = 0.0;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double acc = q[i] + ed[i] * dt;
+#pragma unroll
+            for (int j = 0; j < 5; ++j) acc += Sp[ps(i, j)] * pre[j];
+            ce2 += acc * acc;
+        }
+        const double eh = qh + P.sinv[5] * preh;
+        ce2 += eh * eh;
+        res.ce = sqrt(ce2);
+    }
+#pragma unroll
+    for (int k = 0; k < 5; ++k) f.sig_pre[k * N + v] = pre[k];
+    res.it = it;
+    bool bad = false;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) bad |= !isfinite(res.sig[k]);
+    res.bad = bad;
+}
+
+__global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, int first_call,
+                                                 int rotated, double *__restrict__ partials) {
+    const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
+    const size_t N = (size_t)nvox;
+    const bool valid = v < nvox;
+    const int ph = valid ? f.phase[v] : 0;
+    NrResult r;
+    r.cs = 0.0; r.ce = 0.0; r.it = 0; r.bad = 0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) r.sig[k] = 0.0;
+    // one pass per distinct phase present in the wavefront, with the phase index uniform
+    unsigned long long todo = __ballot(valid && ph >= 1 && ph <= NPH);
+    while (todo) {
+        const int first = __ffsll((unsigned long long)todo) - 1;
+        const int pu = __builtin_amdgcn_readfirstlane(__shfl(ph, first, 64));
+        const bool mine = valid && ph == pu;
+        if (mine && cst->ph[pu - 1].xtal) nr_voxel(cst->ph[pu - 1], cst, f, v, N, first_call, rotated, r);
+        todo &= ~__ballot(mine);
+    }
+    const bool xtal = valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal;
+    if (valid) {
+        // gas voxels: Sig = Edot = 0 (evolution.c:418-423)
+#pragma unroll
+        for (int k = 0; k < 6; ++k) f.sig[k * N + v] = r.sig[k];
+        f.nr_it[v] = r.it;
+        (void)xtal;
+    }
+    // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan
+    __shared__ double red[NR_BLOCK / 64][NR_NPART];
+    double vals[NR_NPART];
+    vals[0] = r.cs;
+    vals[1] = r.ce;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) {
+        vals[2 + k] = r.sig[k];
+        vals[8 + k] = (valid && ph == 1) ? r.sig[k] : 0.0;
+    }
+    vals[14] = (double)r.it;
+    vals[15] = (double)r.it;
+    vals[16] = (double)r.bad;
+    vals[17] = 0.0;
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < NR_NPART; ++k) {
+        const double s = (k == 15) ? wave_max(vals[k]) : wave_sum(vals[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < NR_NPART) {
+        const int k = threadIdx.x;
+        double acc = red[0][k];
+        for (int w = 1; w < NR_BLOCK / 64; ++w) acc = (k == 15) ? fmax(acc, red[w][k]) : acc + red[w][k];
+        partials[(size_t)blockIdx.x * NR_NPART + k] = acc;
+    }
+}
+
+// fixed-order reduction of block partials: out[k] = sum (or max for k in maxmask)
+__global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
+                                                unsigned long long maxmask, double *__restrict__ out) {
+    __shared__ double sh[256];
+    for (int k = 0; k < nvals; ++k) {
+        const bool ismax = (maxmask >> k) & 1ull;
+        double acc = ismax ? -1e308 : 0.0;
+        for (int b = threadIdx.x; b < nblocks; b += 256) {
+            const double x = part[(size_t)b * nvals + k];
+            acc = ismax ? fmax(acc, x) : acc + x;
+        }
+        sh[threadIdx.x] = acc;
+        __syncthreads();
+        for (int s = 128; s > 0; s >>= 1) {
+            if (threadIdx.x < s) sh[threadIdx.x] = ismax ? fmax(sh[threadIdx.x], sh[threadIdx.x + s]) : sh[threadIdx.x] + sh[threadIdx.x + s];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[k] = sh[0];
+        __syncthreads();
+    }
+}
+
+// increment prologue (evolution.c:1115-1128): dg0 = dg; dg += Udot dt (or = set9 under creep)
+__global__ void k_prologue(Fields f, int nvox, double a0, double a1, double a2, double a3, double a4, double a5,
+                           double a6, double a7, double a8, int creep, double s0, double s1, double s2, double s3,
+                           double s4, double s5, double s6, double s7, double s8) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= nvox) return;
+    const size_t N = (size_t)nvox;
+    const double add[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+    const double set[9] = {s0, s1, s2, s3, s4, s5, s6, s7, s8};
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        const double d = f.dg[k * N + v];
+        f.dg0[k * N + v] = d;
+        f.dg[k * N + v] = creep ? set[k] : d + add[k];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// end of increment: Edot/Eps (evolution.c:1297-1301), update_orient
+// (constitutive.c:142-273), DislocationEvolution (constitutive.c:1353-1529)
+// and the partial sums of AvgDislocation and the Eps/Edot averages.
+// ---------------------------------------------------------------------------
+constexpr int END_BLOCK = 128;
+
+__device__ void end_voxel(const DevPhase &P, const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
+                          int lagged_possible, int do_orient, int istep, double *vals) {
+    const double dt = cst->dt;
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+    double Q[5][5];
+    q5_from(a, Q);
+    const bool lagged = lagged_possible && f.nr_it[v] == 1;
+    double rs[NSYS];
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s) rs[s] = f.rho_s[s * N + v];
+    double pre[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) pre[k] = f.sig_pre[k * N + v];
+    // final stress in the crystal frame (Schmid tensors of this increment)
+    double sf[5];
+    {
+        double s6[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) s6[k] = f.sig[k * N + v];
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc += Q[k][i] * s6[k];
+            sf[i] = acc;
+        }
+    }
+    double gam[NSYS], edc[5] = {0, 0, 0, 0, 0};
+    double rFs[NSYS];
+    double sumF = 0, sumP = 0, sumM = 0, rho_tot = 0.0;
+    const int nsys = P.nsys;
+    // rho_tot of evolution.c:1040-1046, from the densities at the start of the increment
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s)
+        if (s < nsys) rho_tot += rs[s] + f.rho_m_old[s * N + v] + sqrt(0.0);
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s) {
+        gam[s] = 0.0;
+        rFs[s] = 0.0;
+        if (s < nsys) {
+            double rP, rF;
+            rho_pf(P, rs, s, rP, rF);
+            const double rmn = f.rho_m[s * N + v];
+            const double rmo = f.rho_m_old[s * N + v];
+            const double tcut = P.kTf * sqrt(rF) / P.c2 / P.c3 / P.b / P.b * 0.1;
+            const double g0 = 1E-3 * rmn * 1.0E10 * P.c2 * P.b * P.expmq / sqrt(rF);
+            const double tp = P.c1Gb * sqrt(fabs(rP + (lagged ? rmo : rmn))) * 1.0E-3;
+            const double inv = 1.0 / (P.tau0 + tcut);
+            const double *p = P.p[s];
+            const double rss = p[0] * pre[0] + p[1] * pre[1] + p[2] * pre[2] + p[3] * pre[3] + p[4] * pre[4];
+            double g, g2;
+            slip_rate(rss, tp, inv, g0, g, g2);
+            gam[s] = g;
+#pragma unroll
+            for (int i = 0; i < 5; ++i) edc[i] += p[i] * g;
+            rFs[s] = rF;
+            f.rho_F[s * N + v] = rF;
+            f.rho_P[s * N + v] = rP;
+            f.gamdot[s * N + v] = g;
+            sumF += rF;
+            sumP += rP;
+            sumM += rmn;
+        }
+    }
+    // Edot in the sample frame and Eps += Edot dt
+    double eds[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc += Q[i][k] * edc[k];
+        eds[i] = acc;
+        f.edot[i * N + v] = acc;
+        const double e = f.eps[i * N + v] + acc * dt;
+        f.eps[i * N + v] = e;
+        vals[i] = e;
+        vals[6 + i] = acc;
+    }
+    vals[5] = 0.0;
+    vals[11] = 0.0;
+    (void)eds;
+    // reorientation (update_orient), with the pre-update TranMat for b and n
+    if (do_orient) {
+        double rl[3][3], lp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double vg[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) vg[k] = (f.dg[k * N + v] - f.dg0[k * N + v]) / dt;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) rl[i][j] = (vg[3 * i + j] - vg[3 * j + i]) / 2.;
+#pragma unroll
+        for (int s = 0; s < NSYS; ++s) {
+            if (s >= nsys) continue;
+            double bs[3], ns[3];
+            for (int i = 0; i < 3; ++i) {
+                bs[i] = a[3 * i] * P.bb[s][0] + a[3 * i + 1] * P.bb[s][1] + a[3 * i + 2] * P.bb[s][2];
+                ns[i] = a[3 * i] * P.bn[s][0] + a[3 * i + 1] * P.bn[s][1] + a[3 * i + 2] * P.bn[s][2];
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) lp[i][j] += bs[i] * ns[j] * gam[s];
+        }
+        double rot[3][3], rsl[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) rsl[i][j] = (lp[i][j] - lp[j][i]) / 2.;
+        vals[18] = sqrt(rsl[2][1] * rsl[2][1] + rsl[0][2] * rsl[0][2] + rsl[1][0] * rsl[1][0]);
+        vals[19] = sqrt(rl[2][1] * rl[2][1] + rl[0][2] * rl[0][2] + rl[1][0] * rl[1][0]);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) rot[i][j] = (cst->Udot_a[3 * i + j] + rl[i][j] - rsl[i][j]) * dt;
+        // Orientation (constitutive.c:142-194)
+        double w[3] = {rot[2][1], rot[0][2], rot[1][0]};
+        double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
+        double wt = tan(wn / 2.0);
+        if (fabs(wn) < 1E-6) wn = 1.0;
+        for (int i = 0; i < 3; ++i) w[i] *= wt / wn;
+        const double om[3][3] = {{0., -w[2], w[1]}, {w[2], 0., -w[0]}, {-w[1], w[0], 0.}};
+        double om2[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) om2[i][j] = om[i][0] * om[0][j] + om[i][1] * om[1][j] + om[i][2] * om[2][j];
+        wt = wt * wt;
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) rot[i][j] = (double)(i == j) + 2. * (om[i][j] + om2[i][j]) / (1. + wt);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j)
+                f.tran[(3 * i + j) * N + v] = rot[i][0] * a[j] + rot[i][1] * a[3 + j] + rot[i][2] * a[6 + j];
+    }
+    // DislocationEvolution, istep <= 2e6 branch
+    const double G = P.G, nu = cst->nu, T = cst->T, D0 = cst->D0;
+    double sR = 0, sD = 0, sFo = 0, sI = 0;
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s) {
+        if (s >= nsys) continue;
+        const double *p = P.p[s];
+        const double rss = p[0] * sf[0] + p[1] * sf[1] + p[2] * sf[2] + p[3] * sf[3] + p[4] * sf[4];
+        const double bb = P.b;
+        const double d_dip = sqrt(3.0) * G * bb / 16.0 / 3.14159265359 / (1 - nu) / (fabs(rss) + 1.0E-4);
+        const double hydro = G * bb / 3.14159265359 / (1 - nu) / d_dip;
+        double r = rs[s];
+        double rdip = f.rho_dipole[s * N + v], rfor = f.rho_forest[s * N + v], rinp = f.rho_inplane[s * N + v];
+        const double gd = gam[s];
+        if (istep <= 2000000) {
+            const double climb = 1E-17 * rho_tot * 0.5 * 1E24 * (d_dip) * (D0 / bb) * pow(r, 2.0) *
+                                 (exp(fabs(hydro) * bb * bb * bb * 100 / (1.38 * T)) - 1.0) * P.expmqb;
+            rdip += climb * dt;
+            rfor += fabs(gd) * (P.c4 * sqrt(rFs[s])) * dt;
+            rinp += fabs(gd) * (P.c5 * r) * dt;
+            const double dot = fabs(gd) * (P.c4 * sqrt(rFs[s]) + P.c6 * (d_dip) * f.rho_m[s * N + v] - P.c5 * r) - climb;
+            r += dot * dt;
+        }
+        f.rho_s[s * N + v] = r;
+        f.rho_dipole[s * N + v] = rdip;
+        f.rho_forest[s * N + v] = rfor;
+        f.rho_inplane[s * N + v] = rinp;
+        f.gam_acum[s * N + v] += gd * dt;
+        sR += r;
+        sD += rdip;
+        sFo += rfor;
+        sI += rinp;
+    }
+    vals[12] = sumF;
+    vals[13] = sumP;
+    vals[14] = sR;
+    vals[15] = sumM;
+    vals[16] = sD;
+    vals[17] = sFo;
+    vals[20] = sI;
+    bool bad = false;
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s) bad |= !isfinite(gam[s]);
+    vals[21] = bad ? 1.0 : 0.0;
+}
+
+__global__ void __launch_bounds__(END_BLOCK) k_end(const DevConst *__restrict__ cst, Fields f, int nvox,
+                                                   int lagged_possible, int do_orient, int istep,
+                                                   double *__restrict__ partials) {
+    const int v = blockIdx.x * END_BLOCK + threadIdx.x;
+    const size_t N = (size_t)nvox;
+    const bool valid = v < nvox;
+    const int ph = valid ? f.phase[v] : 0;
+    double vals[END_NPART];
+#pragma unroll
+    for (int k = 0; k < END_NPART; ++k) vals[k] = 0.0;
+    unsigned long long todo = __ballot(valid && ph >= 1 && ph <= NPH);
+    while (todo) {
+        const int first = __ffsll((unsigned long long)todo) - 1;
+        const int pu = __builtin_amdgcn_readfirstlane(__shfl(ph, first, 64));
+        const bool mine = valid && ph == pu;
+        if (mine && cst->ph[pu - 1].xtal) end_voxel(cst->ph[pu - 1], cst, f, v, N, lagged_possible, do_orient, istep, vals);
+        todo &= ~__ballot(mine);
+    }
+    const bool xtal = valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal;
+    if (valid && !xtal) {
+        // gas: Edot = 0, Eps unchanged (it is zero), gamdot untouched
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            f.edot[i * N + v] = 0.0;
+            vals[i] = f.eps[i * N + v];
+        }
+    }
+    __shared__ double red[END_BLOCK / 64][END_NPART];
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < END_NPART; ++k) {
+        const double s = wave_sum(vals[k]);
+        if (lane == 0) red[wid][k] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < END_NPART) {
+        const int k = threadIdx.x;
+        double acc = red[0][k];
+        for (int w = 1; w < END_BLOCK / 64; ++w) acc += red[w][k];
+        partials[(size_t)blockIdx.x * END_NPART + k] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host launch wrappers
+// ---------------------------------------------------------------------------
+static int min_radix(const FftPlan &p) {
+    int m = 1 << 30;
+    for (int i = 0; i < p.nst; ++i) m = p.rad[i] < m ? p.rad[i] : m;
+    return p.nst ? m : 1;
+}
+
+bool make_plan(int n, FftPlan &p) {
+    p.n = n;
+    p.nst = 0;
+    int m = n;
+    if (n < 1 || n > 1024) return false;
+    while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
+    if (p.nst > MAX_RADIX_STAGES) return false;
+    return m == 1;
+}
+
+// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK and t <= cap
+static int tile_for(const FftPlan &p, int per_tile_comps, int maxb, int cap) {
+    const int mr = min_radix(p);
+    int t = (maxb * FFT_BLOCK * mr) / (p.n * per_tile_comps);
+    if (t > cap) t = cap;
+    if (t < 1) t = 1;
+    return t;
+}
+
+static void check_launch(const char *name) {
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) fprintf(stderr, "srx: launch of %s failed: %s\n", name, hipGetErrorString(e));
+}
+
+void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
+                      const void *twz, const double *mean6) {
+    // ncomp encoded in mean6 == nullptr? no: always the 6 stress comps here
+    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    const double z6[6] = {0, 0, 0, 0, 0, 0};
+    const double *m = mean6 ? mean6 : z6;
+    if (prec == 0) {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        hipLaunchKernelGGL(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
+                           n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    } else {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        hipLaunchKernelGGL(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
+                           n[2], pz, (const cx<float> *)twz, TY, m[0], m[1], m[2], m[3], m[4], m[5]);
+    }
+    check_launch("fwd_z");
+}
+
+void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c, const int n[3],
+                  const FftPlan &py, const void *twy) {
+    (void)sign;
+    const int TX = tile_for(py, 1, 4, n[0] < 16 ? n[0] : 16);
+    const int H = n[2] / 2 + 1;
+    dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
+    if (prec == 0) {
+        size_t sm = (size_t)TX * n[1] * sizeof(cx<double>);
+        hipLaunchKernelGGL(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
+                           n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX);
+    } else {
+        size_t sm = (size_t)TX * n[1] * sizeof(cx<float>);
+        hipLaunchKernelGGL(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
+                           n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX);
+    }
+    check_launch("fft_y");
+}
+
+static void launch_green_generic(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px,
+                                 const void *twx, const DevConst *cst, const double *dc6, int mode, int ncomp) {
+    const int slots = mode == 1 ? 9 : ncomp;
+    // MAXB 6 covers 9 slots of two 128-point pencils with a radix-2 stage
+    const int TY = tile_for(px, slots, 6, n[1] < 2 ? n[1] : 2);
+    const int H = n[2] / 2 + 1;
+    dim3 grid((n[1] + TY - 1) / TY, H);
+    const double z6[6] = {0, 0, 0, 0, 0, 0};
+    const double *d = dc6 ? dc6 : z6;
+    if (prec == 0) {
+        size_t sm = (size_t)slots * TY * n[0] * sizeof(cx<double>);
+        hipLaunchKernelGGL((k_green_x<double, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
+                           px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+    } else {
+        size_t sm = (size_t)slots * TY * n[0] * sizeof(cx<float>);
+        hipLaunchKernelGGL((k_green_x<float, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
+                           px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+    }
+    check_launch("green_x");
+}
+
+void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px, const void *twx,
+                    const DevConst *cst, const double *dc_sig6_sum) {
+    launch_green_generic(s, prec, specC, n, px, twx, cst, dc_sig6_sum, 1, 9);
+}
+
+void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
+                             const void *twz, const double *ddg9, double divisor) {
+    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    const double *a = ddg9;
+    if (prec == 0) {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        hipLaunchKernelGGL(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
+                           n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
+                           a[6], a[7], a[8]);
+    } else {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        hipLaunchKernelGGL(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
+                           n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
+                           a[7], a[8]);
+    }
+    check_launch("inv_z");
+}
+
+int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
+int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
+
+void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
+               double *partials, int) {
+    hipLaunchKernelGGL(k_nr, dim3(nr_block_count(nvox)), dim3(NR_BLOCK), 0, s, cst, f, nvox, first_call, rotated,
+                       partials);
+    check_launch("nr");
+}
+
+void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out) {
+    const unsigned long long mask = nvals == NR_NPART ? (1ull << 15) : 0ull;
+    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, partials, nblocks, nvals, mask, out);
+    check_launch("reduce");
+}
+
+void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int creep, const double *st) {
+    const int B = 256;
+    hipLaunchKernelGGL(k_prologue, dim3((nvox + B - 1) / B), dim3(B), 0, s, f, nvox, a[0], a[1], a[2], a[3], a[4],
+                       a[5], a[6], a[7], a[8], creep, st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7], st[8]);
+    check_launch("prologue");
+}
+
+void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int,
+                int do_orient, int, int istep, double *partials) {
+    hipLaunchKernelGGL(k_end, dim3(end_block_count(nvox)), dim3(END_BLOCK), 0, s, cst, f, nvox, lagged_possible,
+                       do_orient, istep, partials);
+    check_launch("end");
+}
+
+// test path: forward FFT of ncomp real comps to layout C (x pass done), no Green
+void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int ncomp, void *specB, void *specC,
+                                  const int n[3], const FftPlan *pl, void *const *tw) {
+    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    if (prec == 0) {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        hipLaunchKernelGGL(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
+                           n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    } else {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        hipLaunchKernelGGL(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
+                           n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+    }
+    check_launch("fwd_z(test)");
+    launch_fft_y(s, prec, specB, specC, ncomp, -1, 1, n, pl[1], tw[1]);
+    launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 0, ncomp);
+}
+
+// test path: inverse of the above for ncomp comps (layout C in, real out = IFFT * 1/divisor)
+void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *specB, int ncomp, double *out,
+                                  const int n[3], const FftPlan *pl, void *const *tw, double divisor) {
+    launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp);
+    launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1]);
+    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    if (prec == 0) {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        hipLaunchKernelGGL(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
+                           n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
+                           0., 0.);
+    } else {
+        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        hipLaunchKernelGGL(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
+                           n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
+                           0.);
+    }
+    check_launch("inv_z(test)");
+}
+
+}  // namespace srx
