@@ -1,0 +1,204 @@
+#!/usr/bin/env python3
+"""Benchmark: converged voxel-increments/s of the EVP-FFT increment loop on one MI355X.
+
+Workload: periodic Voronoi polycrystal, 128^3 voxels, 1000 grains, uniform
+random orientations, FCC Cu, x3 tension at 1e-3 1/s with stress-free lateral
+faces.  A "step" is one loading increment iterated to the deck's Err (2e-5):
+all outer fixed-point iterations (FFT -> Green operator -> inverse FFT ->
+per-voxel Newton solve -> error norms) plus the end-of-increment hardening.
+
+The increment is TimeStep 0.1 s (0.01% strain).  At 0.1% per increment the
+elastic predictor of the first increment puts the slip-rate exponent near 150
+and the reference's plain Newton iteration overflows; 0.1 s is the TimeStep of
+the reference's own deck.
+
+Both precisions run the same increments:
+  fp64  -- every pass in FP64
+  mixed -- FFT and Green passes in FP32 (mean stress removed before the cast),
+           Newton solve, hardening, DisGrad accumulation and residuals in FP64.
+`value` is the mixed-precision throughput; the FP64 figure, the relative
+macroscopic-stress difference between the two, per-kernel times, roofline
+fractions and a CPU baseline (the FP64 CPU oracle on a bounded sample of the
+same workload) are reported beside it.  Prints one JSON line.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
+FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix), TFLOP/s
+F_NR = 2300.0                  # flops per voxel per Newton iteration (algorithmic count)
+METRIC = "converged voxel-increments/s at 128³, plain FP64 vs mixed-precision at equal stress error"
+
+
+def fft_green_bytes_per_voxel(real_bytes: int, n2: int, c_out: int = 9) -> float:
+    """Algorithmic HBM bytes per voxel of one FFT + Green + inverse FFT + DisGrad update pass:
+    FP64 stress in (6 x 8 B), DisGrad read+write (c_out x 16 B), and the half-spectrum
+    passes: 6 comps x 4 touches forward, c_out comps x 4 touches inverse, each 2*s*h bytes."""
+    h = (n2 // 2 + 1) / n2
+    return 48.0 + 16.0 * c_out + (24.0 + 4.0 * c_out) * 2.0 * real_bytes * h
+
+
+def run_precision(deck: str, precision: int, steps: int, warmup: int) -> dict:
+    import srx_amd
+    s = srx_amd.Solver(deck, precision=precision, reserved=[1])
+    for _ in range(warmup):
+        s.step()
+    kt0 = s.kernel_times()
+    reps = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        reps.append(s.step())
+    t1 = time.perf_counter()
+    kt1 = s.kernel_times()
+    kt = {k: (kt1[k][0] - kt0.get(k, (0, 0))[0], kt1[k][1] - kt0.get(k, (0, 0))[1]) for k in kt1}
+    sig = np.array(reps[-1]["sig_avg"])
+    s.close()
+    return {"seconds": t1 - t0, "reports": reps, "kernels": kt, "sig_avg": sig,
+            "iters": sum(r["iters"] for r in reps), "nr_total": sum(r["nr_iters_total"] for r in reps)}
+
+
+def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: int) -> dict | None:
+    """The FP64 CPU oracle on a bounded sample: the first `outer_iters` outer iterations of
+    increment 0 of the same 128^3 deck, scaled to an increment by the GPU's iterations/increment."""
+    try:
+        from oracle import oracle
+        oracle.build()
+        threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+        o = oracle.Oracle(deck, fixed_iters=outer_iters, nthreads=threads)
+        t0 = time.perf_counter()
+        r = o.step()
+        t1 = time.perf_counter()
+        o.close()
+    except Exception as exc:  # the baseline is reported, never required
+        return {"error": str(exc)}
+    t_iter = (t1 - t0) / outer_iters
+    value = nvox / (t_iter * iters_per_increment)
+    return {"value": value, "unit": "voxel-increments/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (OpenMP, FP64), first %d outer iterations of increment 0 at %d^3 "
+                      "(%.2f s/iteration, %.2f Newton iterations/voxel/iteration); scaled by the GPU run's "
+                      "%.1f outer iterations per increment" % (outer_iters, round(nvox ** (1 / 3)), t_iter,
+                                                               r["nr_iters_total"] / nvox / outer_iters,
+                                                               iters_per_increment)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2, help="timed increments")
+    ap.add_argument("--warmup", type=int, default=1, help="untimed increments before the timed ones")
+    ap.add_argument("--n", type=int, default=128)
+    ap.add_argument("--grains", type=int, default=1000)
+    ap.add_argument("--seed", type=int, default=20261202)
+    ap.add_argument("--precisions", default="fp64,mixed")
+    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--workdir", default=None)
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU only (the whole grid is resident on one MI355X; "
+              "ensembles shard by RVE, see DESIGN.md); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+        return 2
+
+    from srx_amd import inputs
+    import srx_amd
+
+    work = args.workdir or tempfile.mkdtemp(prefix="srx_bench_")
+    n = (args.n, args.n, args.n)
+    nvox = n[0] * n[1] * n[2]
+    deck = os.path.join(work, "deck.in")
+    if not os.path.exists(deck):
+        inputs.make_case(work, n=n, grains=args.grains, seed=args.seed, nsteps=args.warmup + args.steps + 1,
+                         timestep=0.1)
+    print("bench: inputs in %s" % work, file=sys.stderr, flush=True)
+
+    res = {}
+    for name in args.precisions.split(","):
+        prec = srx_amd.PREC_MIXED if name == "mixed" else srx_amd.PREC_FP64
+        print("bench: %s run (%d warmup + %d timed increments)" % (name, args.warmup, args.steps),
+              file=sys.stderr, flush=True)
+        res[name] = run_precision(deck, prec, args.steps, args.warmup)
+        r = res[name]
+        print("bench: %s %.3f s, %d outer iterations, kernels %s" % (
+            name, r["seconds"], r["iters"], {k: round(v[0], 1) for k, v in r["kernels"].items()}),
+            file=sys.stderr, flush=True)
+
+    head = "mixed" if "mixed" in res else "fp64"
+    H = res[head]
+    value = args.steps * nvox / H["seconds"]
+    out = {"metric": METRIC, "value": value, "unit": "voxel-increments/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1000.0 * H["seconds"] / args.steps, "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None,
+           "dtype": "f64+f32" if head == "mixed" else "f64", "data": "synthetic",
+           "config": {"workload": "EVP-FFT increment loop, %d^3 periodic Voronoi, %d grains, FCC Cu, "
+                                  "x3 tension 1e-3/s, TimeStep 0.1 s, Err 2e-5" % (args.n, args.grains),
+                      "n": args.n, "grains": args.grains, "precision": head, "timestep": 0.1, "err": 2e-5}}
+    per = {}
+    for name, r in res.items():
+        iters = max(r["iters"], 1)
+        kt = r["kernels"]
+        fg_names = ("fwd_z", "fwd_y", "green_x", "inv_y", "inv_z")
+        fg_ms = sum(kt[k][0] for k in fg_names if k in kt)
+        nr_ms = kt.get("nr", (0, 0))[0]
+        rb = 4 if name == "mixed" else 8
+        b_fg = fft_green_bytes_per_voxel(rb, n[2]) * nvox
+        per[name] = {
+            "value": args.steps * nvox / r["seconds"], "seconds": r["seconds"],
+            "outer_iters_per_increment": r["iters"] / args.steps,
+            "nr_iters_per_voxel_per_iter": r["nr_total"] / nvox / iters,
+            "kernel_ms": {k: v[0] for k, v in kt.items()},
+            "fft_green_gbs": b_fg * iters / (fg_ms * 1e-3) / 1e9 if fg_ms else None,
+            "fft_green_frac_hbm": (b_fg * iters / (fg_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if fg_ms else None,
+            "nr_tflops": F_NR * r["nr_total"] / (nr_ms * 1e-3) / 1e12 if nr_ms else None,
+            "fft_green_bytes_per_voxel": fft_green_bytes_per_voxel(rb, n[2]),
+        }
+    out["fp64"] = per.get("fp64")
+    out["mixed"] = per.get("mixed")
+    if "fp64" in res and "mixed" in res:
+        a, b = res["mixed"]["sig_avg"], res["fp64"]["sig_avg"]
+        out["mixed_vs_fp64"] = {"speedup": per["mixed"]["value"] / per["fp64"]["value"],
+                                "rel_macro_stress_error": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+    # roofline of the dominant kernel of the headline run, timed with HIP events over the timed region
+    kt = H["kernels"]
+    dom = max(kt, key=lambda k: kt[k][0])
+    iters = max(H["iters"], 1)
+    if dom == "nr":
+        ach = F_NR * H["nr_total"] / (kt["nr"][0] * 1e-3) / 1e12
+        out["roofline"] = {"kernel": "nr", "bound": "mfma", "peak_kind": "FP64 (vector = matrix rate on MI355X)",
+                           "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                           "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                           "algorithmic": "%.0f flop per voxel per Newton iteration" % F_NR}
+    else:
+        rb = 4 if head == "mixed" else 8
+        per_launch = {"green_x": None}
+        # bytes of the single pass: read 6 comps + write 9 comps of the half spectrum
+        h = (n[2] // 2 + 1) / n[2]
+        pass_bytes = {"fwd_z": (48 + 6 * 2 * rb * h), "fwd_y": 6 * 2 * 2 * rb * h, "green_x": 15 * 2 * rb * h,
+                      "inv_y": 9 * 2 * 2 * rb * h, "inv_z": 9 * 2 * rb * h + 144}
+        bpv = pass_bytes.get(dom, 0.0)
+        ach = bpv * nvox * kt[dom][1] / (kt[dom][0] * 1e-3) / 1e9
+        out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": ach / HBM_PEAK_GBS, "traffic": None,
+                           "algorithmic": "%.1f B per voxel per launch" % bpv}
+        del per_launch
+    out["roofline"]["avg_launch_ms"] = kt[dom][0] / max(kt[dom][1], 1)
+    if not args.no_cpu_baseline:
+        ipi = (res.get("fp64") or H)["iters"] / args.steps
+        print("bench: cpu baseline sample", file=sys.stderr, flush=True)
+        out["cpu_baseline"] = cpu_baseline(deck, nvox, ipi, args.cpu_iters)
+    print(json.dumps(out))
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

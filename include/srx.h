@@ -133,8 +133,9 @@ const char *srx_strerror(int code);
 
 void srx_default_options(srx_options *opt);
 
-/* Per-kernel timing of the last srx_step, from HIP events on the solver's
-   stream: names[i] and total milliseconds ms[i] for i < *count. */
+/* Per-kernel timing accumulated since srx_open, from HIP events recorded on
+   the solver's stream around each launch (enabled by options.reserved[0] = 1):
+   names[i], total milliseconds ms[i] and launch count calls[i], i < *count. */
 int srx_kernel_times(srx_ctx *ctx, const char **names, double *ms, int *calls, int *count);
 
 /* ---- single-kernel entry points (for the parity tests) ---------------- */

@@ -614,27 +614,35 @@ static int setup(orc_state *s)
     s->grain = (int *)calloc(s->nxyz, sizeof(int));
     if (!s->ksig || !s->phase) return 2005;
 
-    /* InitMicroStruct (init.c:233-396) */
+    /* InitMicroStruct (init.c:233-396): lines are read in order, the per-voxel
+       rotations run in parallel, C066 is summed in voxel order. */
     {
         FILE *fp;
         char buf[80];
         long nph1 = 0;
+        double *eul = (double *)malloc(sizeof(double) * 3 * s->nxyz);
         resolve(path, s->workdir, d->initial_ms);
         fp = fopen(path, "r");
-        if (!fp) return 2001;
+        if (!fp) { free(eul); return 2001; }
         memset(c066, 0, sizeof(c066));
         for (p = 0; p < s->nxyz; p++) {
-            double t1 = 0, th = 0, om = 0, sa2xt[9], Cs[81], c66[36];
-            int ii = 0, jj = 0, kk = 0, jgr = 0, jph = 0, a, b, c, e;
+            double t1 = 0, th = 0, om = 0;
+            int ii = 0, jj = 0, kk = 0, jgr = 0, jph = 0;
             long idx;
-            if (!fgets(buf, 80, fp)) { fclose(fp); return 1139; }
+            if (!fgets(buf, 80, fp)) { fclose(fp); free(eul); return 1139; }
             sscanf(buf, "%lf %lf %lf %d %d %d %d %d", &t1, &th, &om, &ii, &jj, &kk, &jgr, &jph);
             idx = ((long)(ii - 1) * s->n[1] + jj - 1) * s->n[2] + kk - 1;
-            if (idx != p) { fclose(fp); return 1139; }
-            if (jph < 1 || jph > NPH) { fclose(fp); return 1139; }
+            if (idx != p || jph < 1 || jph > NPH) { fclose(fp); free(eul); return 1139; }
             if (jph == 1) nph1++;
             s->grain[p] = jgr;
             s->phase[p] = jph;
+            eul[3 * p] = t1; eul[3 * p + 1] = th; eul[3 * p + 2] = om;
+        }
+        fclose(fp);
+        #pragma omp parallel for schedule(static)
+        for (p = 0; p < s->nxyz; p++) {
+            double t1 = eul[3 * p], th = eul[3 * p + 1], om = eul[3 * p + 2], sa2xt[9], Cs[81];
+            int jph = s->phase[p], a, b, c, e;
             if (!s->ph[jph - 1].crystalline) continue;
             t1 *= O_PI / 180.; th *= O_PI / 180.; om *= O_PI / 180.;
             orc_euler_to_matrix(&t1, &th, &om, sa2xt, 2);
@@ -651,10 +659,14 @@ static int setup(orc_state *s)
                     Cs[((a * 3 + b) * 3 + c) * 3 + e] = acc;
                 }
             }
-            orc_chg_basis(0, 0, c66, Cs, 4, 6);
-            for (a = 0; a < 36; a++) { s->cgr[36 * p + a] = c66[a]; c066[a / 6][a % 6] += c66[a]; }
+            orc_chg_basis(0, 0, s->cgr + 36 * p, Cs, 4, 6);
         }
-        fclose(fp);
+        free(eul);
+        for (p = 0; p < s->nxyz; p++) {
+            int a;
+            if (!s->ph[s->phase[p] - 1].crystalline) continue;
+            for (a = 0; a < 36; a++) c066[a / 6][a % 6] += s->cgr[36 * p + a];
+        }
         s->wgt_ph1 = (1.0 * (double)nph1) / s->nxyz;
         for (i = 0; i < 6; i++) for (j = 0; j < 6; j++) s->C066[i][j] = c066[i][j] * s->wgt;
         memcpy(s->S066, s->C066, sizeof(s->C066));
@@ -698,6 +710,7 @@ static int setup(orc_state *s)
         s->dDisGradAvg[i][j] = 0.0;
         s->dDisGradAvg_acum[i][j] = 0.0;
     }
+    #pragma omp parallel for schedule(static)
     for (p = 0; p < s->nxyz; p++) {
         int jph = s->phase[p], a, b, c, e;
         double cg[81];
@@ -1183,6 +1196,8 @@ int orc_step(orc_state *s, orc_report *rep)
         #pragma omp parallel for schedule(dynamic, 64)
         for (p = 0; p < N; p++) s->nrit[p] = nr_voxel(s, p, s->conv_s + p, s->conv_e + p);
         for (p = 0; p < N; p++) {
+            /* the reference asserts on a NaN slip rate (constitutive.c:660,683) */
+            if (s->sig[9 * p] != s->sig[9 * p] || s->sig[9 * p + 8] != s->sig[9 * p + 8]) return 3001;
             ee += s->conv_e[p];
             es += s->conv_s[p];
             nrtot += (long)s->nrit[p];

@@ -373,6 +373,8 @@ struct srx_ctx {
     // timing
     bool timing = false;
     std::vector<hipEvent_t> evpool;
+    std::vector<int> evslot;
+    size_t evused = 0;
     struct KT { const char *name; double ms; int calls; };
     KT kt[8];
     int nkt = 0;
@@ -479,6 +481,9 @@ int setup(srx_ctx *c) {
         D.G = P.G;
         D.c4 = d.C_4[p]; D.c5 = d.C_5[p]; D.c6 = d.C_6[p];
         D.expmqb = std::exp(-1.0 * (d.Q_bulk[p] * MY_J_IN_EV / kT));
+        D.c1Gb3 = c1 * P.G * P.b * 1.0E-3;
+        D.k_tcut = (kT / MY_J_IN_EV) / c2 / c3 / P.b / P.b * 0.1;
+        D.k_g0 = 1E-3 * 1.0E10 * c2 * P.b * D.expmq;
     }
     c->hc.dt = d.TimeStep;
     c->hc.nu = d.ElastConst_PhaseI[3];
@@ -731,23 +736,28 @@ void get_smacro(srx_ctx *c, const double sum6[6], const double sum6_ph1[6]) {
     c->s_vm1 = std::sqrt(3. / 2. * s);
 }
 
-// event-timed launch helper
+// event-timed launch helper: events come from a pool that is reused every increment
 struct Timer {
     srx_ctx *c;
     int slot;
-    hipEvent_t a = nullptr, b = nullptr;
+    size_t ia = 0;
     Timer(srx_ctx *cc, int s) : c(cc), slot(s) {
         if (!c->timing) return;
-        hipEventCreate(&a);
-        hipEventCreate(&b);
-        hipEventRecord(a, c->stream);
+        ia = c->evused;
+        if (c->evused + 2 > c->evpool.size()) {
+            for (int k = 0; k < 256; ++k) {
+                hipEvent_t e;
+                hipEventCreate(&e);
+                c->evpool.push_back(e);
+            }
+        }
+        c->evused += 2;
+        set_launch_events(c->evpool[ia], c->evpool[ia + 1]);
     }
     ~Timer() {
         if (!c->timing) return;
-        hipEventRecord(b, c->stream);
-        c->evpool.push_back(a);
-        c->evpool.push_back(b);
-        c->evpool.push_back((hipEvent_t)(intptr_t)slot);
+        set_launch_events(nullptr, nullptr);
+        c->evslot.push_back(slot);
     }
 };
 
@@ -756,17 +766,14 @@ const char *KT_NAMES[8] = {"fwd_z", "fwd_y", "green_x", "inv_y", "inv_z", "nr", 
 void collect_timing(srx_ctx *c) {
     if (!c->timing) return;
     hipStreamSynchronize(c->stream);
-    for (size_t i = 0; i + 2 < c->evpool.size() + 1 && i + 2 < c->evpool.size() + 2; i += 3) {
-        if (i + 2 >= c->evpool.size()) break;
+    for (size_t i = 0; i < c->evslot.size(); ++i) {
         float ms = 0;
-        hipEventElapsedTime(&ms, c->evpool[i], c->evpool[i + 1]);
-        const int slot = (int)(intptr_t)c->evpool[i + 2];
-        c->kt[slot].ms += ms;
-        c->kt[slot].calls += 1;
-        hipEventDestroy(c->evpool[i]);
-        hipEventDestroy(c->evpool[i + 1]);
+        hipEventElapsedTime(&ms, c->evpool[2 * i], c->evpool[2 * i + 1]);
+        c->kt[c->evslot[i]].ms += ms;
+        c->kt[c->evslot[i]].calls += 1;
     }
-    c->evpool.clear();
+    c->evslot.clear();
+    c->evused = 0;
 }
 
 int sync_read(srx_ctx *c, int nvals) {
@@ -828,6 +835,8 @@ int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_
         c->workdir = p == std::string::npos ? std::string(".") : d.substr(0, p);
     }
     c->timing = c->opt.reserved[0] != 0;
+    for (int k = 0; k < 8; ++k) c->kt[k] = {KT_NAMES[k], 0.0, 0};
+    c->nkt = 8;
     int rc = read_deck(deck, c->dk);
     if (!rc) rc = setup(c);
     if (rc) {
@@ -856,10 +865,6 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     srx_step_report r;
     std::memset(&r, 0, sizeof(r));
     if (c->ferr && c->nsteps != 1) std::fprintf(c->ferr, "STEP = %d\n", c->istep);
-    if (c->timing) {
-        for (int k = 0; k < 8; ++k) c->kt[k] = {KT_NAMES[k], 0.0, 0};
-        c->nkt = 8;
-    }
 
     // evolution.c:1115-1137
     double add[9], set[9];
@@ -1152,6 +1157,7 @@ void srx_close(srx_ctx *c) {
         if (c->tw[i]) hipFree(c->tw[i]);
     if (c->dc) hipFree(c->dc);
     if (c->hred) hipHostFree(c->hred);
+    for (hipEvent_t e : c->evpool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }

@@ -14,10 +14,29 @@
 #include "srx_dev.hpp"
 
 #include <hip/hip_runtime.h>
+#include <hip/hip_ext.h>
 #include <cmath>
 #include <cstdio>
 
 namespace srx {
+
+// Optional start/stop events for the next launch: when set, the launch goes
+// through hipExtLaunchKernelGGL, which stamps the events on the kernel's own
+// dispatch packet.
+static thread_local hipEvent_t g_ev_start = nullptr, g_ev_stop = nullptr;
+void set_launch_events(hipEvent_t a, hipEvent_t b) {
+    g_ev_start = a;
+    g_ev_stop = b;
+}
+#define SRX_LAUNCH(kernel, grid, block, shmem, stream, ...)                                              \
+    do {                                                                                                 \
+        if (g_ev_start) {                                                                                \
+            hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, g_ev_start, g_ev_stop, 0, __VA_ARGS__); \
+            g_ev_start = g_ev_stop = nullptr;                                                            \
+        } else {                                                                                         \
+            hipLaunchKernelGGL(kernel, grid, block, shmem, stream, __VA_ARGS__);                         \
+        }                                                                                                \
+    } while (0)
 
 // ---------------------------------------------------------------------------
 // complex helpers
@@ -548,13 +567,16 @@ __device__ __forceinline__ void sprime(const double a[9], const double a0[9], co
 }
 
 // ---------------------------------------------------------------------------
-// per-system constants of StrainRate_Orowan (constitutive.c:556-633)
+// constitutive helpers (constitutive.c:556-684)
 // ---------------------------------------------------------------------------
-struct SysConst {
-    double tp[NSYS];    // tau_pass (with the current rho_m)
-    double inv[NSYS];   // 1/(tau_0 + tau_cut)
-    double g0[NSYS];    // gamma_0
-};
+// 1/x: hardware reciprocal refined by two Newton steps (correctly rounded to a
+// few ulp; the IEEE division sequence is several times longer).
+__device__ __forceinline__ double rcp(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
 
 // forest (chi1) and parallel (chi) densities of system s (constitutive.c:562-577); the
 // tables are zero beyond nsys, so the fixed-length sums equal the reference's.
@@ -563,24 +585,36 @@ __device__ __forceinline__ void rho_pf(const DevPhase &P, const double rs[NSYS],
     rF = 0.0;
 #pragma unroll
     for (int j = 0; j < NSYS; ++j) {
-        rF += P.chi1[s][j] * (rs[j] + 0.0);
-        rP += P.chi[s][j] * (rs[j] + 0.0);
+        rF += P.chi1[s][j] * rs[j];
+        rP += P.chi[s][j] * rs[j];
     }
+}
+
+// mobile density (constitutive.c:632)
+__device__ __forceinline__ double rho_mobile(const DevPhase &P, double rP, double rF) {
+    return P.rhom_k * sqrt(rP * rF) * 160.2177;
+}
+
+// tau_pass with a given mobile density (587), 1/(tau_0 + tau_cut) (630, 639) and gamma_0 (633)
+__device__ __forceinline__ void sys_const(const DevPhase &P, double rP, double rF, double rm_pass, double rmn,
+                                          double &tp, double &inv, double &g0) {
+    const double sF = sqrt(rF);
+    tp = P.c1Gb3 * sqrt(fabs(rP + rm_pass));
+    inv = rcp(P.tau0 + P.k_tcut * sF);
+    g0 = P.k_g0 * rmn * rcp(sF);
 }
 
 // slip rate and its derivative, FORWARD_PLUS_BACKWARD (constitutive.c:634-684):
 //   g  = g0 sinh((|rss|-tp)/(tau0+tau_cut)) sign(rss),  g2 = g0 cosh(..)/(tau0+tau_cut), for |rss| > tp
+// Branch free: every system evaluates the exponential, inactive ones are masked.
 __device__ __forceinline__ void slip_rate(double rss, double tp, double inv, double g0, double &g, double &g2) {
     const double ar = fabs(rss);
-    g = 0.0;
-    g2 = 0.0;
-    if (ar > tp) {
-        const double xx = (ar - tp) * inv;
-        const double e = exp(xx), ei = 1.0 / e;
-        g = g0 * (0.5 * (e - ei));
-        if (!(rss > 0)) g = -g;
-        g2 = g0 * (0.5 * (e + ei)) * inv;
-    }
+    const bool act = ar > tp;
+    const double e = exp(act ? (ar - tp) * inv : 0.0);
+    const double ei = rcp(e);
+    const double gg = act ? g0 * (0.5 * (e - ei)) : 0.0;
+    g = (rss > 0) ? gg : -gg;
+    g2 = act ? g0 * (0.5 * (e + ei)) * inv : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -589,9 +623,11 @@ __device__ __forceinline__ void slip_rate(double rss, double tp, double inv, dou
 // S' the grain compliance and D = d(edot)/d(sig):
 //   Rt(sig) = Sc (sig - lam) + eps_p + edot(sig) dt + S' sig - eps_c  ( = S066 R )
 //   K = Sc + S' + D dt  ( = S066 J, symmetric positive definite)
-// so the Newton step K^-1 Rt equals the reference's J^-1 R.
+// so the Newton step K^-1 Rt equals the reference's J^-1 R; K is factored
+// LDL^T in place.  Stopping rule, iteration cap and the pre-update iterate
+// used for Edot and the errors are the reference's.
 // ---------------------------------------------------------------------------
-constexpr int NR_BLOCK = 128;
+constexpr int NR_BLOCK = 64;
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -611,74 +647,81 @@ struct NrResult {
     int bad;
 };
 
-__device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__restrict__ cst, const Fields &f,
-                                      int v, size_t N, int first_call, int rotated, NrResult &res) {
-    const double dt = cst->dt;
-    double a[9];
+// deviatoric stress block of A = Q^T S066 Q plus the compliance, packed
+__device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const double Q[5][5], double A[15]) {
+    double T1[5][5];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
-    double Q[5][5];
-    q5_from(a, Q);
-    double lam6[6], ec[6], ep[5];
+    for (int i = 0; i < 5; ++i)
 #pragma unroll
-    for (int k = 0; k < 6; ++k) lam6[k] = f.sig[k * N + v];
+        for (int j = 0; j < 5; ++j) {
+            double acc = 0.0;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) ep[k] = f.eps[k * N + v];
-    {
-        double d[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) d[k] = f.dg[k * N + v];
-        // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
-        const double s00 = d[0], s11 = d[4], s22 = d[8];
-        const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
-        ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
-        ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
-        ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
-        ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
-        ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
-        ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
-    }
-    double signorm = 0.0;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) signorm += lam6[k] * lam6[k];
-    signorm = sqrt(signorm);
-
-    // crystal frame: lam' = Q^T lam, q' = Q^T (eps_p - eps_c)
-    double lam[5], q[5];
-#pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double al = 0.0, aq = 0.0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) {
-            al += Q[k][i] * lam6[k];
-            aq += Q[k][i] * (ep[k] - ec[k]);
+            for (int k = 0; k < 5; ++k) acc += S[6 * i + k] * Q[k][j];
+            T1[i][j] = acc;
         }
-        lam[i] = al;
-        q[i] = aq;
-    }
-    const double lamh = lam6[5], qh = -ec[5];
-    // A = Sc + S', r0 = q - Sc lam
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = i; j < 5; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc += Q[k][i] * T1[k][j];
+            A[ps(i, j)] = acc;
+        }
+}
+
+template <bool FIRST, bool ROT, int PH>
+__device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
+                                         double *lds, NrResult &res) {
+    // compile-time phase index: every material constant is a scalar load at a fixed offset
+    const DevPhase &P = cst->ph[PH];
+    const double dt = cst->dt;
+    double lam[5], q[5], lamh, qh, rsig;
     double A[15], r0[5];
     {
-        double T1[5][5];   // S066 Q
+        double a[9];
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+        double Q[5][5];
+        q5_from(a, Q);
+        double lam6[6], ec[6], ep[5];
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                double acc = 0.0;
+        for (int k = 0; k < 6; ++k) lam6[k] = f.sig[k * N + v];
 #pragma unroll
-                for (int k = 0; k < 5; ++k) acc += cst->S066[i][k] * Q[k][j];
-                T1[i][j] = acc;
+        for (int k = 0; k < 5; ++k) ep[k] = f.eps[k * N + v];
+        {
+            double d[9];
+#pragma unroll
+            for (int k = 0; k < 9; ++k) d[k] = f.dg[k * N + v];
+            // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
+            const double s00 = d[0], s11 = d[4], s22 = d[8];
+            const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
+            ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
+            ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
+            ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
+            ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
+            ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
+            ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
+        }
+        double sn = 0.0;
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sn += lam6[k] * lam6[k];
+        rsig = 1.0 / sqrt(sn);
+        // crystal frame: lam' = Q^T lam, q' = Q^T (eps_p - eps_c)
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double al = 0.0, aq = 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                al += Q[k][i] * lam6[k];
+                aq += Q[k][i] * (ep[k] - ec[k]);
             }
-#pragma unroll
-        for (int i = 0; i < 5; ++i)
-#pragma unroll
-            for (int j = i; j < 5; ++j) {
-                double acc = 0.0;
-#pragma unroll
-                for (int k = 0; k < 5; ++k) acc += Q[k][i] * T1[k][j];
-                A[ps(i, j)] = acc;
-            }
+            lam[i] = al;
+            q[i] = aq;
+        }
+        lamh = lam6[5];
+        qh = -ec[5];
+        sc_rotate(&cst->S066[0][0], Q, A);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             double acc = q[i];
@@ -686,8 +729,9 @@ __device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__re
             for (int j = 0; j < 5; ++j) acc -= A[ps(i, j)] * lam[j];
             r0[i] = acc;
         }
-        if (rotated) {
+        if (ROT) {
             double a0[9], Sp[15];
+#pragma unroll
             for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
             sprime(a, a0, P.sinv, Sp);
 #pragma unroll
@@ -699,11 +743,13 @@ __device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__re
     }
     const double Shh = cst->S066[5][5];
     const double Ah = Shh + P.sinv[5];
+    const double rAh = 1.0 / Ah;
     const double r0h = qh - Shh * lamh;
 
-    // per-system constants (constitutive.c:556-633); GND densities are zero
-    SysConst sc;
-    double tpo[NSYS];
+    // per-system constants, kept in LDS (lane-major, conflict free); the first call of an
+    // increment uses the lagged rho_m in tau_pass
+    double *tp = lds + 0 * NSYS * NR_BLOCK + threadIdx.x, *inv = lds + 1 * NSYS * NR_BLOCK + threadIdx.x;
+    double *g0 = lds + 2 * NSYS * NR_BLOCK + threadIdx.x, *tpo = lds + 3 * NSYS * NR_BLOCK + threadIdx.x;
     {
         const int nsys = P.nsys;
         double rs[NSYS];
@@ -714,54 +760,54 @@ __device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__re
             if (s < nsys) {
                 double rP, rF;
                 rho_pf(P, rs, s, rP, rF);
-                const double rmn = P.rhom_k * sqrt(rP * rF) * 160.2177;
-                const double tcut = P.kTf * sqrt(rF) / P.c2 / P.c3 / P.b / P.b * 0.1;
-                sc.g0[s] = 1E-3 * rmn * 1.0E10 * P.c2 * P.b * P.expmq / sqrt(rF);
-                sc.tp[s] = P.c1Gb * sqrt(fabs(rP + rmn)) * 1.0E-3;
-                sc.inv[s] = 1.0 / (P.tau0 + tcut);
-                tpo[s] = sc.tp[s];
-                if (first_call) {
+                const double rmn = rho_mobile(P, rP, rF);
+                double t, iv, g;
+                sys_const(P, rP, rF, rmn, rmn, t, iv, g);
+                tp[s * NR_BLOCK] = t;
+                inv[s * NR_BLOCK] = iv;
+                g0[s * NR_BLOCK] = g;
+                if (FIRST) {
                     const double rmo = f.rho_m[s * N + v];
-                    tpo[s] = P.c1Gb * sqrt(fabs(rP + rmo)) * 1.0E-3;
+                    tpo[s * NR_BLOCK] = P.c1Gb3 * sqrt(fabs(rP + rmo));
                     f.rho_m_old[s * N + v] = rmo;
                     f.rho_m[s * N + v] = rmn;
                 }
             } else {
-                sc.g0[s] = 0.0; sc.tp[s] = 1e300; sc.inv[s] = 0.0; tpo[s] = 1e300;
+                g0[s * NR_BLOCK] = 0.0; tp[s * NR_BLOCK] = 1e300; inv[s * NR_BLOCK] = 0.0;
+                if (FIRST) tpo[s * NR_BLOCK] = 1e300;
             }
         }
     }
 
-    double sg[5], sh = lamh, pre[5] = {0, 0, 0, 0, 0}, preh = lamh, ed[5] = {0, 0, 0, 0, 0};
+    double sg[5], sh = lamh, pre[5], preh = lamh, ed[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) sg[i] = lam[i];
+    for (int i = 0; i < 5; ++i) { sg[i] = lam[i]; pre[i] = lam[i]; ed[i] = 0.0; }
     int it = 0;
     double erral = 10 * 1E-10;
     while (it < 500 && erral > 1E-10) {
         ++it;
-        const bool lag = first_call && it == 1;
-        double D[15];
+        double K[15];
 #pragma unroll
-        for (int k = 0; k < 15; ++k) D[k] = 0.0;
+        for (int k = 0; k < 15; ++k) K[k] = A[k];
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
-#pragma unroll
+        const double *tpu = (FIRST && it == 1) ? tpo : tp;
+#pragma unroll 2
         for (int s = 0; s < NSYS; ++s) {
             const double *p = P.p[s];
             const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
-            const double tp = lag ? tpo[s] : sc.tp[s];
             double g, g2;
-            slip_rate(rss, tp, sc.inv[s], sc.g0[s], g, g2);
+            slip_rate(rss, tpu[s * NR_BLOCK], inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
+            const double g2dt = g2 * dt;
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
                 ed[i] += p[i] * g;
-                const double gp = g2 * p[i];
+                const double gp = g2dt * p[i];
 #pragma unroll
-                for (int j = i; j < 5; ++j) D[ps(i, j)] += gp * p[j];
+                for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
             }
         }
-        // residual and tangent
-        double R[5], K[15];
+        double R[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             double acc = r0[i] + ed[i] * dt;
@@ -769,74 +815,64 @@ __device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__re
             for (int j = 0; j < 5; ++j) acc += A[ps(i, j)] * sg[j];
             R[i] = acc;
         }
-#pragma unroll
-        for (int k = 0; k < 15; ++k) K[k] = A[k] + D[k] * dt;
-        // LDL^T factorisation and solve of K d = R
-        double Ld[5], L[5][5];
+        // LDL^T in place: K[ps(j,j)] <- 1/D_j, K[ps(i,j)] <- L_ij (i > j)
 #pragma unroll
         for (int j = 0; j < 5; ++j) {
             double dj = K[ps(j, j)];
 #pragma unroll
-            for (int k = 0; k < j; ++k) dj -= L[j][k] * L[j][k] * Ld[k];
-            Ld[j] = dj;
-            const double rj = 1.0 / dj;
+            for (int k = 0; k < j; ++k) dj -= K[ps(j, k)] * K[ps(j, k)] * K[ps(k, k)];
+            // K[ps(k,k)] holds D_k until it is inverted below
+            const double rj = rcp(dj);
 #pragma unroll
             for (int i = j + 1; i < 5; ++i) {
                 double acc = K[ps(i, j)];
 #pragma unroll
-                for (int k = 0; k < j; ++k) acc -= L[i][k] * L[j][k] * Ld[k];
-                L[i][j] = acc * rj;
+                for (int k = 0; k < j; ++k) acc -= K[ps(i, k)] * K[ps(j, k)] * K[ps(k, k)];
+                K[ps(i, j)] = acc * rj;
             }
+            K[ps(j, j)] = dj;
         }
         double y[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             double acc = R[i];
 #pragma unroll
-            for (int k = 0; k < i; ++k) acc -= L[i][k] * y[k];
+            for (int k = 0; k < i; ++k) acc -= K[ps(i, k)] * y[k];
             y[i] = acc;
         }
 #pragma unroll
-        for (int i = 0; i < 5; ++i) y[i] /= Ld[i];
-        double d5[5];
+        for (int i = 0; i < 5; ++i) y[i] *= rcp(K[ps(i, i)]);
 #pragma unroll
         for (int i = 4; i >= 0; --i) {
-            double acc = y[i];
 #pragma unroll
-            for (int k = i + 1; k < 5; ++k) acc -= L[k][i] * d5[k];
-            d5[i] = acc;
+            for (int k = i + 1; k < 5; ++k) y[i] -= K[ps(k, i)] * y[k];
         }
-        const double Rh = Ah * sh + r0h;
-        const double dh = Rh / Ah;
+        const double dh = (Ah * sh + r0h) * rAh;
         double cnr = dh * dh;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             pre[i] = sg[i];
-            sg[i] -= d5[i];
-            cnr += d5[i] * d5[i];
+            sg[i] -= y[i];
+            cnr += y[i] * y[i];
         }
         preh = sh;
         sh -= dh;
-        erral = sqrt(cnr) / signorm;
+        erral = sqrt(cnr) * rsig;
     }
-    // outputs in the sample frame; errors of evolution.c:381-391 (rotation invariant)
-    double cs2 = 0.0;
+    // errors of evolution.c:381-391 (rotation invariant)
+    double cs2 = (sh - lamh) * (sh - lamh);
 #pragma unroll
-    for (int i = 0; i < 5; ++i) {
-        double acc = 0.0;
-#pragma unroll
-        for (int k = 0; k < 5; ++k) acc += Q[i][k] * sg[k];
-        res.sig[i] = acc;
-        cs2 += (acc - lam6[i]) * (acc - lam6[i]);
-    }
-    res.sig[5] = sh;
-    cs2 += (sh - lamh) * (sh - lamh);
+    for (int i = 0; i < 5; ++i) cs2 += (sg[i] - lam[i]) * (sg[i] - lam[i]);
     res.cs = sqrt(cs2);
-    // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre
     {
+        double a[9];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+        // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre
         double Sp[15];
-        if (rotated) {
+        if (ROT) {
             double a0[9];
+#pragma unroll
             for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
             sprime(a, a0, P.sinv, Sp);
         } else {
@@ -856,7 +892,18 @@ __device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__re
         const double eh = qh + P.sinv[5] * preh;
         ce2 += eh * eh;
         res.ce = sqrt(ce2);
+        // stress back to the sample frame
+        double Q[5][5];
+        q5_from(a, Q);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < 5; ++k) acc += Q[i][k] * sg[k];
+            res.sig[i] = acc;
+        }
     }
+    res.sig[5] = sh;
 #pragma unroll
     for (int k = 0; k < 5; ++k) f.sig_pre[k * N + v] = pre[k];
     res.it = it;
@@ -866,12 +913,14 @@ __device__ __forceinline__ void nr_voxel(const DevPhase &P, const DevConst *__re
     res.bad = bad;
 }
 
-__global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, int first_call,
-                                                 int rotated, double *__restrict__ partials) {
+template <bool FIRST, bool ROT>
+__global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
+                                                 double *__restrict__ partials) {
     const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
+    __shared__ double lds[(FIRST ? 4 : 3) * NSYS * NR_BLOCK];
     NrResult r;
     r.cs = 0.0; r.ce = 0.0; r.it = 0; r.bad = 0;
 #pragma unroll
@@ -882,19 +931,22 @@ __global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cs
         const int first = __ffsll((unsigned long long)todo) - 1;
         const int pu = __builtin_amdgcn_readfirstlane(__shfl(ph, first, 64));
         const bool mine = valid && ph == pu;
-        if (mine && cst->ph[pu - 1].xtal) nr_voxel(cst->ph[pu - 1], cst, f, v, N, first_call, rotated, r);
+        if (mine) {
+            if (pu == 1) {
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0>(cst, f, v, N, lds, r);
+            } else {
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1>(cst, f, v, N, lds, r);
+            }
+        }
         todo &= ~__ballot(mine);
     }
-    const bool xtal = valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal;
     if (valid) {
         // gas voxels: Sig = Edot = 0 (evolution.c:418-423)
 #pragma unroll
         for (int k = 0; k < 6; ++k) f.sig[k * N + v] = r.sig[k];
         f.nr_it[v] = r.it;
-        (void)xtal;
     }
     // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan
-    __shared__ double red[NR_BLOCK / 64][NR_NPART];
     double vals[NR_NPART];
     vals[0] = r.cs;
     vals[1] = r.ce;
@@ -907,41 +959,33 @@ __global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cs
     vals[15] = (double)r.it;
     vals[16] = (double)r.bad;
     vals[17] = 0.0;
-    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
 #pragma unroll
     for (int k = 0; k < NR_NPART; ++k) {
-        const double s = (k == 15) ? wave_max(vals[k]) : wave_sum(vals[k]);
-        if (lane == 0) red[wid][k] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < NR_NPART) {
-        const int k = threadIdx.x;
-        double acc = red[0][k];
-        for (int w = 1; w < NR_BLOCK / 64; ++w) acc = (k == 15) ? fmax(acc, red[w][k]) : acc + red[w][k];
-        partials[(size_t)blockIdx.x * NR_NPART + k] = acc;
+        const double sred = (k == 15) ? wave_max(vals[k]) : wave_sum(vals[k]);
+        if (lane == 0) partials[(size_t)k * gridDim.x + blockIdx.x] = sred;
     }
 }
 
-// fixed-order reduction of block partials: out[k] = sum (or max for k in maxmask)
+// fixed-order reduction of block partials stored component-major:
+// out[k] = sum over blocks of part[k][b] (max for k in maxmask); one block per k.
 __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
                                                 unsigned long long maxmask, double *__restrict__ out) {
     __shared__ double sh[256];
-    for (int k = 0; k < nvals; ++k) {
-        const bool ismax = (maxmask >> k) & 1ull;
-        double acc = ismax ? -1e308 : 0.0;
-        for (int b = threadIdx.x; b < nblocks; b += 256) {
-            const double x = part[(size_t)b * nvals + k];
-            acc = ismax ? fmax(acc, x) : acc + x;
-        }
-        sh[threadIdx.x] = acc;
-        __syncthreads();
-        for (int s = 128; s > 0; s >>= 1) {
-            if (threadIdx.x < s) sh[threadIdx.x] = ismax ? fmax(sh[threadIdx.x], sh[threadIdx.x + s]) : sh[threadIdx.x] + sh[threadIdx.x + s];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[k] = sh[0];
+    const int k = blockIdx.x;
+    const bool ismax = (maxmask >> k) & 1ull;
+    const double *row = part + (size_t)k * nblocks;
+    double acc = ismax ? -1e308 : 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += 256) acc = ismax ? fmax(acc, row[b]) : acc + row[b];
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int st = 128; st > 0; st >>= 1) {
+        if (threadIdx.x < st)
+            sh[threadIdx.x] = ismax ? fmax(sh[threadIdx.x], sh[threadIdx.x + st]) : sh[threadIdx.x] + sh[threadIdx.x + st];
         __syncthreads();
     }
+    if (threadIdx.x == 0) out[k] = sh[0];
+    (void)nvals;
 }
 
 // increment prologue (evolution.c:1115-1128): dg0 = dg; dg += Udot dt (or = set9 under creep)
@@ -968,8 +1012,10 @@ __global__ void k_prologue(Fields f, int nvox, double a0, double a1, double a2, 
 // ---------------------------------------------------------------------------
 constexpr int END_BLOCK = 128;
 
-__device__ void end_voxel(const DevPhase &P, const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                          int lagged_possible, int do_orient, int istep, double *vals) {
+template <int PH>
+__device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
+                                          int lagged_possible, int do_orient, int istep, double *vals) {
+    const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
     double a[9];
 #pragma unroll
@@ -1014,10 +1060,8 @@ __device__ void end_voxel(const DevPhase &P, const DevConst *__restrict__ cst, c
             rho_pf(P, rs, s, rP, rF);
             const double rmn = f.rho_m[s * N + v];
             const double rmo = f.rho_m_old[s * N + v];
-            const double tcut = P.kTf * sqrt(rF) / P.c2 / P.c3 / P.b / P.b * 0.1;
-            const double g0 = 1E-3 * rmn * 1.0E10 * P.c2 * P.b * P.expmq / sqrt(rF);
-            const double tp = P.c1Gb * sqrt(fabs(rP + (lagged ? rmo : rmn))) * 1.0E-3;
-            const double inv = 1.0 / (P.tau0 + tcut);
+            double tp, inv, g0;
+            sys_const(P, rP, rF, lagged ? rmo : rmn, rmn, tp, inv, g0);
             const double *p = P.p[s];
             const double rss = p[0] * pre[0] + p[1] * pre[1] + p[2] * pre[2] + p[3] * pre[3] + p[4] * pre[4];
             double g, g2;
@@ -1155,7 +1199,13 @@ __global__ void __launch_bounds__(END_BLOCK) k_end(const DevConst *__restrict__ 
         const int first = __ffsll((unsigned long long)todo) - 1;
         const int pu = __builtin_amdgcn_readfirstlane(__shfl(ph, first, 64));
         const bool mine = valid && ph == pu;
-        if (mine && cst->ph[pu - 1].xtal) end_voxel(cst->ph[pu - 1], cst, f, v, N, lagged_possible, do_orient, istep, vals);
+        if (mine) {
+            if (pu == 1) {
+                if (cst->ph[0].xtal) end_voxel<0>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
+            } else {
+                if (cst->ph[1].xtal) end_voxel<1>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
+            }
+        }
         todo &= ~__ballot(mine);
     }
     const bool xtal = valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal;
@@ -1179,7 +1229,7 @@ __global__ void __launch_bounds__(END_BLOCK) k_end(const DevConst *__restrict__ 
         const int k = threadIdx.x;
         double acc = red[0][k];
         for (int w = 1; w < END_BLOCK / 64; ++w) acc += red[w][k];
-        partials[(size_t)blockIdx.x * END_NPART + k] = acc;
+        partials[(size_t)k * gridDim.x + blockIdx.x] = acc;
     }
 }
 
@@ -1228,11 +1278,11 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
     const double *m = mean6 ? mean6 : z6;
     if (prec == 0) {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
-        hipLaunchKernelGGL(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
+        SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     } else {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
-        hipLaunchKernelGGL(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
+        SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, m[0], m[1], m[2], m[3], m[4], m[5]);
     }
     check_launch("fwd_z");
@@ -1246,11 +1296,11 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
         size_t sm = (size_t)TX * n[1] * sizeof(cx<double>);
-        hipLaunchKernelGGL(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
+        SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX);
     } else {
         size_t sm = (size_t)TX * n[1] * sizeof(cx<float>);
-        hipLaunchKernelGGL(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
+        SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX);
     }
     check_launch("fft_y");
@@ -1267,11 +1317,11 @@ static void launch_green_generic(hipStream_t s, int prec, void *specC, const int
     const double *d = dc6 ? dc6 : z6;
     if (prec == 0) {
         size_t sm = (size_t)slots * TY * n[0] * sizeof(cx<double>);
-        hipLaunchKernelGGL((k_green_x<double, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
+        SRX_LAUNCH((k_green_x<double, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
                            px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
     } else {
         size_t sm = (size_t)slots * TY * n[0] * sizeof(cx<float>);
-        hipLaunchKernelGGL((k_green_x<float, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
+        SRX_LAUNCH((k_green_x<float, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
                            px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
     }
     check_launch("green_x");
@@ -1289,12 +1339,12 @@ void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double 
     const double *a = ddg9;
     if (prec == 0) {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
-        hipLaunchKernelGGL(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
+        SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
                            a[6], a[7], a[8]);
     } else {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
-        hipLaunchKernelGGL(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
+        SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
                            a[7], a[8]);
     }
@@ -1306,27 +1356,30 @@ int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
                double *partials, int) {
-    hipLaunchKernelGGL(k_nr, dim3(nr_block_count(nvox)), dim3(NR_BLOCK), 0, s, cst, f, nvox, first_call, rotated,
-                       partials);
+    const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
+    if (first_call && rotated) SRX_LAUNCH((k_nr<true, true>), g, b, 0, s, cst, f, nvox, partials);
+    else if (first_call) SRX_LAUNCH((k_nr<true, false>), g, b, 0, s, cst, f, nvox, partials);
+    else if (rotated) SRX_LAUNCH((k_nr<false, true>), g, b, 0, s, cst, f, nvox, partials);
+    else SRX_LAUNCH((k_nr<false, false>), g, b, 0, s, cst, f, nvox, partials);
     check_launch("nr");
 }
 
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out) {
     const unsigned long long mask = nvals == NR_NPART ? (1ull << 15) : 0ull;
-    hipLaunchKernelGGL(k_reduce, dim3(1), dim3(256), 0, s, partials, nblocks, nvals, mask, out);
+    SRX_LAUNCH(k_reduce, dim3(nvals), dim3(256), 0, s, partials, nblocks, nvals, mask, out);
     check_launch("reduce");
 }
 
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int creep, const double *st) {
     const int B = 256;
-    hipLaunchKernelGGL(k_prologue, dim3((nvox + B - 1) / B), dim3(B), 0, s, f, nvox, a[0], a[1], a[2], a[3], a[4],
+    SRX_LAUNCH(k_prologue, dim3((nvox + B - 1) / B), dim3(B), 0, s, f, nvox, a[0], a[1], a[2], a[3], a[4],
                        a[5], a[6], a[7], a[8], creep, st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7], st[8]);
     check_launch("prologue");
 }
 
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int,
                 int do_orient, int, int istep, double *partials) {
-    hipLaunchKernelGGL(k_end, dim3(end_block_count(nvox)), dim3(END_BLOCK), 0, s, cst, f, nvox, lagged_possible,
+    SRX_LAUNCH(k_end, dim3(end_block_count(nvox)), dim3(END_BLOCK), 0, s, cst, f, nvox, lagged_possible,
                        do_orient, istep, partials);
     check_launch("end");
 }
@@ -1338,11 +1391,11 @@ void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
-        hipLaunchKernelGGL(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
+        SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     } else {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
-        hipLaunchKernelGGL(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
+        SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     }
     check_launch("fwd_z(test)");
@@ -1359,12 +1412,12 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
-        hipLaunchKernelGGL(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
+        SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
                            0., 0.);
     } else {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
-        hipLaunchKernelGGL(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
+        SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
                            0.);
     }

@@ -31,6 +31,9 @@ struct DevPhase {
     double expmq;         // exp(-q_slip)
     double G, c4, c5, c6;
     double expmqb;        // exp(-q_bulk)
+    double c1Gb3;         // c_1*G*b*1e-3: tau_pass = c1Gb3*sqrt(|rho_P + rho_m|)
+    double k_tcut;        // kT/MY_J_IN_EV/c2/c3/b^2*0.1: tau_cut = k_tcut*sqrt(rho_f)
+    double k_g0;          // 1e-3*TRIAL_FREQ*c2*b*exp(-q_slip): gamma_0 = k_g0*rho_m/sqrt(rho_f)
 };
 
 struct DevConst {
@@ -76,6 +79,7 @@ constexpr int NR_NPART = 18;   // conv_s, conv_e, sig6[6], sig6_ph1[6], nr_sum, 
 constexpr int END_NPART = 24;  // eps6[6], edot6[6], rhoF, rhoP, rhos, rhom, rhodip, rhofor, rhoinp, rsl, rlc, nan, pad..
 
 bool make_plan(int n, FftPlan &p);
+void set_launch_events(hipEvent_t start, hipEvent_t stop);
 
 // ---- launch wrappers (kernels.hip) -------------------------------------
 // Spectrum buffers: specB [c][kz][x][y], specC [c][kz][y][x]; T = double or float.
