@@ -84,6 +84,8 @@ def lib():
                                       P(C.c_double), P(C.c_double)]
         L.orc_strain_rate.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double), P(C.c_double)]
         L.orc_nr_voxel.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
+        L.orc_update_schmid.argtypes = [C.c_void_p]
+        L.orc_get_schmid.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
         _lib = L
     return _lib
 

@@ -870,6 +870,14 @@ static int nr_voxel(orc_state *s, long p, double *cs, double *ce)
 
 int orc_nr_voxel(orc_state *s, int idx, double *err_s, double *err_e) { return nr_voxel(s, idx, err_s, err_e); }
 
+void orc_update_schmid(orc_state *s) { update_schmid(s); }
+
+int orc_get_schmid(orc_state *s, int idx, double *out60)
+{
+    memcpy(out60, s->schm + 60 * (long)idx, 60 * sizeof(double));
+    return 0;
+}
+
 /* One k point of CalcGAMMA (init.c:542-592) contracted as evolution.c:1169-1183 */
 static void green_point(const double *C0, const double *nv, int dc, int ebc, const double *S0,
                         const double *ks, double *du)

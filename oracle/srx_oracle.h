@@ -97,6 +97,9 @@ void orc_green_point(const double *c0, const double *nvec, int dc, int elast_bc,
 int  orc_strain_rate(orc_state *s, int idx, const double *sig6, double *edot6, double *dedot);
 /* One NR solve (UpdateStress body) for voxel idx; returns NR iterations. */
 int  orc_nr_voxel(orc_state *s, int idx, double *err_s, double *err_e);
+/* update_schmid (kinematics.c:430-463) and the sample-frame Schmid tensors of a voxel */
+void orc_update_schmid(orc_state *s);
+int  orc_get_schmid(orc_state *s, int idx, double *out60);
 
 #ifdef __cplusplus
 }

@@ -406,7 +406,7 @@ bool grid_supported(const FftPlan pl[3], const int n[3]) {
         const int slots = ax == 0 ? 9 : 1;
         const int maxb = ax == 0 ? 6 : 4;
         if ((long)slots * p.n / std::max(mr, 1) > (long)maxb * 256) return false;
-        if ((long)slots * p.n * 16 > 65536) return false;
+        if ((long)slots * p.n * 16 > 48 * 1024) return false;   // one tile row within the LDS budget
     }
     return n[0] >= 1 && n[1] >= 1 && n[2] >= 1;
 }

@@ -1255,10 +1255,17 @@ bool make_plan(int n, FftPlan &p) {
     return m == 1;
 }
 
-// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK and t <= cap
-static int tile_for(const FftPlan &p, int per_tile_comps, int maxb, int cap) {
+// LDS budget of one FFT block (bytes)
+constexpr int FFT_LDS_MAX = 48 * 1024;
+
+// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*n complex values within
+// FFT_LDS_MAX and t <= cap
+static int tile_for(const FftPlan &p, int per_tile_comps, int maxb, int cap, int prec) {
     const int mr = min_radix(p);
     int t = (maxb * FFT_BLOCK * mr) / (p.n * per_tile_comps);
+    const int csz = prec == 0 ? 16 : 8;
+    const int tl = FFT_LDS_MAX / (per_tile_comps * p.n * csz);
+    if (t > tl) t = tl;
     if (t > cap) t = cap;
     if (t < 1) t = 1;
     return t;
@@ -1272,7 +1279,7 @@ static void check_launch(const char *name) {
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
                       const void *twz, const double *mean6) {
     // ncomp encoded in mean6 == nullptr? no: always the 6 stress comps here
-    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16);
+    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     const double z6[6] = {0, 0, 0, 0, 0, 0};
     const double *m = mean6 ? mean6 : z6;
@@ -1291,7 +1298,7 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c, const int n[3],
                   const FftPlan &py, const void *twy) {
     (void)sign;
-    const int TX = tile_for(py, 1, 4, n[0] < 16 ? n[0] : 16);
+    const int TX = tile_for(py, 1, 4, n[0] < 16 ? n[0] : 16, prec);
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
@@ -1310,7 +1317,7 @@ static void launch_green_generic(hipStream_t s, int prec, void *specC, const int
                                  const void *twx, const DevConst *cst, const double *dc6, int mode, int ncomp) {
     const int slots = mode == 1 ? 9 : ncomp;
     // MAXB 6 covers 9 slots of two 128-point pencils with a radix-2 stage
-    const int TY = tile_for(px, slots, 6, n[1] < 2 ? n[1] : 2);
+    const int TY = tile_for(px, slots, 6, n[1] < 2 ? n[1] : 2, prec);
     const int H = n[2] / 2 + 1;
     dim3 grid((n[1] + TY - 1) / TY, H);
     const double z6[6] = {0, 0, 0, 0, 0, 0};
@@ -1334,7 +1341,7 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
                              const void *twz, const double *ddg9, double divisor) {
-    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16);
+    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     const double *a = ddg9;
     if (prec == 0) {
@@ -1387,7 +1394,7 @@ void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, i
 // test path: forward FFT of ncomp real comps to layout C (x pass done), no Green
 void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int ncomp, void *specB, void *specC,
                                   const int n[3], const FftPlan *pl, void *const *tw) {
-    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16);
+    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
@@ -1408,7 +1415,7 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
                                   const int n[3], const FftPlan *pl, void *const *tw, double divisor) {
     launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp);
     launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1]);
-    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16);
+    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
         size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);

@@ -1,0 +1,84 @@
+"""The C-ABI library loads on a machine without a GPU, exports every symbol include/srx.h
+declares, and returns the reference's setup error codes (no HIP call happens before them)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import srx_amd
+from srx_amd import inputs
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    text = open(os.path.join(ROOT, "include", "srx.h")).read()
+    return sorted(set(re.findall(r"^\s*(?:int|void|const char \*)\s*(srx_\w+)\s*\(", text, re.M)))
+
+
+def test_library_exports_every_declared_symbol():
+    lib = C.CDLL(srx_amd.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+    assert set(srx_amd.EXPORTS) <= set(names)
+
+
+def test_strerror_and_default_options():
+    L = srx_amd.lib()
+    assert L.srx_strerror(1139).decode().startswith("error in reading the microstructure")
+    assert L.srx_strerror(srx_amd.SrxError.__init__ and 3001).decode() == "NaN in the constitutive update"
+    o = srx_amd.options()
+    assert o.precision == 0 and o.fixed_iters == 0
+
+
+def open_code(deck):
+    L = srx_amd.lib()
+    h = C.c_void_p()
+    o = srx_amd.options()
+    rc = L.srx_open(deck.encode(), os.path.dirname(deck).encode(), C.byref(o), C.byref(h))
+    assert rc != 0, "expected a setup error"
+    return rc
+
+
+def test_setup_error_codes(tmp_path):
+    inputs.make_case(str(tmp_path), n=(4, 4, 4), grains=2, seed=1, nsteps=1, timestep=0.1)
+    d = str(tmp_path)
+    bad = os.path.join(d, "bad116.in")
+    inputs.write_deck(bad, n=(4, 4, 4), extra={"VelGrad_BC_Flag": "1 1 0 1 1 1", "Stress_BC_Flag": "0 0 1 0 0 0"})
+    assert open_code(bad) == 116
+    bad = os.path.join(d, "bad114.in")
+    inputs.write_deck(bad, n=(4, 4, 4), extra={"Stress_BC_Flag": "1 1 1 0 0 0"})
+    assert open_code(bad) == 114
+    lines = open(os.path.join(d, "mesh.txt")).read().splitlines()
+    lines[3], lines[4] = lines[4], lines[3]
+    open(os.path.join(d, "mesh_swapped.txt"), "w").write("\n".join(lines) + "\n")
+    bad = os.path.join(d, "bad1139.in")
+    inputs.write_deck(bad, n=(4, 4, 4), mesh="mesh_swapped.txt")
+    assert open_code(bad) == 1139
+    assert open_code(os.path.join(d, "missing.in")) == 2001
+    bad = os.path.join(d, "bad_grid.in")
+    inputs.write_deck(bad, n=(7, 4, 4))
+    assert open_code(bad) == 2002
+    bad = os.path.join(d, "bad_sx.in")
+    inputs.write_deck(bad, n=(4, 4, 4), sx="nope.sx")
+    assert open_code(bad) == 2001
+    # slip system not orthogonal (init.c:195-197)
+    sx = open(os.path.join(d, "Cu_p.sx")).read().replace("1 1 -1\t\t0 1 1", "1 1 -1\t\t1 1 1", 1)
+    open(os.path.join(d, "bad.sx"), "w").write(sx)
+    bad = os.path.join(d, "bad7.in")
+    inputs.write_deck(bad, n=(4, 4, 4), sx="bad.sx")
+    assert open_code(bad) == 7
+    bad = os.path.join(d, "bad3002.in")
+    inputs.write_deck(bad, n=(4, 4, 4), update_flag=0)
+    assert open_code(bad) == 3002
+
+
+def test_bench_refuses_more_than_one_gpu():
+    import subprocess
+    import sys
+    r = subprocess.run([sys.executable, os.path.join(ROOT, "bench.py"), "--gpus", "2"], capture_output=True, text=True)
+    assert r.returncode == 2 and "one GPU" in r.stderr

@@ -1,5 +1,13 @@
-"""GPU solver parity against the CPU oracle through the C ABI (srx_open/srx_step/srx_get_field)."""
+"""GPU solver parity against the CPU oracle through the C ABI (srx_open/srx_step/srx_get_field).
+
+Tolerances: the GPU and the oracle evaluate the same algorithm in a different floating-point
+order (crystal-frame Newton with an LDL^T solve vs the reference's sample-frame LU inverse;
+radix-4 vs radix-2 FFTs), so per-voxel fields agree to ~1e-12 relative after a few increments.
+Tests with a fixed number of outer iterations compare at 1e-9 relative; converged runs use the
+deck's Err and compare at a few times Err (the iteration count may then differ by one).
+"""
 import os
+import shutil
 
 import numpy as np
 import pytest
@@ -8,6 +16,8 @@ import srx_amd
 from srx_amd import inputs
 
 pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_inputs")
 
 
 def make(tmp_path, n=(16, 16, 16), grains=8, seed=1, nsteps=4, **kw):
@@ -18,18 +28,140 @@ def rel(a, b):
     return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
 
 
+def compare_fields(g, o, tol, names=("sig", "disgrad", "eps", "edot", "rho_s", "rho_m", "gamdot", "tranmat",
+                                      "gam_acum", "rho_forest", "rho_inplane", "rho_dipole", "velgrad")):
+    for f in names:
+        e = rel(g.field(f), o.field(f))
+        assert e < tol, (f, e)
+
+
+def compare_reports(rg, ro, tol):
+    for k in ("s_vm", "s_vm1", "e_vm", "evmp", "d_vm", "dvmp", "rho_s_avg", "rho_m_avg", "rho_forest_avg",
+              "rho_inplane_avg", "rho_dipole_avg"):
+        assert abs(rg[k] - ro[k]) <= tol * max(abs(ro[k]), 1e-300), (k, rg[k], ro[k])
+    np.testing.assert_allclose(rg["sig_avg"], ro["sig_avg"], rtol=0, atol=tol * np.abs(ro["sig_avg"]).max())
+
+
 def test_fixed_iterations_match_oracle(tmp_path, oracle_lib):
     deck = make(tmp_path, nsteps=4)
     o = oracle_lib.Oracle(deck, fixed_iters=6)
     g = srx_amd.Solver(deck, fixed_iters=6)
     for step in range(4):
         ro, rg = o.step(), g.step()
-        print(step, ro["sig_avg"][8], rg["sig_avg"][8], ro["s_vm"], rg["s_vm"], ro["nr_iters_max"], rg["nr_iters_max"])
         assert rg["iters"] == ro["iters"] == 6
-        assert abs(rg["s_vm"] - ro["s_vm"]) <= 1e-9 * abs(ro["s_vm"])
-        for f in ("sig", "disgrad", "eps"):
-            e = rel(g.field(f), o.field(f))
-            assert e < 1e-9, (step, f, e)
-        for f in ("rho_s", "rho_m", "gamdot", "tranmat"):
-            e = rel(g.field(f), o.field(f))
-            assert e < 1e-8, (step, f, e)
+        assert rg["nr_iters_max"] == ro["nr_iters_max"]
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9)
+
+
+def test_converged_increments_match_oracle(tmp_path, oracle_lib):
+    """Increments iterated to the deck's Err = 2e-5, including istep >= 2 where the texture
+    (and with it the crystal frame of the Newton solve) evolves."""
+    deck = make(tmp_path, n=(16, 16, 16), grains=8, seed=4, nsteps=5)
+    o = oracle_lib.Oracle(deck)
+    g = srx_amd.Solver(deck)
+    for step in range(5):
+        ro, rg = o.step(), g.step()
+        assert rg["converged"] and ro["converged"]
+        assert abs(rg["iters"] - ro["iters"]) <= 1, (rg["iters"], ro["iters"])
+        compare_reports(rg, ro, 1e-4)
+    if rg["iters"] == ro["iters"]:
+        compare_fields(g, o, 1e-6)
+
+
+def test_reference_deck_two_crystalline_phases_2d(tmp_path, oracle_lib):
+    """The reference's own deck: 256x256x1 (a one-voxel z extent), both phases crystalline with
+    different slip-mode files, all six velocity components prescribed."""
+    for f in ("input", "Cu_p.sx", "Cu_p_hard.sx"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path / f)
+    n = (256, 256, 1)
+    eul, gid = inputs.voronoi(n, 12, seed=9)
+    phase = np.where(np.arange(n[0] * n[1]) // n[1] < 128, 1, 2)
+    inputs.write_mesh(str(tmp_path / "Parent_Twin_Hori.txt"), n, eul, gid, phase)
+    deck = str(tmp_path / "input")
+    o = oracle_lib.Oracle(deck, fixed_iters=3)
+    g = srx_amd.Solver(deck, fixed_iters=3)
+    for step in range(3):
+        ro, rg = o.step(), g.step()
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9, names=("sig", "disgrad", "eps", "rho_s", "rho_m", "gamdot", "tranmat"))
+
+
+def test_bicrystal_out888(tmp_path, oracle_lib):
+    """The 8x8x8 bicrystal of the fftw3 variant (CRLF mesh, y tension, stress-controlled 11 and
+    33, tau0xf = 0), run with the top-level reference's algorithm."""
+    for f in ("input", "Cu_p.sx", "out888.txt"):
+        shutil.copy(os.path.join(GOLD, "fftw3", f), tmp_path / f)
+    deck = str(tmp_path / "input")
+    o = oracle_lib.Oracle(deck, fixed_iters=8)
+    g = srx_amd.Solver(deck, fixed_iters=8)
+    for step in range(4):
+        ro, rg = o.step(), g.step()
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9)
+
+
+def test_elastbc0_dc_term(tmp_path, oracle_lib):
+    """ElastBC 0: the k = 0 Green operator is -S0 (init.c:558-569)."""
+    deck = make(tmp_path, n=(8, 8, 8), grains=3, seed=2, nsteps=3, elast_bc=0)
+    o = oracle_lib.Oracle(deck, fixed_iters=5)
+    g = srx_amd.Solver(deck, fixed_iters=5)
+    for step in range(3):
+        ro, rg = o.step(), g.step()
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9, names=("sig", "disgrad", "eps", "rho_s"))
+
+
+def test_mixed_precision_error_vs_fp64(tmp_path):
+    """FP32 FFT/Green inside the FP64 fixed point: macroscopic stress within 1e-4 relative of the
+    FP64 GPU path (the stated tolerance), per-voxel von Mises stress within 1e-3 relative."""
+    deck = make(tmp_path, n=(32, 32, 32), grains=20, seed=6, nsteps=4)
+    a = srx_amd.Solver(deck)
+    b = srx_amd.Solver(deck, precision=srx_amd.PREC_MIXED)
+    for step in range(4):
+        ra, rb = a.step(), b.step()
+        sa, sb = np.array(ra["sig_avg"]), np.array(rb["sig_avg"])
+        assert np.linalg.norm(sb - sa) <= 1e-4 * np.linalg.norm(sa)
+    def vm(s):
+        s = s.reshape(-1, 3, 3)
+        d = s - np.trace(s, axis1=1, axis2=2)[:, None, None] * np.eye(3) / 3
+        return np.sqrt(1.5 * (d * d).sum(axis=(1, 2)))
+    va, vb = vm(a.field("sig")), vm(b.field("sig"))
+    assert np.abs(va - vb).max() <= 1e-3 * va.max()
+    assert rel(b.field("rho_s"), a.field("rho_s")) < 1e-6
+
+
+def test_output_files_match_oracle(tmp_path, oracle_lib):
+    """vm.out / dd.out / err.out written by the GPU path carry the oracle's numbers (%e)."""
+    deck = make(tmp_path, n=(8, 8, 8), grains=3, seed=7, nsteps=3)
+    go, oo = tmp_path / "gpu", tmp_path / "orc"
+    go.mkdir()
+    oo.mkdir()
+    for f in ("Cu_p.sx", "mesh.txt"):
+        shutil.copy(tmp_path / f, go / f)
+        shutil.copy(tmp_path / f, oo / f)
+    g = srx_amd.Solver(deck, workdir=str(go), write_files=1, fixed_iters=6)
+    o = oracle_lib.Oracle(deck, workdir=str(oo), write_files=1, fixed_iters=6)
+    for _ in range(3):
+        g.step()
+        o.step()
+    g.close()
+    o.close()
+    for name in ("vm.out", "dd.out"):
+        A = open(go / name).read().splitlines()
+        B = open(oo / name).read().splitlines()
+        assert A[0] == B[0] and len(A) == len(B) == 4
+        a = np.array([[float(x) for x in l.split("\t") if x] for l in A[1:]])
+        b = np.array([[float(x) for x in l.split("\t") if x] for l in B[1:]])
+        np.testing.assert_allclose(a, b, rtol=2e-6)
+    assert open(go / "err.out").read() == open(oo / "err.out").read()
+
+
+def test_nan_is_reported_not_aborted(tmp_path):
+    """0.1% strain in one increment from the elastic predictor overflows the plain Newton solve
+    (the reference asserts, constitutive.c:683); the C ABI returns SRX_E_NAN instead."""
+    deck = inputs.make_case(str(tmp_path), n=(8, 8, 8), grains=4, seed=1, nsteps=2, timestep=1.0)
+    g = srx_amd.Solver(deck)
+    with pytest.raises(srx_amd.SrxError) as e:
+        g.step()
+    assert e.value.code == 3001

@@ -1,0 +1,247 @@
+"""Pins the oracle's constitutive law (constitutive.c), interaction tables (init.c:594-703),
+deck reading (io.c:96-164) and mixed boundary conditions against closed forms."""
+import itertools
+import os
+import shutil
+
+import numpy as np
+import pytest
+
+from oracle.oracle import Oracle, dptr, lib, open_status
+from srx_amd import inputs
+
+GOLD = os.path.join(os.path.dirname(__file__), "golden", "reference_inputs")
+
+
+@pytest.fixture(scope="module")
+def case16(tmp_path_factory):
+    d = tmp_path_factory.mktemp("c16")
+    return inputs.make_case(str(d), n=(8, 8, 8), grains=4, seed=3, nsteps=3, timestep=0.1)
+
+
+def test_closed_form_constants(case16):
+    """Appendix-A constants: Cu at 300 K (init.c:822-827), kT, b, q (evp.h:84-94, init.c:183)."""
+    o = Oracle(case16)
+    c11, c12, c44, nu = o.setup(6, 4)
+    assert (c11, c12, c44) == pytest.approx((169665.0, 122500.0, 75827.0), rel=1e-12)
+    kT, q_slip, q_bulk, b, G = o.setup(2, 5)
+    # tolerances: half a unit in the last printed digit of the closed-form values
+    assert kT == pytest.approx(0.025852157, abs=5e-10)
+    assert b == pytest.approx(0.25597265, abs=5e-9)
+    assert q_slip == pytest.approx(48.286155, abs=5e-7)
+    assert q_bulk == pytest.approx(38.628924, abs=5e-7)
+    assert G == pytest.approx(47565.524, abs=5e-4)
+
+
+def test_slip_law_known_answers(case16):
+    """StrainRate_Orowan at rho_s = 1 (constitutive.c:556-684): rho_P = 0.6, rho_f = 2.6,
+    rho_m = 0.0129696 after the first call, tau_pass 9.4310811 on the first call (rho_m = 0) and
+    9.5324673 after, tau_cut 0.10193114, gamma_0 2.20406e-17 (Appendix A)."""
+    o = Oracle(case16)
+    L = lib()
+    L.orc_update_schmid(o.h)
+    sch = np.zeros(60)
+    L.orc_get_schmid(o.h, 0, dptr(sch))
+    p = sch.reshape(12, 5)
+    # stress along system 0 with resolved shear t on it
+    t = 16.022
+    sig = np.zeros(6)
+    sig[:5] = t * p[0] / (p[0] @ p[0])
+    rss = p @ sig[:5]
+    ed, dd = np.zeros(6), np.zeros(36)
+
+    def gdot(tp):
+        x = np.abs(rss) - tp
+        return np.where(x > 0, 2.20406e-17 * np.sinh(np.maximum(x, 0) / (0.1 + 0.10193114)) * np.sign(rss), 0.0)
+
+    # first call: lagged rho_m = 0
+    L.orc_strain_rate(o.h, 0, dptr(sig), dptr(ed), dptr(dd))
+    np.testing.assert_allclose(o.field("rho_P")[0], 0.6, rtol=1e-12)
+    np.testing.assert_allclose(o.field("rho_F")[0], 2.6, rtol=1e-12)
+    np.testing.assert_allclose(o.field("rho_m")[0], 0.0129696, rtol=2e-6)
+    np.testing.assert_allclose(o.field("gamdot")[0], gdot(9.4310811), rtol=2e-5, atol=1e-30)
+    # second call: tau_pass with the updated rho_m
+    L.orc_strain_rate(o.h, 0, dptr(sig), dptr(ed), dptr(dd))
+    g = o.field("gamdot")[0]
+    np.testing.assert_allclose(g, gdot(9.5324673), rtol=2e-5, atol=1e-30)
+    assert g[0] == pytest.approx(1e-3, rel=3e-3)   # |rss| for 1e-3 1/s with tau_0 = 0.1 is 16.022
+    np.testing.assert_allclose(ed[:5], p.T @ g, rtol=1e-12, atol=1e-18)
+    assert ed[5] == 0.0
+
+
+def test_slip_law_jacobian_matches_finite_differences(case16):
+    o = Oracle(case16)
+    L = lib()
+    L.orc_update_schmid(o.h)
+    rng = np.random.default_rng(11)
+    sig0 = np.zeros(6)
+    sig0[:5] = rng.standard_normal(5) * 12.0
+    ed, dd = np.zeros(6), np.zeros(36)
+    L.orc_strain_rate(o.h, 0, dptr(sig0), dptr(ed), dptr(dd))   # settle rho_m
+    L.orc_strain_rate(o.h, 0, dptr(sig0), dptr(ed), dptr(dd))
+    J = dd.reshape(6, 6).copy()
+    h = 1e-5
+    fd = np.zeros((6, 6))
+    for k in range(5):
+        sp, sm = sig0.copy(), sig0.copy()
+        sp[k] += h
+        sm[k] -= h
+        ep, em = np.zeros(6), np.zeros(6)
+        L.orc_strain_rate(o.h, 0, dptr(sp), dptr(ep), dptr(np.zeros(36)))
+        L.orc_strain_rate(o.h, 0, dptr(sm), dptr(em), dptr(np.zeros(36)))
+        fd[:, k] = (ep - em) / (2 * h)
+    scale = np.abs(J).max()
+    assert scale > 0
+    np.testing.assert_allclose(J, fd, atol=1e-6 * scale)
+    np.testing.assert_allclose(J, J.T, atol=1e-12 * scale)
+
+
+def test_newton_solution_has_zero_residual(case16):
+    """After UpdateStress the local residual R = sig - lam + C066 (eps_p + edot dt + S sig - eps_c)
+    of evolution.c:250-255 vanishes to the NR tolerance."""
+    o = Oracle(case16, fixed_iters=2)
+    o.step()
+    L = lib()
+    C066 = o.setup(0, 36).reshape(6, 6)
+    from tests.test_oracle_kinematics import t_to_v
+    idx = 0
+    lam = t_to_v(o.field("sig")[idx].reshape(3, 3))
+    dg = o.field("disgrad")[idx].reshape(3, 3)
+    ec = t_to_v((dg + dg.T) / 2)
+    ep = t_to_v(o.field("eps")[idx].reshape(3, 3))
+    cgr = o.setup(13, 36).reshape(6, 6)
+    S = np.linalg.inv(cgr)
+    es, ee = np.zeros(1), np.zeros(1)
+    it = L.orc_nr_voxel(o.h, idx, dptr(es), dptr(ee))
+    assert 1 <= it < 500
+    sig = t_to_v(o.field("sig")[idx].reshape(3, 3))
+    ed = np.zeros(6)
+    L.orc_strain_rate(o.h, idx, dptr(sig), dptr(ed), dptr(np.zeros(36)))
+    R = sig - lam + C066 @ (ep + ed * 0.1 + S @ sig - ec)
+    assert np.linalg.norm(R) < 1e-8 * np.linalg.norm(lam)
+
+
+def test_hardening_step_known_answer(case16):
+    """DislocationEvolution (constitutive.c:1407-1416) for one increment from rho_s = 1 with
+    C_6 = 0: rho_s' - rho_s = dt |gamdot| (C_4 sqrt(rho_F) - C_5 rho_s) - dt*climb, climb < 1e-9."""
+    o = Oracle(case16)
+    rs0 = o.field("rho_s").copy()
+    o.step()
+    g = o.field("gamdot")
+    rF = o.field("rho_F")
+    expect = rs0 + 0.1 * np.abs(g) * (600.0 * np.sqrt(rF) - 17.6 * rs0)
+    np.testing.assert_allclose(o.field("rho_s"), expect, rtol=0, atol=1e-9)
+    np.testing.assert_allclose(o.field("rho_forest"), 0.1 * np.abs(g) * 600.0 * np.sqrt(rF), rtol=1e-12, atol=1e-300)
+    np.testing.assert_allclose(o.field("gam_acum"), 0.1 * g, rtol=1e-12, atol=1e-300)
+
+
+def test_interaction_table_matches_slip_geometry(tmp_path):
+    """Arsenlis-Parks classification of the 12 FCC systems (init.c:594-703): coplanar = same
+    plane, cross slip = same Burgers vector, Hirth = orthogonal b, glissile = junction b in one of
+    the two planes, Lomer = in neither."""
+    vals = dict(Selfinter0=0.11, Coplanar0=0.13, CrossSlip0=0.17, GlissileJunction0=0.19, HirthLock0=0.23,
+                LomerCottrellLock0=0.29)
+    deck = inputs.make_case(str(tmp_path), n=(4, 4, 4), grains=2, seed=1, nsteps=1, timestep=0.1,
+                            extra={k: "%r %r" % (v, v) for k, v in vals.items()})
+    o = Oracle(deck)
+    chi = o.setup(4, 144).reshape(12, 12)
+    chi1 = o.setup(5, 144).reshape(12, 12)
+    sl = [(np.array(n, float) / np.linalg.norm(n), np.array(b, float) / np.linalg.norm(b)) for n, b in inputs.FCC_SLIP]
+    counts = {"self": 0, "cop": 0, "cross": 0, "gli": 0, "hirth": 0, "lomer": 0}
+    for i, j in itertools.product(range(12), range(12)):
+        (ni, bi), (nj, bj) = sl[i], sl[j]
+        if i == j:
+            kind = "self"
+        elif abs(abs(ni @ nj) - 1) < 1e-12:
+            kind = "cop"
+        elif abs(abs(bi @ bj) - 1) < 1e-12:
+            kind = "cross"
+        elif abs(bi @ bj) < 1e-12:
+            kind = "hirth"
+        else:
+            s = 1.0 if np.linalg.norm(bi - bj) < 1.01 else -1.0
+            jb = bi - s * bj
+            kind = "gli" if (abs(jb @ ni) < 1e-12 or abs(jb @ nj) < 1e-12) else "lomer"
+        want_chi = {"self": 0.11, "cop": 0.13}.get(kind, 0.0)
+        want_chi1 = {"cross": 0.17, "gli": 0.19, "hirth": 0.23, "lomer": 0.29}.get(kind, 0.0)
+        assert chi[i, j] == want_chi and chi1[i, j] == want_chi1, (i, j, kind)
+        if i == 0:
+            counts[kind] += 1
+    assert counts == {"self": 1, "cop": 2, "cross": 1, "gli": 4, "hirth": 2, "lomer": 2}
+
+
+def test_cubic_single_crystal_uniaxial_is_closed_form(tmp_path):
+    """One crystal at Euler (0,0,0) under x3 extension with stress-free lateral faces and no
+    plastic flow (C_1 large): the fields stay homogeneous and, once the mixed boundary conditions
+    of get_smacro (evolution.c:619-631) have acted, eps11 = eps22 = -c12/(c11+c12) eps33 and
+    sig33 = (c11 - 2 c12^2/(c11+c12)) eps33.
+
+    Increment 0 is a fixed point of the local solve from the elastic predictor, so its loop
+    stops after one iteration (the stop test looks at Err_s and Err_e only, evolution.c:1150);
+    the correction it computed enters DisGradAvg_actual (evolution.c:1274) without reaching the
+    voxels.  Later increments satisfy the closed form."""
+    n = (4, 4, 4)
+    inputs.write_sx(str(tmp_path / "Cu_p.sx"))
+    N = 64
+    inputs.write_mesh(str(tmp_path / "mesh.txt"), n, np.zeros((N, 3)), np.ones(N, dtype=int))
+    deck = str(tmp_path / "deck.in")
+    inputs.write_deck(deck, n=n, nsteps=3, timestep=0.1, err=1e-12, itermax=3000, extra={"C_1": "1e4 1e4"})
+    o = Oracle(deck)
+    c11, c12 = 169665.0, 122500.0
+    r0 = o.step()
+    assert r0["iters"] == 1 and r0["sig_avg"][0] == pytest.approx(c12 * 1e-4, rel=1e-10)
+    prev = None
+    for k in (1, 2):
+        r = o.step()
+        E = np.array(r["disgrad_avg_actual"]).reshape(3, 3)
+        S = np.array(r["sig_avg"]).reshape(3, 3)
+        e33 = (k + 1) * 1e-4
+        dg = o.field("disgrad").reshape(-1, 3, 3).mean(axis=0)
+        assert dg[2, 2] == pytest.approx(e33, rel=1e-12)
+        assert dg[0, 0] / e33 == pytest.approx(-c12 / (c11 + c12), rel=1e-8)
+        assert dg[1, 1] / e33 == pytest.approx(-c12 / (c11 + c12), rel=1e-8)
+        assert S[2, 2] == pytest.approx((c11 - 2 * c12 ** 2 / (c11 + c12)) * e33, rel=1e-8)
+        assert abs(S[0, 0]) < 1e-8 and abs(S[1, 1]) < 1e-8
+        np.testing.assert_allclose(o.field("fluct"), 0.0, atol=1e-15)
+        if prev is not None:
+            assert (E[0, 0] - prev[0, 0]) / 1e-4 == pytest.approx(-c12 / (c11 + c12), rel=1e-8)
+        prev = E
+
+
+def test_reference_deck_two_crystalline_phases(tmp_path):
+    """The reference's own deck (input): 256x256x1, phases 1 and 2 crystalline with different
+    slip-mode files; its mesh file is not shipped, so a two-phase 256x256x1 mesh is written."""
+    for f in ("input", "Cu_p.sx", "Cu_p_hard.sx"):
+        shutil.copy(os.path.join(GOLD, f), tmp_path / f)
+    n = (256, 256, 1)
+    eul, gid = inputs.voronoi(n, 12, seed=9)
+    phase = np.where(np.arange(n[0] * n[1]) // n[1] < 128, 1, 2)
+    inputs.write_mesh(str(tmp_path / "Parent_Twin_Hori.txt"), n, eul, gid, phase)
+    o = Oracle(str(tmp_path / "input"), fixed_iters=1, nthreads=8)
+    assert o.n == n
+    c11, c12, c44, nu = o.setup(6, 4)
+    assert (c11, c12, c44, nu) == pytest.approx((169665.0, 122500.0, 75827.0, 0.35))
+    U = o.setup(7, 9).reshape(3, 3)
+    np.testing.assert_allclose(U, [[0, 1e-3, 0], [1e-3, 0, 0], [0, 0, 0]])
+    r = o.step()
+    assert np.isfinite(r["s_vm"]) and r["iters"] == 1
+
+
+def test_error_codes(tmp_path):
+    """Setup errors return the reference's PError codes instead of aborting (init.c:195-197,
+    263-266, 865-867, 888-891)."""
+    deck = inputs.make_case(str(tmp_path), n=(4, 4, 4), grains=2, seed=1, nsteps=1, timestep=0.1)
+    assert open_status(deck) == 0
+    bad = str(tmp_path / "bad116.in")
+    inputs.write_deck(bad, n=(4, 4, 4), extra={"VelGrad_BC_Flag": "1 1 0 1 1 1", "Stress_BC_Flag": "0 0 1 0 0 0"})
+    assert open_status(bad) == 116
+    bad = str(tmp_path / "bad114.in")
+    inputs.write_deck(bad, n=(4, 4, 4), extra={"Stress_BC_Flag": "1 1 1 0 0 0"})
+    assert open_status(bad) == 114
+    lines = open(tmp_path / "mesh.txt").read().splitlines()
+    lines[3], lines[4] = lines[4], lines[3]
+    open(tmp_path / "mesh_swapped.txt", "w").write("\n".join(lines) + "\n")
+    bad = str(tmp_path / "bad1139.in")
+    inputs.write_deck(bad, n=(4, 4, 4), mesh="mesh_swapped.txt")
+    assert open_status(bad) == 1139
+    assert open_status(str(tmp_path / "missing.in")) == 2001
