@@ -195,8 +195,6 @@ def test_green_operator_equilibrium_and_isotropic_closed_form():
     np.testing.assert_allclose((s + np.einsum("ijkl,kl->ij", c4, du)) @ n, 0, atol=1e-11)
     # k = 0: Gamma = 0 (ElastBC 1) or -S0 (ElastBC 0)
     np.testing.assert_allclose(green(C0, np.zeros(3), s, dc=True), 0, atol=0)
-    S0 = np.linalg.tensorinv(C0.reshape(9, 9).reshape(3, 3, 3, 3), ind=2) if False else None
     s0 = rng.standard_normal((3, 3, 3, 3))
     np.testing.assert_allclose(green(C0, np.zeros(3), s, dc=True, elast_bc=0, s0=s0),
                                -np.einsum("ijkl,kl->ij", s0, s), atol=1e-13)
-    del S0
