@@ -57,7 +57,7 @@ typedef struct {
     int    svm_mode;         /* 0: reproduce evolution.c:648 (out-of-bounds term := svm_stray); 1: standard */
     double svm_stray;
     int    write_files;      /* 1: write vm.out, dd.out, err.out, drx.out into the work directory */
-    int    write_fields;     /* 1: also the .iout field dumps at PrintControl steps */
+    int    write_fields;     /* 1: also the .iout field dumps at the deck's PrintControl steps (evolution.c:1456-1483) */
     int    nthreads;         /* host threads for setup (0: default) */
     int    verbose;          /* 1: per-iteration errors on stdout */
     int    warm_start;       /* reserved */

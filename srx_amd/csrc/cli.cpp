@@ -1,6 +1,7 @@
 // cli.cpp -- `srx <deck>`: the drop-in for the reference's `trial <deck>` (evp.c:7-31).
-// Runs every increment of the deck on the GPU and writes vm.out, dd.out, err.out
-// and drx.out into the current directory, as the reference does.
+// Runs every increment of the deck on the GPU and writes vm.out, dd.out, err.out,
+// drx.out and the .iout field dumps of the deck's PrintControl into the current
+// directory, as the reference does.
 #include "../../include/srx.h"
 
 #include <cstdio>
@@ -9,15 +10,16 @@
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <deck> [--mixed] [--fields]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <deck> [--mixed] [--no-fields] [--verbose]\n", argv[0]);
         return 1;
     }
     srx_options o;
     srx_default_options(&o);
     o.write_files = 1;
+    o.write_fields = 1;
     for (int i = 2; i < argc; ++i) {
         if (!std::strcmp(argv[i], "--mixed")) o.precision = SRX_PREC_MIXED;
-        if (!std::strcmp(argv[i], "--fields")) o.write_fields = 1;
+        if (!std::strcmp(argv[i], "--no-fields")) o.write_fields = 0;
         if (!std::strcmp(argv[i], "--verbose")) o.verbose = 1;
     }
     srx_ctx *ctx = nullptr;

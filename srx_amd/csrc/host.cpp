@@ -368,6 +368,7 @@ struct srx_ctx {
     double e_vm = 0, d_vm = 0, s_vm = 0, s_vm1 = 0, TimeTot = 0, err = 0, Err_e = 0, Err_s = 0;
     double sig_sum6[6] = {0, 0, 0, 0, 0, 0};
     int itermax = 0, nsteps = 0, istep = 0, creep = 0;
+    int fcc = 0;   // every crystalline phase has the standard FCC Schmid table
     std::vector<int> grain;
     FILE *fvm = nullptr, *fdd = nullptr, *ferr = nullptr, *fdrx = nullptr;
     // timing
@@ -404,7 +405,7 @@ bool grid_supported(const FftPlan pl[3], const int n[3]) {
         for (int i = 0; i < p.nst; ++i) mr = std::min(mr, p.rad[i]);
         if (p.nst == 0) mr = 1;
         const int slots = ax == 0 ? 9 : 1;
-        const int maxb = ax == 0 ? 6 : 4;
+        const int maxb = ax == 0 ? 4 : 2;   // butterflies per thread of the x pass and of the y/z passes
         if ((long)slots * p.n / std::max(mr, 1) > (long)maxb * 256) return false;
         if ((long)slots * p.n * 16 > 48 * 1024) return false;   // one tile row within the LDS budget
     }
@@ -434,6 +435,8 @@ int setup(srx_ctx *c) {
     c->wgt = 1. / c->nvox;
     const double kT = (d.T__K * BOLZ * J_IN_EV);
     if (d.Update_Flag != 1) return SRX_E_UPDATE_FLAG;
+    // istep % PrintControl[1] at evolution.c:1456 divides by zero for a period <= 0
+    if (c->opt.write_fields && d.PrintControl[0] == 1 && d.PrintControl[1] <= 0) return SRX_E_ARG;
 
     // phases (init.c:815-856)
     std::memset(&c->hc, 0, sizeof(c->hc));
@@ -485,6 +488,9 @@ int setup(srx_ctx *c) {
         D.k_tcut = (kT / MY_J_IN_EV) / c2 / c3 / P.b / P.b * 0.1;
         D.k_g0 = 1E-3 * 1.0E10 * c2 * P.b * D.expmq;
     }
+    c->fcc = 1;
+    for (int p = 0; p < NPH; ++p)
+        if (c->ph[p].xtal && !fcc_table_matches(c->hc.ph[p].p, c->hc.ph[p].nsys)) c->fcc = 0;
     c->hc.dt = d.TimeStep;
     c->hc.nu = d.ElastConst_PhaseI[3];
     c->hc.T = d.T__K;
@@ -559,6 +565,15 @@ int setup(srx_ctx *c) {
             for (int j = 0; j < 6; ++j) c->hc.S066[i][j] = c->S066[6 * i + j];
         c66_to_c4(c->C066, c->hc.C0);
         c66_to_c4(c->S066, c->hc.S0);
+        // G6[(ik)][(jl)] = C0_ijkl (+ C0_ilkj for j != l): A_ik = sum_jl C0_ijkl n_j n_l as a 6x6 product
+        static const int PI6[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b) {
+                const int i = PI6[a][0], k = PI6[a][1], j = PI6[b][0], l = PI6[b][1];
+                double v = c->hc.C0[((i * 3 + j) * 3 + k) * 3 + l];
+                if (j != l) v += c->hc.C0[((i * 3 + l) * 3 + k) * 3 + j];
+                c->hc.G6[6 * a + b] = v;
+            }
     }
 
     // boundary conditions (init.c:865-901)
@@ -648,7 +663,8 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.rho_m_old, 12, 0.0)) || (rc = dalloc(&F.rho_F, 12, 0.0)) || (rc = dalloc(&F.rho_P, 12, 0.0)) ||
         (rc = dalloc(&F.gamdot, 12, 0.0)) || (rc = dalloc(&F.gam_acum, 12, 0.0)) ||
         (rc = dalloc(&F.rho_dipole, 12, 0.0)) || (rc = dalloc(&F.rho_forest, 12, 0.0)) ||
-        (rc = dalloc(&F.rho_inplane, 12, 0.0)))
+        (rc = dalloc(&F.rho_inplane, 12, 0.0)) || (rc = dalloc(&F.rho_tot, 1, 0.0)) ||
+        (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
     HIPCK(hipMalloc(&F.phase, sizeof(int) * N));
     HIPCK(hipMalloc(&F.nr_it, sizeof(int) * N));
@@ -854,6 +870,142 @@ int srx_dims(srx_ctx *c, int *n3) {
     return 0;
 }
 
+// ---- field dumps at PrintControl steps (evolution.c:1456-1483) -----------
+// Raw native-endian records, one per voxel in the reference's voxel order, with
+// the file names and record layouts of io.c.  Record holes are written as zeros
+// (MPI-IO leaves them unwritten in a new file).
+
+// EulerToTransMatrix, opt 1 (kinematics.c:196-214): sample->crystal matrix to Bunge
+// angles in degrees, with the truncated pi of evp.h:84.
+static void matrix_to_euler_deg(const double a[3][3], double &t1, double &ph, double &t2) {
+    const double PI_TRUNC = 3.14159265359;
+    ph = std::acos(a[2][2]);
+    if (std::fabs(a[2][2]) > 0.9999) {
+        t2 = 0.;
+        t1 = std::atan2(a[0][1], a[0][0]);
+    } else {
+        const double sph = std::sin(ph);
+        t1 = std::atan2(a[2][0] / sph, -1.0 * a[2][1] / sph);
+        t2 = std::atan2(a[0][2] / sph, a[1][2] / sph);
+    }
+    ph *= 180. / PI_TRUNC;
+    t1 *= 180. / PI_TRUNC;
+    t2 *= 180. / PI_TRUNC;
+}
+
+static int write_blob(const srx_ctx *c, const char *name, const void *data, size_t bytes) {
+    FILE *fp = std::fopen(resolve(c->workdir, name).c_str(), "wb");
+    if (!fp) return SRX_E_FILE;
+    const size_t w = std::fwrite(data, 1, bytes, fp);
+    std::fclose(fp);
+    return w == bytes ? 0 : SRX_E_FILE;
+}
+
+// Six symmetric components in the order 11, 22, 33, 23, 13, 12 (io.c:322-437).
+static const int IOUT_SYM[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+
+static int write_field_dumps(srx_ctx *c, int step) {
+    const long N = c->nvox;
+    int rc = 0;
+    std::vector<double> sig(9 * N), eps(9 * N), dg(9 * N), tr(9 * N), rs(12 * N), ga(12 * N), fl(9 * N), ph(N);
+    if ((rc = srx_get_field(c, SRX_F_SIG, sig.data(), sig.size())) ||
+        (rc = srx_get_field(c, SRX_F_EPS, eps.data(), eps.size())) ||
+        (rc = srx_get_field(c, SRX_F_DISGRAD, dg.data(), dg.size())) ||
+        (rc = srx_get_field(c, SRX_F_TRANMAT, tr.data(), tr.size())) ||
+        (rc = srx_get_field(c, SRX_F_RHO_S, rs.data(), rs.size())) ||
+        (rc = srx_get_field(c, SRX_F_TRIAL_GAM_ACUM, ga.data(), ga.size())) ||
+        (rc = srx_get_field(c, SRX_F_FLUCT, fl.data(), fl.size())) ||
+        (rc = srx_get_field(c, SRX_F_PHASE, ph.data(), ph.size())))
+        return rc;
+    char name[256];
+    std::vector<double> buf(N);
+    // WriteEpsMPI, WriteElsMPI, WriteSigMPI
+    for (int which = 0; which < 3; ++which) {
+        const char *stem = which == 0 ? "eps" : which == 1 ? "els" : "sig_afterDRX";
+        for (int i = 0; i < 6; ++i) {
+            const int a = IOUT_SYM[i][0], b = IOUT_SYM[i][1];
+            for (long v = 0; v < N; ++v) {
+                if (which == 0) buf[v] = eps[9 * v + 3 * a + b];
+                else if (which == 2) buf[v] = sig[9 * v + 3 * a + b];
+                else if (i < 3) buf[v] = dg[9 * v + 3 * a + b] - eps[9 * v + 3 * a + b];
+                else buf[v] = (dg[9 * v + 3 * a + b] + dg[9 * v + 3 * b + a]) / 2.0 - eps[9 * v + 3 * a + b];
+            }
+            std::snprintf(name, sizeof(name), "%s_%01d_S%06d.iout", stem, i + 1, step);
+            if ((rc = write_blob(c, name, buf.data(), sizeof(double) * N))) return rc;
+        }
+    }
+    // WriteRhoMPI(..., "SSD", ...): sum of rho_s over the phase's systems (io.c:891-941)
+    for (long v = 0; v < N; ++v) {
+        const int jph = (int)ph[v];
+        const int ns = (jph >= 1 && jph <= NPH && c->ph[jph - 1].xtal) ? c->hc.ph[jph - 1].nsys : 0;
+        double acc = 0.0;
+        for (int i = 0; i < ns; ++i) acc += rs[12 * v + i];
+        buf[v] = acc;
+    }
+    std::snprintf(name, sizeof(name), "rho_afterDRX_SSD_S%06d.iout", step);
+    if ((rc = write_blob(c, name, buf.data(), sizeof(double) * N))) return rc;
+    // WriteTextureMPI: 48-byte records {double angle[3]; int coord[3], jgr, jph, buff} (io.c:478-541)
+    {
+        struct TexRec { double angle[3]; int coord[3]; int jgr, jph, buff; };
+        static_assert(sizeof(TexRec) == 48, "tex record");
+        std::vector<TexRec> rec(N);
+        for (long v = 0; v < N; ++v) {
+            double sa2xt[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) sa2xt[i][j] = tr[9 * v + 3 * j + i];
+            TexRec &r = rec[v];
+            matrix_to_euler_deg(sa2xt, r.angle[0], r.angle[1], r.angle[2]);
+            const long x = v / ((long)c->n[1] * c->n[2]), y = (v / c->n[2]) % c->n[1], z = v % c->n[2];
+            r.coord[0] = (int)x + 1;
+            r.coord[1] = (int)y + 1;
+            r.coord[2] = (int)z + 1;
+            r.jgr = c->grain[v];
+            r.jph = (int)ph[v];
+            r.buff = 0;
+        }
+        std::snprintf(name, sizeof(name), "tex_afterDRX_S%06d.iout", step);
+        if ((rc = write_blob(c, name, rec.data(), sizeof(TexRec) * N))) return rc;
+    }
+    // WriteSLIPMPI / WriteDDMPI: 104-byte records {int jgr; 4-byte hole; double v[12]} (io.c:630-719, 798-887)
+    {
+        struct SysRec { int jgr; int hole; double v[12]; };
+        static_assert(sizeof(SysRec) == 104, "slip record");
+        std::vector<SysRec> rec(N);
+        for (int which = 0; which < 2; ++which) {
+            const std::vector<double> &src = which == 0 ? ga : rs;
+            for (long v = 0; v < N; ++v) {
+                rec[v].jgr = c->grain[v];
+                rec[v].hole = 0;
+                for (int k = 0; k < 12; ++k) rec[v].v[k] = src[12 * v + k];
+            }
+            std::snprintf(name, sizeof(name), "%s_S%04d.iout", which == 0 ? "slip_afterDRX" : "dislocation_afterDRX",
+                          step);
+            if ((rc = write_blob(c, name, rec.data(), sizeof(SysRec) * N))) return rc;
+        }
+    }
+    // WriteNewPositionMPI: 24-byte records (io.c:723-796).  new_position is the affine image
+    // of the voxel under DisGrad (evolution.c:1437-1441); displacement_fluct[i] is what the
+    // transform block of evolution.c:1401-1428 leaves in the voxel, -fluctuation[i][2] (its
+    // out-of-bounds store lies outside the array and is not reproduced).
+    {
+        std::vector<double> rec(3 * N);
+        for (long v = 0; v < N; ++v) {
+            const long x = v / ((long)c->n[1] * c->n[2]), y = (v / c->n[2]) % c->n[1], z = v % c->n[2];
+            const double *D = &dg[9 * v];
+            const double X = (double)x, Y = (double)y, Z = (double)z;
+            const double np0 = (1.0 + D[0]) * X + D[1] * Y + D[2] * Z;
+            const double np1 = (1.0 + D[4]) * Y + D[3] * X + D[5] * Z;
+            const double np2 = (1.0 + D[8]) * Z + D[6] * X + D[7] * Y;
+            rec[3 * v + 0] = np0 + (-fl[9 * v + 2]) - X;
+            rec[3 * v + 1] = np1 + (-fl[9 * v + 5]) - Y;
+            rec[3 * v + 2] = np2 + (-fl[9 * v + 8]) - Z;
+        }
+        std::snprintf(name, sizeof(name), "new_position_afterDRX_S%04d.iout", step);
+        if ((rc = write_blob(c, name, rec.data(), sizeof(double) * 3 * N))) return rc;
+    }
+    return 0;
+}
+
 int srx_step(srx_ctx *c, srx_step_report *rep) {
     if (!c) return SRX_E_ARG;
     if (c->istep >= c->nsteps) return SRX_E_DONE;
@@ -872,7 +1024,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
         add[k] = c->Udot[k] * dt;
         set[k] = c->DisGradAvg[k] + c->acum[k];
     }
-    launch_prologue(c->stream, c->f, N, add, c->creep, set);
+    launch_prologue(c->stream, c->f, N, add, c->creep, set, c->uniform);
     if (c->creep == 1) for (int k = 0; k < 9; ++k) c->uniform[k] = set[k];
     else for (int k = 0; k < 9; ++k) c->uniform[k] += add[k];
     for (int k = 0; k < 9; ++k) c->dDisGradAvg[k] = c->acum[k] = 0.0;
@@ -900,7 +1052,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
         { Timer t(c, 3); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1]); }
         { Timer t(c, 4); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], c->dDisGradAvg, divisor); }
         for (int k = 0; k < 9; ++k) c->uniform[k] += c->dDisGradAvg[k];
-        { Timer t(c, 5); launch_nr(c->stream, c->dc, c->f, N, iter == 1, rotated, c->partials, c->nr_blocks); }
+        { Timer t(c, 5); launch_nr(c->stream, c->dc, c->f, N, iter == 1, rotated, c->partials, c->fcc); }
         { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red); }
         int rc = sync_read(c, NR_NPART);
         if (rc) return rc;
@@ -985,13 +1137,16 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
                      c->s_vm1);
         std::fflush(c->fvm);
     }
+    int frc = 0;
+    if (c->opt.write_fields && d.PrintControl[0] == 1 && c->istep % d.PrintControl[1] == 0)
+        frc = write_field_dumps(c, c->istep);
     if (c->ferr) std::fflush(c->ferr);
     // evolution.c:1513-1515
     for (int k = 0; k < 9; ++k) c->DisGradAvg[k] = c->DisGradAvg_t[k] + c->Udot[k] * dt;
     collect_timing(c);
     c->istep++;
     if (rep) *rep = r;
-    return 0;
+    return frc;
 }
 
 int srx_run(srx_ctx *c) {
@@ -1022,7 +1177,7 @@ static int field_info(int id, int &nc) {
     case SRX_F_FLUCT: nc = 9; return 0;
     case SRX_F_RHO_S: case SRX_F_RHO_M: case SRX_F_RHO_F: case SRX_F_RHO_P: case SRX_F_GAMDOT: case SRX_F_GAM_ACUM:
     case SRX_F_RHO_DIPOLE: case SRX_F_RHO_FOREST: case SRX_F_RHO_INPLANE: case SRX_F_TRIAL_GAM_ACUM: nc = 12; return 0;
-    case SRX_F_NR_ITERS: case SRX_F_PHASE: case SRX_F_GRAIN: nc = 1; return 0;
+    case SRX_F_NR_ITERS: case SRX_F_PHASE: case SRX_F_GRAIN: case SRX_F_RHO_TOT: nc = 1; return 0;
     default: return SRX_E_FIELD;
     }
 }
@@ -1056,15 +1211,15 @@ int srx_get_field(srx_ctx *c, int id, double *out, size_t n) {
         tmp.resize((size_t)9 * N);
         if ((rc = fetch(id == SRX_F_TRANMAT ? c->f.tran : c->f.dg, sizeof(double) * 9 * N, tmp.data()))) return rc;
         std::vector<double> t0;
-        if (id == SRX_F_VELGRAD) {
+        if (id == SRX_F_VELGRAD || (id == SRX_F_FLUCT && c->f.fl0)) {
             t0.resize((size_t)9 * N);
-            if ((rc = fetch(c->f.dg0, sizeof(double) * 9 * N, t0.data()))) return rc;
+            if ((rc = fetch(id == SRX_F_VELGRAD ? c->f.dg0 : c->f.fl0, sizeof(double) * 9 * N, t0.data()))) return rc;
         }
         for (long v = 0; v < N; ++v)
             for (int k = 0; k < 9; ++k) {
                 double x = tmp[k * N + v];
                 if (id == SRX_F_VELGRAD) x = (x - t0[k * N + v]) / c->dk.TimeStep;
-                if (id == SRX_F_FLUCT) x -= c->uniform[k];
+                if (id == SRX_F_FLUCT) x = (x - c->uniform[k]) + (t0.empty() ? 0.0 : t0[k * N + v]);
                 out[9 * v + k] = x;
             }
         return 0;
@@ -1090,6 +1245,7 @@ int srx_get_field(srx_ctx *c, int id, double *out, size_t n) {
         for (long v = 0; v < N; ++v) out[v] = c->grain[v];
         return 0;
     }
+    if (id == SRX_F_RHO_TOT) return fetch(c->f.rho_tot, sizeof(double) * N, out);
     return SRX_E_FIELD;
 }
 
@@ -1146,7 +1302,8 @@ void srx_close(srx_ctx *c) {
     if (c->ferr) std::fclose(c->ferr);
     double *ptrs[] = {c->f.sig, c->f.sig_pre, c->f.eps, c->f.edot, c->f.dg, c->f.dg0, c->f.tran, c->f.tran0,
                       c->f.rho_s, c->f.rho_m, c->f.rho_m_old, c->f.rho_F, c->f.rho_P, c->f.gamdot, c->f.gam_acum,
-                      c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->partials, c->red};
+                      c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->f.rho_tot, c->f.fl0,
+                      c->partials, c->red};
     for (double *p : ptrs)
         if (p) hipFree(p);
     if (c->f.phase) hipFree(c->f.phase);

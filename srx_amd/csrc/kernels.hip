@@ -116,6 +116,29 @@ struct Dft<T, 5> {
     }
 };
 
+template <typename T>
+struct Dft<T, 8> {
+    // even/odd split into two radix-4 transforms
+    __device__ __forceinline__ static void run(cx<T> *v, int sign) {
+        cx<T> e[4] = {v[0], v[2], v[4], v[6]}, o[4] = {v[1], v[3], v[5], v[7]};
+        Dft<T, 4>::run(e, sign);
+        Dft<T, 4>::run(o, sign);
+        const T h = T(0.70710678118654752440);
+        const T sh = T(sign) * h;
+        const cx<T> o1 = cmul(o[1], cx<T>{h, sh});
+        const cx<T> o2 = sign < 0 ? cx<T>{o[2].y, -o[2].x} : cx<T>{-o[2].y, o[2].x};
+        const cx<T> o3 = cmul(o[3], cx<T>{-h, sh});
+        v[0] = e[0] + o[0];
+        v[4] = e[0] - o[0];
+        v[1] = e[1] + o1;
+        v[5] = e[1] - o1;
+        v[2] = e[2] + o2;
+        v[6] = e[2] - o2;
+        v[3] = e[3] + o3;
+        v[7] = e[3] - o3;
+    }
+};
+
 constexpr int FFT_BLOCK = 256;
 
 // One Stockham stage (radix R, sub-transform span L) over `npen` pencils of
@@ -170,6 +193,7 @@ __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
         switch (R) {
+        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign); break;
         case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign); break;
         case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign); break;
         case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign); break;
@@ -189,6 +213,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fwd_z(const double *__restrict__ 
                                                      double m2, double m3, double m4, double m5) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *stw = buf + TY * n2;   // twiddle table staged in LDS
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
     const int ty = min(TY, n1 - y0);
@@ -196,6 +221,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fwd_z(const double *__restrict__ 
     const size_t N = (size_t)n0 * n1 * n2;
     const size_t HS = (size_t)H * n0 * n1;
     const double mean[6] = {m0, m1, m2, m3, m4, m5};
+    for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     for (int ca = 0; ca < ncomp; ca += 2) {
         const int cb = ca + 1;
         const bool two = cb < ncomp;
@@ -208,7 +234,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fwd_z(const double *__restrict__ 
             buf[yy * n2 + z] = cx<T>{T(va), T(vb)};
         }
         __syncthreads();
-        lds_fft<T, 4>(buf, ty, pz, tw, -1);
+        lds_fft<T, 2>(buf, ty, pz, stw, -1);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
             const int kz = i / ty, yy = i - kz * ty;
             const cx<T> z1 = buf[yy * n2 + kz];
@@ -232,6 +258,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
                                                      const cx<T> *__restrict__ tw, int TX) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *stw = buf + TX * n1;
+    for (int i = threadIdx.x; i < n1; i += FFT_BLOCK) stw[i] = tw[i];
     const int H = n2 / 2 + 1;
     const int x0 = blockIdx.x * TX;
     const int tx = min(TX, n0 - x0);
@@ -251,7 +279,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
         }
     }
     __syncthreads();
-    lds_fft<T, 4>(buf, tx, py, tw, b_to_c ? -1 : +1);
+    lds_fft<T, 2>(buf, tx, py, stw, b_to_c ? -1 : +1);
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int y = i / tx, xx = i - y * tx;
@@ -333,6 +361,53 @@ __device__ __forceinline__ void b6_to_t9(const V v[6], V t[9]) {
     t[1] = t[3] = v[4] * V(SRX_R2);
 }
 
+// B-basis 6-vector -> symmetric tensor entries (00, 11, 22, 12, 02, 01)
+__device__ __forceinline__ void b6_to_sym(const double v[6], double t[6]) {
+    t[0] = v[0] * (-SRX_R2) + v[1] * (-SRX_R6) + v[5] * SRX_R3;
+    t[1] = v[0] * SRX_R2 + v[1] * (-SRX_R6) + v[5] * SRX_R3;
+    t[2] = v[1] * (2. * SRX_R6) + v[5] * SRX_R3;
+    t[3] = v[2] * SRX_R2;
+    t[4] = v[3] * SRX_R2;
+    t[5] = v[4] * SRX_R2;
+}
+__device__ __forceinline__ void sym_to_t9(const double t[6], double m[9]) {
+    m[0] = t[0]; m[4] = t[1]; m[8] = t[2];
+    m[5] = m[7] = t[3];
+    m[2] = m[6] = t[4];
+    m[1] = m[3] = t[5];
+}
+
+// Green operator at one k point from the symmetric contraction table G6:
+//   A_ik = sum_jl C0_ijkl n_j n_l = sum_b G6[(ik)][b] m_b,  m = (n0^2, n1^2, n2^2, n1n2, n0n2, n0n1)
+//   w = A^-1 (sig n)  (cofactors of the symmetric A, as M3Inverse, init.c:518-540)
+// and du_ij = -w_i n_j  (init.c:585 with evolution.c:1169-1183).
+__device__ __forceinline__ void green_w6(const double *__restrict__ G6, const double n[3], const double sr[6],
+                                         const double si[6], double wr[3], double wi[3]) {
+    const double m[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[1] * n[2], n[0] * n[2], n[0] * n[1]};
+    double A[6];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) acc += G6[6 * a + b] * m[b];
+        A[a] = acc;
+    }
+    // A = [[A0, A5, A4], [A5, A1, A3], [A4, A3, A2]]
+    const double c00 = A[1] * A[2] - A[3] * A[3], c11 = A[0] * A[2] - A[4] * A[4], c22 = A[0] * A[1] - A[5] * A[5];
+    const double c01 = A[4] * A[3] - A[5] * A[2], c02 = A[5] * A[3] - A[4] * A[1], c12 = A[5] * A[4] - A[0] * A[3];
+    const double rdet = 1.0 / (A[0] * c00 + A[5] * c01 + A[4] * c02);
+    // t = sig n with sig = [[s0, s5, s4], [s5, s1, s3], [s4, s3, s2]]
+    const double t0r = sr[0] * n[0] + sr[5] * n[1] + sr[4] * n[2], t0i = si[0] * n[0] + si[5] * n[1] + si[4] * n[2];
+    const double t1r = sr[5] * n[0] + sr[1] * n[1] + sr[3] * n[2], t1i = si[5] * n[0] + si[1] * n[1] + si[3] * n[2];
+    const double t2r = sr[4] * n[0] + sr[3] * n[1] + sr[2] * n[2], t2i = si[4] * n[0] + si[3] * n[1] + si[2] * n[2];
+    wr[0] = (c00 * t0r + c01 * t1r + c02 * t2r) * rdet;
+    wi[0] = (c00 * t0i + c01 * t1i + c02 * t2i) * rdet;
+    wr[1] = (c01 * t0r + c11 * t1r + c12 * t2r) * rdet;
+    wi[1] = (c01 * t0i + c11 * t1i + c12 * t2i) * rdet;
+    wr[2] = (c02 * t0r + c12 * t1r + c22 * t2r) * rdet;
+    wi[2] = (c02 * t0i + c12 * t1i + c22 * t2i) * rdet;
+}
+
 // x pass with the Green operator.  grid: (ceil(n1/TY), H).  LDS: 9 slots of TY*n0.
 // mode 1: forward x of 6 comps, Green, inverse x of 9 comps (the solver pass)
 // mode 0: forward x only (ncomp comps, testing)   mode 2: inverse x only (testing)
@@ -351,14 +426,15 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
     const int slot = ty * n0;
     const int cin = mode == 1 ? 6 : ncomp;
     const int cout = mode == 1 ? 9 : ncomp;
+    cx<T> *stw = buf + (mode == 1 ? 9 : ncomp) * slot;
+    for (int i = threadIdx.x; i < n0; i += FFT_BLOCK) stw[i] = tw[i];
     for (int i = threadIdx.x; i < cin * slot; i += FFT_BLOCK) {
         const int c = i / slot, r = i - c * slot, yy = r / n0, x = r - yy * n0;
         buf[c * slot + yy * n0 + x] = spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x];
     }
     __syncthreads();
-    if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, tw, -1);
+    if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, stw, -1);
     if (mode == 1) {
-        const double *C0 = cst->C0;
         double tzv, tyv;
         unit_k(kz, n2, tzv);
         const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
@@ -369,75 +445,76 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
             unit_k(kx, n0, txv);
             const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
             const bool nyy = (n1 % 2 == 0) && (ky == n1 / 2);
-            double sr6[6], si6[6];
+            double sr[6], si[6];
+            {
+                double vr[6], vi[6];
 #pragma unroll
-            for (int c = 0; c < 6; ++c) {
-                const cx<T> s = buf[c * slot + i];
-                sr6[c] = (double)s.x;
-                si6[c] = (double)s.y;
+                for (int c = 0; c < 6; ++c) {
+                    const cx<T> q = buf[c * slot + i];
+                    vr[c] = (double)q.x;
+                    vi[c] = (double)q.y;
+                }
+                b6_to_sym(vr, sr);
+                b6_to_sym(vi, si);
             }
-            double sr[9], si[9];
-            b6_to_t9(sr6, sr);
-            b6_to_t9(si6, si);
-            double dr[9], di[9];
             if (kx == 0 && ky == 0 && kz == 0) {
-                // k = 0: Gamma = -S0 (ElastBC 0) or 0 (init.c:558-569).  dc*: sum of sig over the grid
-                // that the mean-subtracted spectrum lacks (mixed precision), B basis.
+                // k = 0: Gamma = -S0 (ElastBC 0) or 0 (init.c:558-569).  dc*: the grid sum of sig
+                // that a mean-subtracted (mixed precision) spectrum lacks, B basis.
+                double dr[9], di[9];
                 if (cst->elast_bc == 0) {
                     const double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
-                    double dct[9];
-                    b6_to_t9(dc6, dct);
-                    for (int a = 0; a < 9; ++a) {
+                    double dcs[6], t9r[9], t9i[9];
+                    b6_to_sym(dc6, dcs);
+                    sym_to_t9(sr, t9r);
+                    sym_to_t9(si, t9i);
+                    double d9[9];
+                    sym_to_t9(dcs, d9);
+                    for (int q = 0; q < 9; ++q) {
                         double accr = 0.0, acci = 0.0;
-                        for (int b = 0; b < 9; ++b) {
-                            accr += -cst->S0[a * 9 + b] * (sr[b] + dct[b]);
-                            acci += -cst->S0[a * 9 + b] * si[b];
+                        for (int r = 0; r < 9; ++r) {
+                            accr += -cst->S0[q * 9 + r] * (t9r[r] + d9[r]);
+                            acci += -cst->S0[q * 9 + r] * t9i[r];
                         }
-                        dr[a] = accr;
-                        di[a] = acci;
+                        dr[q] = accr;
+                        di[q] = acci;
                     }
                 } else {
 #pragma unroll
-                    for (int a = 0; a < 9; ++a) dr[a] = di[a] = 0.0;
+                    for (int q = 0; q < 9; ++q) dr[q] = di[q] = 0.0;
                 }
+#pragma unroll
+                for (int q = 0; q < 9; ++q) buf[q * slot + i] = cx<T>{T(dr[q]), T(di[q])};
             } else {
                 double nv[3] = {txv, tyv, tzv};
-                {
-                    const double g2 = sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-                    const double rg = 1. / g2;
-                    nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
-                }
+                const double rg = 1. / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+                nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
                 double wr[3], wi[3];
-                green_w(C0, nv, sr, si, wr, wi);
-#pragma unroll
-                for (int a = 0; a < 3; ++a)
-#pragma unroll
-                    for (int b = 0; b < 3; ++b) {
-                        dr[3 * a + b] = -wr[a] * nv[b];
-                        di[3 * a + b] = -wi[a] * nv[b];
-                    }
-                if (nyx || nyy || nyz) {
-                    // Taking Re() of the inverse transform (evolution.c:1197) keeps the Hermitian part
-                    // of du; at a Nyquist index that part averages Gamma(n) with Gamma(n') where n'
-                    // has the Nyquist components negated.
-                    double np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
-                    double wr2[3], wi2[3];
-                    green_w(C0, np, sr, si, wr2, wi2);
+                green_w6(cst->G6, nv, sr, si, wr, wi);
+                if (!(nyx || nyy || nyz)) {
 #pragma unroll
                     for (int a = 0; a < 3; ++a)
 #pragma unroll
-                        for (int b = 0; b < 3; ++b) {
-                            dr[3 * a + b] = 0.5 * (dr[3 * a + b] - wr2[a] * np[b]);
-                            di[3 * a + b] = 0.5 * (di[3 * a + b] - wi2[a] * np[b]);
-                        }
+                        for (int b = 0; b < 3; ++b)
+                            buf[(3 * a + b) * slot + i] = cx<T>{T(-wr[a] * nv[b]), T(-wi[a] * nv[b])};
+                } else {
+                    // Taking Re() of the inverse transform (evolution.c:1197) keeps the Hermitian part
+                    // of du; at a Nyquist index that part averages Gamma(n) with Gamma(n') where n'
+                    // has the Nyquist components negated.
+                    const double np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
+                    double wr2[3], wi2[3];
+                    green_w6(cst->G6, np, sr, si, wr2, wi2);
+#pragma unroll
+                    for (int a = 0; a < 3; ++a)
+#pragma unroll
+                        for (int b = 0; b < 3; ++b)
+                            buf[(3 * a + b) * slot + i] = cx<T>{T(0.5 * (-wr[a] * nv[b] - wr2[a] * np[b])),
+                                                                T(0.5 * (-wi[a] * nv[b] - wi2[a] * np[b]))};
                 }
             }
-#pragma unroll
-            for (int c = 0; c < 9; ++c) buf[c * slot + i] = cx<T>{T(dr[c]), T(di[c])};
         }
         __syncthreads();
     }
-    if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, tw, +1);
+    if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, stw, +1);
     for (int i = threadIdx.x; i < cout * slot; i += FFT_BLOCK) {
         const int c = i / slot, r = i - c * slot, yy = r / n0, x = r - yy * n0;
         spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x] = buf[c * slot + yy * n0 + x];
@@ -456,6 +533,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_inv_z(const cx<T> *__restrict__ s
                                                      double a4, double a5, double a6, double a7, double a8) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
+    cx<T> *stw = buf + TY * n2;
+    for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
     const int ty = min(TY, n1 - y0);
@@ -475,7 +554,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_inv_z(const cx<T> *__restrict__ s
             if (kz >= 1 && kz <= n2 - H) buf[yy * n2 + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
         }
         __syncthreads();
-        lds_fft<T, 4>(buf, ty, pz, tw, +1);
+        lds_fft<T, 2>(buf, ty, pz, stw, +1);
         for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
             const int yy = i / n2, z = i - yy * n2;
             const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
@@ -564,6 +643,25 @@ __device__ __forceinline__ void sprime(const double a[9], const double a0[9], co
     double Q2[5][5];
     q5_from(R, Q2);
     rot_diag(Q2, d, S);
+}
+
+// ---------------------------------------------------------------------------
+// crystal-frame Schmid tensors (B basis) of the 12 FCC {111}<110> systems in the order of the
+// reference's Cu_p.sx, exactly as the host computes them from the slip-mode file.  A phase whose
+// table matches bit for bit runs the specialised Newton kernel, in which the zero components are
+// compile-time constants and their products are skipped (adding +-0 changes nothing).
+// ---------------------------------------------------------------------------
+constexpr double FCC_A = 0.28867513459481287, FCC_B = 0.5, FCC_C = 0.5773502691896257;
+__host__ __device__ constexpr double fcc_p(int s, int k) {
+    constexpr double T[12][5] = {
+        {FCC_A, -FCC_B, 0.0, FCC_A, FCC_A},   {-FCC_A, -FCC_B, FCC_A, 0.0, FCC_A},
+        {-FCC_C, 0.0, FCC_A, -FCC_A, 0.0},    {-FCC_A, FCC_B, 0.0, -FCC_A, FCC_A},
+        {-FCC_A, -FCC_B, -FCC_A, 0.0, -FCC_A}, {-FCC_C, 0.0, -FCC_A, -FCC_A, 0.0},
+        {-FCC_A, FCC_B, 0.0, FCC_A, FCC_A},   {-FCC_A, -FCC_B, FCC_A, 0.0, -FCC_A},
+        {-FCC_C, 0.0, FCC_A, FCC_A, 0.0},     {FCC_A, -FCC_B, 0.0, -FCC_A, FCC_A},
+        {-FCC_A, -FCC_B, -FCC_A, 0.0, FCC_A}, {-FCC_C, 0.0, -FCC_A, FCC_A, 0.0},
+    };
+    return T[s][k];
 }
 
 // ---------------------------------------------------------------------------
@@ -670,7 +768,7 @@ __device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const do
         }
 }
 
-template <bool FIRST, bool ROT, int PH>
+template <bool FIRST, bool ROT, int PH, bool FCC>
 __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
                                          double *lds, NrResult &res) {
     // compile-time phase index: every material constant is a scalar load at a fixed offset
@@ -792,19 +890,45 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
         const double *tpu = (FIRST && it == 1) ? tpo : tp;
+        if (FCC) {
+#pragma unroll
+            for (int s = 0; s < NSYS; ++s) {
+                double rss = 0.0;
+#pragma unroll
+                for (int k = 0; k < 5; ++k)
+                    if (fcc_p(s, k) != 0.0) rss += fcc_p(s, k) * sg[k];
+                const double tps = tpu[s * NR_BLOCK];
+                // a system inactive on every lane of the wavefront contributes exactly nothing
+                if (__any(fabs(rss) > tps)) {
+                    double g, g2;
+                    slip_rate(rss, tps, inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
+                    const double g2dt = g2 * dt;
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        if (fcc_p(s, i) == 0.0) continue;
+                        ed[i] += fcc_p(s, i) * g;
+                        const double gp = g2dt * fcc_p(s, i);
+#pragma unroll
+                        for (int j = i; j < 5; ++j)
+                            if (fcc_p(s, j) != 0.0) K[ps(i, j)] += gp * fcc_p(s, j);
+                    }
+                }
+            }
+        } else {
 #pragma unroll 2
-        for (int s = 0; s < NSYS; ++s) {
-            const double *p = P.p[s];
-            const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
-            double g, g2;
-            slip_rate(rss, tpu[s * NR_BLOCK], inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
-            const double g2dt = g2 * dt;
+            for (int s = 0; s < NSYS; ++s) {
+                const double *p = P.p[s];
+                const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
+                double g, g2;
+                slip_rate(rss, tpu[s * NR_BLOCK], inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
+                const double g2dt = g2 * dt;
 #pragma unroll
-            for (int i = 0; i < 5; ++i) {
-                ed[i] += p[i] * g;
-                const double gp = g2dt * p[i];
+                for (int i = 0; i < 5; ++i) {
+                    ed[i] += p[i] * g;
+                    const double gp = g2dt * p[i];
 #pragma unroll
-                for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
+                    for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
+                }
             }
         }
         double R[5];
@@ -913,7 +1037,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     res.bad = bad;
 }
 
-template <bool FIRST, bool ROT>
+template <bool FIRST, bool ROT, bool FCC>
 __global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                  double *__restrict__ partials) {
     const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
@@ -933,9 +1057,9 @@ __global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cs
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0>(cst, f, v, N, lds, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, FCC>(cst, f, v, N, lds, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1>(cst, f, v, N, lds, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, FCC>(cst, f, v, N, lds, r);
             }
         }
         todo &= ~__ballot(mine);
@@ -968,18 +1092,33 @@ __global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cs
 }
 
 // fixed-order reduction of block partials stored component-major:
-// out[k] = sum over blocks of part[k][b] (max for k in maxmask); one block per k.
-__global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
-                                                unsigned long long maxmask, double *__restrict__ out) {
-    __shared__ double sh[256];
+// out[k] = sum over blocks of part[k][b] (max for k in maxmask); one block per k.  Each thread
+// keeps eight independent accumulators over a fixed stride, combined in a fixed order, so the
+// result is the same on every run.
+constexpr int RED_BLOCK = 1024;
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
+                                                      unsigned long long maxmask, double *__restrict__ out) {
+    __shared__ double sh[RED_BLOCK];
     const int k = blockIdx.x;
     const bool ismax = (maxmask >> k) & 1ull;
     const double *row = part + (size_t)k * nblocks;
-    double acc = ismax ? -1e308 : 0.0;
-    for (int b = threadIdx.x; b < nblocks; b += 256) acc = ismax ? fmax(acc, row[b]) : acc + row[b];
-    sh[threadIdx.x] = acc;
+    const double init = ismax ? -1e308 : 0.0;
+    double acc[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) acc[u] = init;
+    for (int b0 = threadIdx.x; b0 < nblocks; b0 += 8 * RED_BLOCK) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int b = b0 + u * RED_BLOCK;
+            if (b < nblocks) acc[u] = ismax ? fmax(acc[u], row[b]) : acc[u] + row[b];
+        }
+    }
+    double a = acc[0];
+#pragma unroll
+    for (int u = 1; u < 8; ++u) a = ismax ? fmax(a, acc[u]) : a + acc[u];
+    sh[threadIdx.x] = a;
     __syncthreads();
-    for (int st = 128; st > 0; st >>= 1) {
+    for (int st = RED_BLOCK / 2; st > 0; st >>= 1) {
         if (threadIdx.x < st)
             sh[threadIdx.x] = ismax ? fmax(sh[threadIdx.x], sh[threadIdx.x + st]) : sh[threadIdx.x] + sh[threadIdx.x + st];
         __syncthreads();
@@ -989,19 +1128,23 @@ __global__ void __launch_bounds__(256) k_reduce(const double *__restrict__ part,
 }
 
 // increment prologue (evolution.c:1115-1128): dg0 = dg; dg += Udot dt (or = set9 under creep)
-__global__ void k_prologue(Fields f, int nvox, double a0, double a1, double a2, double a3, double a4, double a5,
-                           double a6, double a7, double a8, int creep, double s0, double s1, double s2, double s3,
-                           double s4, double s5, double s6, double s7, double s8) {
+struct Vec9 {
+    double v[9];
+};
+
+// evolution.c:1115-1128.  In a creep run DisGrad is reset to the macroscopic value; the
+// fluctuation it held (DisGrad minus the uniform part uold) is carried in fl0 so that the
+// accumulated fluctuation of evolution.c:1198 survives the reset.
+__global__ void k_prologue(Fields f, int nvox, Vec9 add, int creep, Vec9 set, Vec9 uold) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
     const size_t N = (size_t)nvox;
-    const double add[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
-    const double set[9] = {s0, s1, s2, s3, s4, s5, s6, s7, s8};
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const double d = f.dg[k * N + v];
         f.dg0[k * N + v] = d;
-        f.dg[k * N + v] = creep ? set[k] : d + add[k];
+        f.dg[k * N + v] = creep ? set.v[k] : d + add.v[k];
+        if (creep && f.fl0) f.fl0[k * N + v] += d - uold.v[k];
     }
 }
 
@@ -1051,6 +1194,7 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
 #pragma unroll
     for (int s = 0; s < NSYS; ++s)
         if (s < nsys) rho_tot += rs[s] + f.rho_m_old[s * N + v] + sqrt(0.0);
+    f.rho_tot[v] = rho_tot;
 #pragma unroll
     for (int s = 0; s < NSYS; ++s) {
         gam[s] = 0.0;
@@ -1242,13 +1386,29 @@ static int min_radix(const FftPlan &p) {
     return p.nst ? m : 1;
 }
 
+bool fcc_table_matches(const double p[NSYS][5], int nsys) {
+    if (nsys != NSYS) return false;
+    for (int si = 0; si < NSYS; ++si)
+        for (int k = 0; k < 5; ++k)
+            if (p[si][k] != fcc_p(si, k)) return false;
+    return true;
+}
+
 bool make_plan(int n, FftPlan &p) {
     p.n = n;
     p.nst = 0;
     int m = n;
     if (n < 1 || n > 1024) return false;
-    while (m % 4 == 0) { p.rad[p.nst++] = 4; m /= 4; }
-    while (m % 2 == 0) { p.rad[p.nst++] = 2; m /= 2; }
+    // powers of two: as many radix-8 stages as possible without leaving a radix-2 stage
+    // (2^7 = 8*4*4, 2^8 = 8*8*4), then radix 3 and 5
+    int k2 = 0;
+    while (m % 2 == 0) { ++k2; m /= 2; }
+    int n8 = k2 / 3, n4 = 0, n2 = 0;
+    if (k2 % 3 == 1) { if (n8 > 0) { --n8; n4 = 2; } else { n2 = 1; } }
+    else if (k2 % 3 == 2) n4 = 1;
+    for (int i = 0; i < n8; ++i) p.rad[p.nst++] = 8;
+    for (int i = 0; i < n4; ++i) p.rad[p.nst++] = 4;
+    for (int i = 0; i < n2; ++i) p.rad[p.nst++] = 2;
     while (m % 3 == 0) { p.rad[p.nst++] = 3; m /= 3; }
     while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
     if (p.nst > MAX_RADIX_STAGES) return false;
@@ -1264,7 +1424,7 @@ static int tile_for(const FftPlan &p, int per_tile_comps, int maxb, int cap, int
     const int mr = min_radix(p);
     int t = (maxb * FFT_BLOCK * mr) / (p.n * per_tile_comps);
     const int csz = prec == 0 ? 16 : 8;
-    const int tl = FFT_LDS_MAX / (per_tile_comps * p.n * csz);
+    const int tl = (FFT_LDS_MAX - p.n * csz) / (per_tile_comps * p.n * csz);
     if (t > tl) t = tl;
     if (t > cap) t = cap;
     if (t < 1) t = 1;
@@ -1279,16 +1439,16 @@ static void check_launch(const char *name) {
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
                       const void *twz, const double *mean6) {
     // ncomp encoded in mean6 == nullptr? no: always the 6 stress comps here
-    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16, prec);
+    const int TY = tile_for(pz, 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     const double z6[6] = {0, 0, 0, 0, 0, 0};
     const double *m = mean6 ? mean6 : z6;
     if (prec == 0) {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     } else {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, m[0], m[1], m[2], m[3], m[4], m[5]);
     }
@@ -1298,15 +1458,15 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c, const int n[3],
                   const FftPlan &py, const void *twy) {
     (void)sign;
-    const int TX = tile_for(py, 1, 4, n[0] < 16 ? n[0] : 16, prec);
+    const int TX = tile_for(py, 1, 2, n[0] < 16 ? n[0] : 16, prec);
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
-        size_t sm = (size_t)TX * n[1] * sizeof(cx<double>);
+        size_t sm = (size_t)(TX + 1) * n[1] * sizeof(cx<double>);
         SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX);
     } else {
-        size_t sm = (size_t)TX * n[1] * sizeof(cx<float>);
+        size_t sm = (size_t)(TX + 1) * n[1] * sizeof(cx<float>);
         SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX);
     }
@@ -1317,19 +1477,29 @@ static void launch_green_generic(hipStream_t s, int prec, void *specC, const int
                                  const void *twx, const DevConst *cst, const double *dc6, int mode, int ncomp) {
     const int slots = mode == 1 ? 9 : ncomp;
     // MAXB 6 covers 9 slots of two 128-point pencils with a radix-2 stage
-    const int TY = tile_for(px, slots, 6, n[1] < 2 ? n[1] : 2, prec);
+    // two butterflies per thread when one row of `slots` pencils allows it, four otherwise
+    const bool wide = (long)slots * n[0] / min_radix(px) > 2 * FFT_BLOCK;
+    const int TY = tile_for(px, slots, wide ? 4 : 2, n[1] < 2 ? n[1] : 2, prec);
     const int H = n[2] / 2 + 1;
     dim3 grid((n[1] + TY - 1) / TY, H);
     const double z6[6] = {0, 0, 0, 0, 0, 0};
     const double *d = dc6 ? dc6 : z6;
     if (prec == 0) {
-        size_t sm = (size_t)slots * TY * n[0] * sizeof(cx<double>);
-        SRX_LAUNCH((k_green_x<double, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
-                           px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+        size_t sm = (size_t)(slots * TY + 1) * n[0] * sizeof(cx<double>);
+        if (wide)
+            SRX_LAUNCH((k_green_x<double, 4>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
+                       px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+        else
+            SRX_LAUNCH((k_green_x<double, 2>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
+                       px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
     } else {
-        size_t sm = (size_t)slots * TY * n[0] * sizeof(cx<float>);
-        SRX_LAUNCH((k_green_x<float, 6>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
-                           px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+        size_t sm = (size_t)(slots * TY + 1) * n[0] * sizeof(cx<float>);
+        if (wide)
+            SRX_LAUNCH((k_green_x<float, 4>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
+                       px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+        else
+            SRX_LAUNCH((k_green_x<float, 2>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
+                       px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
     }
     check_launch("green_x");
 }
@@ -1341,16 +1511,16 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
                              const void *twz, const double *ddg9, double divisor) {
-    const int TY = tile_for(pz, 1, 4, n[1] < 16 ? n[1] : 16, prec);
+    const int TY = tile_for(pz, 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     const double *a = ddg9;
     if (prec == 0) {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
                            a[6], a[7], a[8]);
     } else {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
                            a[7], a[8]);
@@ -1362,25 +1532,38 @@ int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
 int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, int) {
+               double *partials, int fcc) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
-    if (first_call && rotated) SRX_LAUNCH((k_nr<true, true>), g, b, 0, s, cst, f, nvox, partials);
-    else if (first_call) SRX_LAUNCH((k_nr<true, false>), g, b, 0, s, cst, f, nvox, partials);
-    else if (rotated) SRX_LAUNCH((k_nr<false, true>), g, b, 0, s, cst, f, nvox, partials);
-    else SRX_LAUNCH((k_nr<false, false>), g, b, 0, s, cst, f, nvox, partials);
+    if (fcc) {
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, true>), g, b, 0, s, cst, f, nvox, partials);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, true>), g, b, 0, s, cst, f, nvox, partials);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, true>), g, b, 0, s, cst, f, nvox, partials);
+        else SRX_LAUNCH((k_nr<false, false, true>), g, b, 0, s, cst, f, nvox, partials);
+    } else {
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, false>), g, b, 0, s, cst, f, nvox, partials);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, false>), g, b, 0, s, cst, f, nvox, partials);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, false>), g, b, 0, s, cst, f, nvox, partials);
+        else SRX_LAUNCH((k_nr<false, false, false>), g, b, 0, s, cst, f, nvox, partials);
+    }
     check_launch("nr");
 }
 
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out) {
     const unsigned long long mask = nvals == NR_NPART ? (1ull << 15) : 0ull;
-    SRX_LAUNCH(k_reduce, dim3(nvals), dim3(256), 0, s, partials, nblocks, nvals, mask, out);
+    SRX_LAUNCH(k_reduce, dim3(nvals), dim3(RED_BLOCK), 0, s, partials, nblocks, nvals, mask, out);
     check_launch("reduce");
 }
 
-void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int creep, const double *st) {
+void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int creep, const double *st,
+                     const double *uo) {
     const int B = 256;
-    SRX_LAUNCH(k_prologue, dim3((nvox + B - 1) / B), dim3(B), 0, s, f, nvox, a[0], a[1], a[2], a[3], a[4],
-                       a[5], a[6], a[7], a[8], creep, st[0], st[1], st[2], st[3], st[4], st[5], st[6], st[7], st[8]);
+    Vec9 add, set, uold;
+    for (int k = 0; k < 9; ++k) {
+        add.v[k] = a[k];
+        set.v[k] = st[k];
+        uold.v[k] = uo[k];
+    }
+    SRX_LAUNCH(k_prologue, dim3((nvox + B - 1) / B), dim3(B), 0, s, f, nvox, add, creep, set, uold);
     check_launch("prologue");
 }
 
@@ -1394,14 +1577,14 @@ void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, i
 // test path: forward FFT of ncomp real comps to layout C (x pass done), no Green
 void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int ncomp, void *specB, void *specC,
                                   const int n[3], const FftPlan *pl, void *const *tw) {
-    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16, prec);
+    const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     } else {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     }
@@ -1415,15 +1598,15 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
                                   const int n[3], const FftPlan *pl, void *const *tw, double divisor) {
     launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp);
     launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1]);
-    const int TY = tile_for(pl[2], 1, 4, n[1] < 16 ? n[1] : 16, prec);
+    const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<double>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
                            0., 0.);
     } else {
-        size_t sm = (size_t)TY * n[2] * sizeof(cx<float>);
+        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
                            0.);

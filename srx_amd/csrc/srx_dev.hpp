@@ -41,6 +41,7 @@ struct DevConst {
     double S066[6][6];    // inverse of the average stiffness, sample frame, B basis
     double C0[81];        // average stiffness, 4th order (init.c:370)
     double S0[81];        // its inverse, 4th order (init.c:371)
+    double G6[36];        // C0 contracted over symmetric index pairs: A_ik = sum_b G6[(ik)][b] (n n)_b
     double dt;            // TimeStep
     double nu;            // ElastConst_PhaseI[3], used for every phase (constitutive.c:1383)
     double T, D0;
@@ -62,6 +63,8 @@ struct Fields {
     double *tran0;     // [9][N] TranMat_xt2sa at setup (C_gr is never rotated)
     double *rho_s, *rho_m, *rho_m_old, *rho_F, *rho_P, *gamdot, *gam_acum;   // [12][N]
     double *rho_dipole, *rho_forest, *rho_inplane;                          // [12][N]
+    double *rho_tot;   // [N] rho_tot of the last hardening update (evolution.c:1030-1048)
+    double *fl0;       // [9][N] fluctuation carried over DisGrad resets (creep runs only, else null)
     int *phase;        // [N]
     int *nr_it;        // [N]
 };
@@ -91,10 +94,13 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
                     const DevConst *cst, const double *dc_sig6_sum);
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3],
                              const FftPlan &pz, const void *twz, const double *ddg9, double scale);
+// fcc = 1 when every crystalline phase has the 12 FCC systems of fcc_table_matches
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, int nblocks_max);
+               double *partials, int fcc);
+bool fcc_table_matches(const double p[NSYS][5], int nsys);
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out);
-void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9);
+void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9,
+                     const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,
                 int do_orient, int do_hardening, int istep, double *partials);
 int nr_block_count(int nvox);
