@@ -107,7 +107,7 @@ def main() -> int:
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the whole grid is resident on one MI355X; "
-              "ensembles shard by RVE, see DESIGN.md); --gpus %d is not supported" % args.gpus, file=sys.stderr)
+              "ensembles would shard by RVE); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
 
     from srx_amd import inputs

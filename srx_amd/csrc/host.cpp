@@ -488,7 +488,11 @@ int setup(srx_ctx *c) {
         D.k_tcut = (kT / MY_J_IN_EV) / c2 / c3 / P.b / P.b * 0.1;
         D.k_g0 = 1E-3 * 1.0E10 * c2 * P.b * D.expmq;
     }
+#ifdef SRX_NR_GENERIC_ONLY
+    c->fcc = 0;   // build-time choice: always the table-driven Newton kernel
+#else
     c->fcc = 1;
+#endif
     for (int p = 0; p < NPH; ++p)
         if (c->ph[p].xtal && !fcc_table_matches(c->hc.ph[p].p, c->hc.ph[p].nsys)) c->fcc = 0;
     c->hc.dt = d.TimeStep;

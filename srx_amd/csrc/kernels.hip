@@ -919,15 +919,18 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             for (int s = 0; s < NSYS; ++s) {
                 const double *p = P.p[s];
                 const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
-                double g, g2;
-                slip_rate(rss, tpu[s * NR_BLOCK], inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
-                const double g2dt = g2 * dt;
+                const double tps = tpu[s * NR_BLOCK];
+                if (__any(fabs(rss) > tps)) {
+                    double g, g2;
+                    slip_rate(rss, tps, inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
+                    const double g2dt = g2 * dt;
 #pragma unroll
-                for (int i = 0; i < 5; ++i) {
-                    ed[i] += p[i] * g;
-                    const double gp = g2dt * p[i];
+                    for (int i = 0; i < 5; ++i) {
+                        ed[i] += p[i] * g;
+                        const double gp = g2dt * p[i];
 #pragma unroll
-                    for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
+                        for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
+                    }
                 }
             }
         }
@@ -1037,8 +1040,18 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     res.bad = bad;
 }
 
+// Waves per SIMD the register allocator targets for the Newton kernel (0: no target).
+#ifndef SRX_NR_WAVES_PER_EU
+#define SRX_NR_WAVES_PER_EU 0
+#endif
+#if SRX_NR_WAVES_PER_EU > 0
+#define SRX_NR_ATTR __attribute__((amdgpu_waves_per_eu(SRX_NR_WAVES_PER_EU, 8)))
+#else
+#define SRX_NR_ATTR
+#endif
+
 template <bool FIRST, bool ROT, bool FCC>
-__global__ void __launch_bounds__(NR_BLOCK) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
+__global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                  double *__restrict__ partials) {
     const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
     const size_t N = (size_t)nvox;
