@@ -38,6 +38,9 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix), TFLOP/s
 F_NR = 2300.0                  # flops per voxel per Newton iteration (algorithmic count)
+# HBM bytes per voxel per launch of the Newton kernel, from rocprofv3 FETCH_SIZE (x2.0) and
+# WRITE_SIZE (x1.0) runs calibrated on a 1 GiB device copy (profiles/evp_fft_128cubed_pmc.md)
+NR_TRAFFIC_BYTES_PER_VOXEL = 505.2
 METRIC = "converged voxel-increments/s at 128³, plain FP64 vs mixed-precision at equal stress error"
 
 
@@ -65,7 +68,8 @@ def run_precision(deck: str, precision: int, steps: int, warmup: int) -> dict:
     sig = np.array(reps[-1]["sig_avg"])
     s.close()
     return {"seconds": t1 - t0, "reports": reps, "kernels": kt, "sig_avg": sig,
-            "iters": sum(r["iters"] for r in reps), "nr_total": sum(r["nr_iters_total"] for r in reps)}
+            "iters": sum(r["iters"] for r in reps), "nr_total": sum(r["nr_iters_total"] for r in reps),
+            "nr_wave": sum(r["nr_wave_iters"] for r in reps)}
 
 
 def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: int) -> dict | None:
@@ -96,7 +100,8 @@ def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2, help="timed increments")
-    ap.add_argument("--warmup", type=int, default=1, help="untimed increments before the timed ones")
+    ap.add_argument("--warmup", type=int, default=3,
+                    help="untimed increments before the timed ones (the texture starts to rotate at increment 3)")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--grains", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=20261202)
@@ -156,6 +161,7 @@ def main() -> int:
             "value": args.steps * nvox / r["seconds"], "seconds": r["seconds"],
             "outer_iters_per_increment": r["iters"] / args.steps,
             "nr_iters_per_voxel_per_iter": r["nr_total"] / nvox / iters,
+            "nr_wavefront_divergence": 64.0 * r["nr_wave"] / max(r["nr_total"], 1),
             "kernel_ms": {k: v[0] for k, v in kt.items()},
             "fft_green_gbs": b_fg * iters / (fg_ms * 1e-3) / 1e9 if fg_ms else None,
             "fft_green_frac_hbm": (b_fg * iters / (fg_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if fg_ms else None,
@@ -176,7 +182,8 @@ def main() -> int:
         ach = F_NR * H["nr_total"] / (kt["nr"][0] * 1e-3) / 1e12
         out["roofline"] = {"kernel": "nr", "bound": "mfma", "peak_kind": "FP64 (vector = matrix rate on MI355X)",
                            "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": ach / FP64_PEAK_TFLOPS, "traffic": None,
+                           "frac": ach / FP64_PEAK_TFLOPS, "traffic": NR_TRAFFIC_BYTES_PER_VOXEL * nvox,
+                           "traffic_unit": "bytes per launch (rocprofv3 PMC)",
                            "algorithmic": "%.0f flop per voxel per Newton iteration" % F_NR}
     else:
         rb = 4 if head == "mixed" else 8

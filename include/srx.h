@@ -78,7 +78,10 @@ typedef struct {
     double rho_dipole_avg, rho_forest_avg, rho_inplane_avg; /* AvgDislocation */
     double sig_avg[9], sig_avg1[9], eps_avg[9], edot_avg[9];
     double disgrad_avg_actual[9], velgrad_avg[9];
-    double reserved[16];
+    double nr_wave_iters;       /* sum over groups of 64 consecutive voxels and outer iterations of the
+                                   group's largest NR iteration count (64x this = lane-iterations a
+                                   wavefront-per-group solver executes) */
+    double reserved[15];
 } srx_step_report;
 
 /* Field ids for srx_get_field / srx_set_field (values per voxel in brackets). */

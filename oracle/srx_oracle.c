@@ -1138,6 +1138,7 @@ int orc_step(orc_state *s, orc_report *rep)
     long p;
     int iter = 0, i, j, nrmax = 0;
     long nrtot = 0;
+    double nrwave = 0.0;
     orc_report r;
     memset(&r, 0, sizeof(r));
     if (s->istep >= s->nsteps) return 2006;
@@ -1211,6 +1212,12 @@ int orc_step(orc_state *s, orc_report *rep)
             nrtot += (long)s->nrit[p];
             if ((int)s->nrit[p] > nrmax) nrmax = (int)s->nrit[p];
         }
+        for (p = 0; p < N; p += 64) {
+            double gm = 0.0;
+            long q;
+            for (q = p; q < N && q < p + 64; q++) if (s->nrit[q] > gm) gm = s->nrit[q];
+            nrwave += gm;
+        }
         s->Err_e = ee * s->wgt;
         s->Err_s = es * s->wgt;
         get_smacro(s);
@@ -1264,6 +1271,7 @@ int orc_step(orc_state *s, orc_report *rep)
     r.iters = iter;
     r.nr_iters_max = nrmax;
     r.nr_iters_total = nrtot;
+    r.nr_wave_iters = nrwave;
     r.err_s = s->Err_s;
     r.err_e = s->Err_e;
     r.time_tot = s->TimeTot;

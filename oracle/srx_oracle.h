@@ -55,7 +55,8 @@ typedef struct {
     double rho_dipole_avg, rho_forest_avg, rho_inplane_avg;
     double sig_avg[9], sig_avg1[9], eps_avg[9], edot_avg[9];
     double disgrad_avg_actual[9], velgrad_avg[9];
-    double reserved[16];
+    double nr_wave_iters;       /* sum over groups of 64 consecutive voxels of the group's largest NR count */
+    double reserved[15];
 } orc_report;
 
 /* Field ids, same as include/srx.h; all returned voxel-major, x slowest. */

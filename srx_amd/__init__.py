@@ -46,7 +46,8 @@ class StepReport(C.Structure):
         ("rho_forest_avg", C.c_double), ("rho_inplane_avg", C.c_double),
         ("sig_avg", C.c_double * 9), ("sig_avg1", C.c_double * 9), ("eps_avg", C.c_double * 9),
         ("edot_avg", C.c_double * 9), ("disgrad_avg_actual", C.c_double * 9),
-        ("velgrad_avg", C.c_double * 9), ("reserved", C.c_double * 16),
+        ("velgrad_avg", C.c_double * 9), ("nr_wave_iters", C.c_double),
+        ("reserved", C.c_double * 15),
     ]
 
     def as_dict(self):

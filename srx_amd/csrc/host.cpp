@@ -368,7 +368,6 @@ struct srx_ctx {
     double e_vm = 0, d_vm = 0, s_vm = 0, s_vm1 = 0, TimeTot = 0, err = 0, Err_e = 0, Err_s = 0;
     double sig_sum6[6] = {0, 0, 0, 0, 0, 0};
     int itermax = 0, nsteps = 0, istep = 0, creep = 0;
-    int fcc = 0;   // every crystalline phase has the standard FCC Schmid table
     std::vector<int> grain;
     FILE *fvm = nullptr, *fdd = nullptr, *ferr = nullptr, *fdrx = nullptr;
     // timing
@@ -488,13 +487,6 @@ int setup(srx_ctx *c) {
         D.k_tcut = (kT / MY_J_IN_EV) / c2 / c3 / P.b / P.b * 0.1;
         D.k_g0 = 1E-3 * 1.0E10 * c2 * P.b * D.expmq;
     }
-#ifdef SRX_NR_GENERIC_ONLY
-    c->fcc = 0;   // build-time choice: always the table-driven Newton kernel
-#else
-    c->fcc = 1;
-#endif
-    for (int p = 0; p < NPH; ++p)
-        if (c->ph[p].xtal && !fcc_table_matches(c->hc.ph[p].p, c->hc.ph[p].nsys)) c->fcc = 0;
     c->hc.dt = d.TimeStep;
     c->hc.nu = d.ElastConst_PhaseI[3];
     c->hc.T = d.T__K;
@@ -1037,6 +1029,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     const int rotated = c->istep > 2 ? 1 : 0;   // update_orient first runs at the end of istep 2
     int iter = 0, nrmax = 0;
     long nrtot = 0;
+    double nrwave = 0.0;
     const double divisor = (double)c->nvox;
     for (;;) {
         if (c->opt.fixed_iters > 0) {
@@ -1056,7 +1049,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
         { Timer t(c, 3); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1]); }
         { Timer t(c, 4); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], c->dDisGradAvg, divisor); }
         for (int k = 0; k < 9; ++k) c->uniform[k] += c->dDisGradAvg[k];
-        { Timer t(c, 5); launch_nr(c->stream, c->dc, c->f, N, iter == 1, rotated, c->partials, c->fcc); }
+        { Timer t(c, 5); launch_nr(c->stream, c->dc, c->f, N, iter == 1, rotated, c->partials); }
         { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red); }
         int rc = sync_read(c, NR_NPART);
         if (rc) return rc;
@@ -1064,6 +1057,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
         if (h[16] > 0) return SRX_E_NAN;
         for (int k = 0; k < 6; ++k) c->sig_sum6[k] = h[2 + k];
         nrtot += (long)h[14];
+        nrwave += h[17];
         nrmax = std::max(nrmax, (int)h[15]);
         c->Err_e = h[1] * c->wgt;
         c->Err_s = h[0] * c->wgt;
@@ -1127,6 +1121,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     r.iters = iter;
     r.nr_iters_max = nrmax;
     r.nr_iters_total = nrtot;
+    r.nr_wave_iters = nrwave;
     r.err_s = c->Err_s;
     r.err_e = c->Err_e;
     r.time_tot = c->TimeTot;

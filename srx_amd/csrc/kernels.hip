@@ -646,25 +646,6 @@ __device__ __forceinline__ void sprime(const double a[9], const double a0[9], co
 }
 
 // ---------------------------------------------------------------------------
-// crystal-frame Schmid tensors (B basis) of the 12 FCC {111}<110> systems in the order of the
-// reference's Cu_p.sx, exactly as the host computes them from the slip-mode file.  A phase whose
-// table matches bit for bit runs the specialised Newton kernel, in which the zero components are
-// compile-time constants and their products are skipped (adding +-0 changes nothing).
-// ---------------------------------------------------------------------------
-constexpr double FCC_A = 0.28867513459481287, FCC_B = 0.5, FCC_C = 0.5773502691896257;
-__host__ __device__ constexpr double fcc_p(int s, int k) {
-    constexpr double T[12][5] = {
-        {FCC_A, -FCC_B, 0.0, FCC_A, FCC_A},   {-FCC_A, -FCC_B, FCC_A, 0.0, FCC_A},
-        {-FCC_C, 0.0, FCC_A, -FCC_A, 0.0},    {-FCC_A, FCC_B, 0.0, -FCC_A, FCC_A},
-        {-FCC_A, -FCC_B, -FCC_A, 0.0, -FCC_A}, {-FCC_C, 0.0, -FCC_A, -FCC_A, 0.0},
-        {-FCC_A, FCC_B, 0.0, FCC_A, FCC_A},   {-FCC_A, -FCC_B, FCC_A, 0.0, -FCC_A},
-        {-FCC_C, 0.0, FCC_A, FCC_A, 0.0},     {FCC_A, -FCC_B, 0.0, -FCC_A, FCC_A},
-        {-FCC_A, -FCC_B, -FCC_A, 0.0, FCC_A}, {-FCC_C, 0.0, -FCC_A, FCC_A, 0.0},
-    };
-    return T[s][k];
-}
-
-// ---------------------------------------------------------------------------
 // constitutive helpers (constitutive.c:556-684)
 // ---------------------------------------------------------------------------
 // 1/x: hardware reciprocal refined by two Newton steps (correctly rounded to a
@@ -768,7 +749,7 @@ __device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const do
         }
 }
 
-template <bool FIRST, bool ROT, int PH, bool FCC>
+template <bool FIRST, bool ROT, int PH>
 __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
                                          double *lds, NrResult &res) {
     // compile-time phase index: every material constant is a scalar load at a fixed offset
@@ -890,47 +871,21 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
         const double *tpu = (FIRST && it == 1) ? tpo : tp;
-        if (FCC) {
-#pragma unroll
-            for (int s = 0; s < NSYS; ++s) {
-                double rss = 0.0;
-#pragma unroll
-                for (int k = 0; k < 5; ++k)
-                    if (fcc_p(s, k) != 0.0) rss += fcc_p(s, k) * sg[k];
-                const double tps = tpu[s * NR_BLOCK];
-                // a system inactive on every lane of the wavefront contributes exactly nothing
-                if (__any(fabs(rss) > tps)) {
-                    double g, g2;
-                    slip_rate(rss, tps, inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
-                    const double g2dt = g2 * dt;
-#pragma unroll
-                    for (int i = 0; i < 5; ++i) {
-                        if (fcc_p(s, i) == 0.0) continue;
-                        ed[i] += fcc_p(s, i) * g;
-                        const double gp = g2dt * fcc_p(s, i);
-#pragma unroll
-                        for (int j = i; j < 5; ++j)
-                            if (fcc_p(s, j) != 0.0) K[ps(i, j)] += gp * fcc_p(s, j);
-                    }
-                }
-            }
-        } else {
 #pragma unroll 2
-            for (int s = 0; s < NSYS; ++s) {
-                const double *p = P.p[s];
-                const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
-                const double tps = tpu[s * NR_BLOCK];
-                if (__any(fabs(rss) > tps)) {
-                    double g, g2;
-                    slip_rate(rss, tps, inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
-                    const double g2dt = g2 * dt;
+        for (int s = 0; s < NSYS; ++s) {
+            const double *p = P.p[s];
+            const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
+            const double tps = tpu[s * NR_BLOCK];
+            if (__any(fabs(rss) > tps)) {
+                double g, g2;
+                slip_rate(rss, tps, inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
+                const double g2dt = g2 * dt;
 #pragma unroll
-                    for (int i = 0; i < 5; ++i) {
-                        ed[i] += p[i] * g;
-                        const double gp = g2dt * p[i];
+                for (int i = 0; i < 5; ++i) {
+                    ed[i] += p[i] * g;
+                    const double gp = g2dt * p[i];
 #pragma unroll
-                        for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
-                    }
+                    for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
                 }
             }
         }
@@ -1050,7 +1005,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #define SRX_NR_ATTR
 #endif
 
-template <bool FIRST, bool ROT, bool FCC>
+template <bool FIRST, bool ROT>
 __global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                  double *__restrict__ partials) {
     const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
@@ -1070,9 +1025,9 @@ __global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__r
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, FCC>(cst, f, v, N, lds, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0>(cst, f, v, N, lds, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, FCC>(cst, f, v, N, lds, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1>(cst, f, v, N, lds, r);
             }
         }
         todo &= ~__ballot(mine);
@@ -1083,7 +1038,7 @@ __global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__r
         for (int k = 0; k < 6; ++k) f.sig[k * N + v] = r.sig[k];
         f.nr_it[v] = r.it;
     }
-    // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan
+    // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan, wavefront nr_max
     double vals[NR_NPART];
     vals[0] = r.cs;
     vals[1] = r.ce;
@@ -1095,8 +1050,10 @@ __global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__r
     vals[14] = (double)r.it;
     vals[15] = (double)r.it;
     vals[16] = (double)r.bad;
-    vals[17] = 0.0;
     const int lane = threadIdx.x & 63;
+    // the wavefront's largest iteration count (the lane-iterations it executed / 64)
+    const double wmax = wave_max((double)r.it);
+    vals[17] = lane == 0 ? wmax : 0.0;
 #pragma unroll
     for (int k = 0; k < NR_NPART; ++k) {
         const double sred = (k == 15) ? wave_max(vals[k]) : wave_sum(vals[k]);
@@ -1399,14 +1356,6 @@ static int min_radix(const FftPlan &p) {
     return p.nst ? m : 1;
 }
 
-bool fcc_table_matches(const double p[NSYS][5], int nsys) {
-    if (nsys != NSYS) return false;
-    for (int si = 0; si < NSYS; ++si)
-        for (int k = 0; k < 5; ++k)
-            if (p[si][k] != fcc_p(si, k)) return false;
-    return true;
-}
-
 bool make_plan(int n, FftPlan &p) {
     p.n = n;
     p.nst = 0;
@@ -1545,19 +1494,12 @@ int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
 int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, int fcc) {
+               double *partials) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
-    if (fcc) {
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, true>), g, b, 0, s, cst, f, nvox, partials);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, true>), g, b, 0, s, cst, f, nvox, partials);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, true>), g, b, 0, s, cst, f, nvox, partials);
-        else SRX_LAUNCH((k_nr<false, false, true>), g, b, 0, s, cst, f, nvox, partials);
-    } else {
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, false>), g, b, 0, s, cst, f, nvox, partials);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, false>), g, b, 0, s, cst, f, nvox, partials);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, false>), g, b, 0, s, cst, f, nvox, partials);
-        else SRX_LAUNCH((k_nr<false, false, false>), g, b, 0, s, cst, f, nvox, partials);
-    }
+    if (first_call && rotated) SRX_LAUNCH((k_nr<true, true>), g, b, 0, s, cst, f, nvox, partials);
+    else if (first_call) SRX_LAUNCH((k_nr<true, false>), g, b, 0, s, cst, f, nvox, partials);
+    else if (rotated) SRX_LAUNCH((k_nr<false, true>), g, b, 0, s, cst, f, nvox, partials);
+    else SRX_LAUNCH((k_nr<false, false>), g, b, 0, s, cst, f, nvox, partials);
     check_launch("nr");
 }
 

@@ -77,7 +77,7 @@ struct FftPlan {
 };
 
 // Per-block partial sums of the NR kernel epilogue.
-constexpr int NR_NPART = 18;   // conv_s, conv_e, sig6[6], sig6_ph1[6], nr_sum, nr_max, nan_count, pad
+constexpr int NR_NPART = 18;   // conv_s, conv_e, sig6[6], sig6_ph1[6], nr_sum, nr_max, nan_count, wave nr_max sum
 // Per-block partial sums of the end-of-increment kernel.
 constexpr int END_NPART = 24;  // eps6[6], edot6[6], rhoF, rhoP, rhos, rhom, rhodip, rhofor, rhoinp, rsl, rlc, nan, pad..
 
@@ -94,10 +94,8 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
                     const DevConst *cst, const double *dc_sig6_sum);
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3],
                              const FftPlan &pz, const void *twz, const double *ddg9, double scale);
-// fcc = 1 when every crystalline phase has the 12 FCC systems of fcc_table_matches
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, int fcc);
-bool fcc_table_matches(const double p[NSYS][5], int nsys);
+               double *partials);
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out);
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9,
                      const double *uold9);

@@ -50,6 +50,8 @@ def test_fixed_iterations_match_oracle(tmp_path, oracle_lib):
         ro, rg = o.step(), g.step()
         assert rg["iters"] == ro["iters"] == 6
         assert rg["nr_iters_max"] == ro["nr_iters_max"]
+        assert rg["nr_iters_total"] == ro["nr_iters_total"]
+        assert rg["nr_wave_iters"] == ro["nr_wave_iters"] > 0
         compare_reports(rg, ro, 1e-10)
         compare_fields(g, o, 1e-9)
 
