@@ -72,7 +72,8 @@ def run_precision(deck: str, precision: int, steps: int, warmup: int) -> dict:
             "nr_wave": sum(r["nr_wave_iters"] for r in reps)}
 
 
-def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: int) -> dict | None:
+def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: int,
+                 gpu_nr_per_iter: float) -> dict | None:
     """The FP64 CPU oracle on a bounded sample: the first `outer_iters` outer iterations of
     increment 0 of the same 128^3 deck, scaled to an increment by the GPU's iterations/increment."""
     try:
@@ -90,10 +91,11 @@ def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: 
     value = nvox / (t_iter * iters_per_increment)
     return {"value": value, "unit": "voxel-increments/s", "cores": threads, "kind": "port",
             "sample": "CPU oracle (OpenMP, FP64), first %d outer iterations of increment 0 at %d^3 "
-                      "(%.2f s/iteration, %.2f Newton iterations/voxel/iteration); scaled by the GPU run's "
+                      "(%.2f s/iteration, %.2f Newton iterations/voxel/iteration against %.2f in the GPU's "
+                      "timed increments, so an upper bound for the CPU); scaled by the GPU run's "
                       "%.1f outer iterations per increment" % (outer_iters, round(nvox ** (1 / 3)), t_iter,
                                                                r["nr_iters_total"] / nvox / outer_iters,
-                                                               iters_per_increment)}
+                                                               gpu_nr_per_iter, iters_per_increment)}
 
 
 def main() -> int:
@@ -202,7 +204,8 @@ def main() -> int:
     if not args.no_cpu_baseline:
         ipi = (res.get("fp64") or H)["iters"] / args.steps
         print("bench: cpu baseline sample", file=sys.stderr, flush=True)
-        out["cpu_baseline"] = cpu_baseline(deck, nvox, ipi, args.cpu_iters)
+        ref = res.get("fp64") or H
+        out["cpu_baseline"] = cpu_baseline(deck, nvox, ipi, args.cpu_iters, ref["nr_total"] / nvox / max(ref["iters"], 1))
     print(json.dumps(out))
     return 0
 

@@ -683,17 +683,48 @@ __device__ __forceinline__ void sys_const(const DevPhase &P, double rP, double r
     g0 = P.k_g0 * rmn * rcp(sF);
 }
 
+// sinh(x) and cosh(x) for x >= 0 from one reduction x = k ln2 + r, |r| <= ln2/2 (Cody-Waite
+// split of ln2): C = cosh r and S = sinh r by their Taylor series (to r^12 and r^13, truncation
+// below 3e-18), e^x / 2 = 2^(k-1) (C + S), e^-x / 2 = 2^(-k-1) (C - S).  For k = 0 the series are
+// the result, so small arguments lose nothing to cancellation.  Accurate to a few ulp; the two
+// short series replace one long exp polynomial and a reciprocal.
+__device__ __forceinline__ void sinh_cosh(double x, double &sh, double &ch) {
+    const double kf = rint(x * 1.4426950408889634);
+    double r = fma(-kf, 6.93147180369123816490e-01, x);
+    r = fma(-kf, 1.90821492927058770002e-10, r);
+    const double r2 = r * r;
+    double c = 2.08767569878680989792e-09;            // 1/12!
+    c = fma(c, r2, 2.75573192239858906526e-07);       // 1/10!
+    c = fma(c, r2, 2.48015873015873015873e-05);       // 1/8!
+    c = fma(c, r2, 1.38888888888888888889e-03);       // 1/6!
+    c = fma(c, r2, 4.16666666666666666667e-02);       // 1/4!
+    c = fma(c, r2, 0.5);
+    c = fma(c, r2, 1.0);
+    double o = 1.60590438368216145994e-10;            // 1/13!
+    o = fma(o, r2, 2.50521083854417187751e-08);       // 1/11!
+    o = fma(o, r2, 2.75573192239858906526e-06);       // 1/9!
+    o = fma(o, r2, 1.98412698412698412698e-04);       // 1/7!
+    o = fma(o, r2, 8.33333333333333333333e-03);       // 1/5!
+    o = fma(o, r2, 1.66666666666666666667e-01);       // 1/3!
+    o = fma(o, r2, 1.0);
+    o *= r;
+    const int k = (int)kf;
+    const double ep = ldexp(c + o, k - 1), em = ldexp(c - o, -k - 1);   // e^x / 2, e^-x / 2
+    sh = k == 0 ? o : ep - em;
+    ch = k == 0 ? c : ep + em;
+}
+
 // slip rate and its derivative, FORWARD_PLUS_BACKWARD (constitutive.c:634-684):
 //   g  = g0 sinh((|rss|-tp)/(tau0+tau_cut)) sign(rss),  g2 = g0 cosh(..)/(tau0+tau_cut), for |rss| > tp
-// Branch free: every system evaluates the exponential, inactive ones are masked.
+// Branch free: inactive systems evaluate at 0 and are masked.
 __device__ __forceinline__ void slip_rate(double rss, double tp, double inv, double g0, double &g, double &g2) {
     const double ar = fabs(rss);
     const bool act = ar > tp;
-    const double e = exp(act ? (ar - tp) * inv : 0.0);
-    const double ei = rcp(e);
-    const double gg = act ? g0 * (0.5 * (e - ei)) : 0.0;
+    double sh, ch;
+    sinh_cosh(act ? (ar - tp) * inv : 0.0, sh, ch);
+    const double gg = act ? g0 * sh : 0.0;
     g = (rss > 0) ? gg : -gg;
-    g2 = act ? g0 * (0.5 * (e + ei)) * inv : 0.0;
+    g2 = act ? g0 * ch * inv : 0.0;
 }
 
 // ---------------------------------------------------------------------------
@@ -707,6 +738,10 @@ __device__ __forceinline__ void slip_rate(double rss, double tp, double inv, dou
 // used for Edot and the errors are the reference's.
 // ---------------------------------------------------------------------------
 constexpr int NR_BLOCK = 64;
+// slip systems evaluated per trip of the Newton kernel's system loop
+#ifndef SRX_NR_UNROLL
+#define SRX_NR_UNROLL 2
+#endif
 
 __device__ __forceinline__ double wave_sum(double v) {
 #pragma unroll
@@ -749,9 +784,21 @@ __device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const do
         }
 }
 
+// Staging slots of the Newton kernel's inputs in LDS (one 64-voxel slice per component).
+constexpr int ST_SIG = 0, ST_EPS = 6, ST_DG = 11, ST_TRAN = 20, ST_TRAN0 = 29;
+constexpr int ST_COMPS_ROT = 38, ST_COMPS = 29;
+
+// Input component k of a field: from the LDS staging slice (lane-major, stg points at this
+// lane's element of slot 0) when STAGED, else from global memory.
+template <bool STAGED>
+__device__ __forceinline__ double nr_in(const double *stg, int slot, const double *g, int k, size_t N, int v) {
+    return STAGED ? stg[(slot + k) * NR_BLOCK] : g[k * N + v];
+}
+
 template <bool FIRST, bool ROT, int PH>
 __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                                         double *lds, NrResult &res) {
+                                         double *lds, const double *stg, NrResult &res) {
+    constexpr bool STAGED = !FIRST;
     // compile-time phase index: every material constant is a scalar load at a fixed offset
     const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
@@ -760,18 +807,18 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     {
         double a[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+        for (int k = 0; k < 9; ++k) a[k] = nr_in<STAGED>(stg, ST_TRAN, f.tran, k, N, v);
         double Q[5][5];
         q5_from(a, Q);
         double lam6[6], ec[6], ep[5];
 #pragma unroll
-        for (int k = 0; k < 6; ++k) lam6[k] = f.sig[k * N + v];
+        for (int k = 0; k < 6; ++k) lam6[k] = nr_in<STAGED>(stg, ST_SIG, f.sig, k, N, v);
 #pragma unroll
-        for (int k = 0; k < 5; ++k) ep[k] = f.eps[k * N + v];
+        for (int k = 0; k < 5; ++k) ep[k] = nr_in<STAGED>(stg, ST_EPS, f.eps, k, N, v);
         {
             double d[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) d[k] = f.dg[k * N + v];
+            for (int k = 0; k < 9; ++k) d[k] = nr_in<STAGED>(stg, ST_DG, f.dg, k, N, v);
             // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
             const double s00 = d[0], s11 = d[4], s22 = d[8];
             const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
@@ -811,7 +858,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         if (ROT) {
             double a0[9], Sp[15];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
+            for (int k = 0; k < 9; ++k) a0[k] = nr_in<STAGED>(stg, ST_TRAN0, f.tran0, k, N, v);
             sprime(a, a0, P.sinv, Sp);
 #pragma unroll
             for (int k = 0; k < 15; ++k) A[k] += Sp[k];
@@ -871,7 +918,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
         const double *tpu = (FIRST && it == 1) ? tpo : tp;
-#pragma unroll 2
+#pragma unroll SRX_NR_UNROLL
         for (int s = 0; s < NSYS; ++s) {
             const double *p = P.p[s];
             const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
@@ -947,6 +994,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     for (int i = 0; i < 5; ++i) cs2 += (sg[i] - lam[i]) * (sg[i] - lam[i]);
     res.cs = sqrt(cs2);
     {
+        // the staging slots now hold the per-system constants: orientations from global memory
         double a[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
@@ -995,24 +1043,85 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     res.bad = bad;
 }
 
-// Waves per SIMD the register allocator targets for the Newton kernel (0: no target).
+// Waves per SIMD the register allocator targets for the Newton kernel.  The kernel is bound by
+// FP64 dependency chains and LDS/scalar-load latency, not by HBM: a second wavefront per SIMD
+// to issue from is worth a few spilled registers (measured 31% faster than one wavefront with
+// no spills).  The first call of an increment (more inputs, more live state) keeps one.
 #ifndef SRX_NR_WAVES_PER_EU
-#define SRX_NR_WAVES_PER_EU 0
+#define SRX_NR_WAVES_PER_EU 2
 #endif
-#if SRX_NR_WAVES_PER_EU > 0
-#define SRX_NR_ATTR __attribute__((amdgpu_waves_per_eu(SRX_NR_WAVES_PER_EU, 8)))
-#else
-#define SRX_NR_ATTR
+#ifndef SRX_NR_FIRST_WAVES_PER_EU
+#define SRX_NR_FIRST_WAVES_PER_EU 1
 #endif
 
+typedef __attribute__((address_space(3))) void lds_void;
+
+// asynchronous global -> LDS copy of `bytes` (4 or 16) per lane to lds + lane * bytes
+__device__ __forceinline__ void copy_to_lds(const void *src, double *lds, int bytes) {
+#if __HIP_DEVICE_COMPILE__
+    if (bytes == 16) __builtin_amdgcn_global_load_lds(src, (lds_void *)lds, 16, 0, 0);
+    else __builtin_amdgcn_global_load_lds(src, (lds_void *)lds, 4, 0, 0);
+#endif
+}
+
+// Copies components [0, NC) of the staging layout for voxels [v0, v0 + 64) of this block into
+// `stage` (slot c at stage + 64 c).  With an even voxel count every component slice is 16-byte
+// aligned and one 16-byte load per lane moves two components; otherwise 4-byte loads.  Lanes
+// past the last voxel re-read the last valid element (the values are never used).
+template <int NC>
+__device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage) {
+    const size_t N = (size_t)nvox;
+    const int lane = threadIdx.x;
+    const int v0 = blockIdx.x * NR_BLOCK;
+    auto base = [&](int c) -> const double * {
+        return c < ST_EPS ? f.sig + (size_t)(c - ST_SIG) * N
+             : c < ST_DG ? f.eps + (size_t)(c - ST_EPS) * N
+             : c < ST_TRAN ? f.dg + (size_t)(c - ST_DG) * N
+             : c < ST_TRAN0 ? f.tran + (size_t)(c - ST_TRAN) * N
+             : f.tran0 + (size_t)(c - ST_TRAN0) * N;
+    };
+    if ((nvox & 1) == 0) {
+        const int pair = min(v0 + 2 * (lane & 31), nvox - 2);
+#pragma unroll
+        for (int c = 0; c < NC; c += 2) {
+            const int cc = min(c + (lane >> 5), NC - 1);
+            copy_to_lds(base(cc) + pair, stage + c * NR_BLOCK, 16);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < NC; ++c)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+                const int d = h * 64 + lane;
+                const int vv = min(v0 + (d >> 1), nvox - 1);
+                const char *src = reinterpret_cast<const char *>(base(c) + vv) + 4 * (d & 1);
+                copy_to_lds(src, stage + c * NR_BLOCK + h * 32, 4);
+            }
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+}
+
 template <bool FIRST, bool ROT>
-__global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
+__global__ void __launch_bounds__(NR_BLOCK)
+__attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8))) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                  double *__restrict__ partials) {
     const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
-    __shared__ double lds[(FIRST ? 4 : 3) * NSYS * NR_BLOCK];
+    // After the first call of an increment, the wavefront's inputs (stress, plastic strain,
+    // DisGrad, orientations) are copied to LDS by one burst of asynchronous global->LDS loads
+    // and a single wait, instead of several dependent round trips to HBM.  Each lane's
+    // per-system constants then overwrite its own, already consumed, staging slots, so the
+    // block needs no more LDS than the larger of the two (an even number of slots: the
+    // 16-byte copies fill slots in pairs).
+    constexpr int NST = FIRST ? 0 : (ROT ? ST_COMPS_ROT : ST_COMPS);
+    constexpr int NCONST = (FIRST ? 4 : 3) * NSYS;
+    constexpr int NSLOT = ((NST + 1) & ~1) > NCONST ? ((NST + 1) & ~1) : NCONST;
+    __shared__ __align__(16) double lds[NSLOT * NR_BLOCK];
+    if constexpr (!FIRST) stage_inputs<NST>(f, nvox, lds);
+    const double *stg = lds + threadIdx.x;
     NrResult r;
     r.cs = 0.0; r.ce = 0.0; r.it = 0; r.bad = 0;
 #pragma unroll
@@ -1025,9 +1134,9 @@ __global__ void __launch_bounds__(NR_BLOCK) SRX_NR_ATTR k_nr(const DevConst *__r
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0>(cst, f, v, N, lds, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0>(cst, f, v, N, lds, stg, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1>(cst, f, v, N, lds, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1>(cst, f, v, N, lds, stg, r);
             }
         }
         todo &= ~__ballot(mine);
