@@ -185,10 +185,87 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
     __syncthreads();
 }
 
+// The same stage with the pencil length and span known at compile time (powers of two):
+// the index arithmetic reduces to shifts and masks.
+template <typename T, int R, int MAXB, int N, int L>
+__device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign) {
+    constexpr int nb = N / R;
+    constexpr int tstep = N / (L * R);
+    const int total = npen * nb;
+    cx<T> v[MAXB][R];
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        const int t = threadIdx.x + m * FFT_BLOCK;
+        if (t < total) {
+            const int p = t / nb, j = t % nb;
+            const cx<T> *src = buf + p * N + j;
+#pragma unroll
+            for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < MAXB; ++m) {
+        const int t = threadIdx.x + m * FFT_BLOCK;
+        if (t < total) {
+            const int p = t / nb, j = t % nb;
+            const int k = j % L;
+            if (k) {
+#pragma unroll
+                for (int r = 1; r < R; ++r) {
+                    cx<T> w = tw[r * k * tstep];
+                    if (sign > 0) w.y = -w.y;
+                    v[m][r] = cmul(v[m][r], w);
+                }
+            }
+            Dft<T, R>::run(v[m], sign);
+            cx<T> *dst = buf + p * N + (j / L) * L * R + k;
+#pragma unroll
+            for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
+        }
+    }
+    __syncthreads();
+}
+
+// radix of stage st of a power-of-two length N, in the order make_plan chooses
+// (radix 8 as often as possible without a radix-2 stage, then 4s, then a 2)
+__host__ __device__ constexpr int pow2_radix(int N, int st) {
+    int k2 = 0;
+    while ((1 << k2) < N) ++k2;
+    int n8 = k2 / 3, n4 = 0, n2 = 0;
+    if (k2 % 3 == 1) { if (n8 > 0) { --n8; n4 = 2; } else { n2 = 1; } }
+    else if (k2 % 3 == 2) n4 = 1;
+    if (st < n8) return 8;
+    if (st < n8 + n4) return 4;
+    if (st < n8 + n4 + n2) return 2;
+    return 0;
+}
+
+template <typename T, int MAXB, int N, int ST, int L>
+__device__ __forceinline__ void lds_fft_pow2(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign) {
+    constexpr int R = pow2_radix(N, ST);
+    if constexpr (R != 0) {
+        stockham_stage_ct<T, R, MAXB, N, L>(buf, npen, tw, sign);
+        lds_fft_pow2<T, MAXB, N, ST + 1, L * R>(buf, npen, tw, sign);
+    }
+}
+
 // Full 1-D transform of npen LDS-resident pencils.  Caller guarantees
 // npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.
-template <typename T, int MAXB>
+// POW2: lengths 32..256 use the compile-time stages (measured faster in the x and y passes;
+// in the z passes, which interleave the transform with the real/complex packing, the extra
+// registers cost an occupancy step and the runtime stages are faster).
+template <typename T, int MAXB, bool POW2 = true>
 __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign) {
+    if constexpr (POW2) {
+        switch (pl.n) {
+        case 32: lds_fft_pow2<T, MAXB, 32, 0, 1>(buf, npen, tw, sign); return;
+        case 64: lds_fft_pow2<T, MAXB, 64, 0, 1>(buf, npen, tw, sign); return;
+        case 128: lds_fft_pow2<T, MAXB, 128, 0, 1>(buf, npen, tw, sign); return;
+        case 256: lds_fft_pow2<T, MAXB, 256, 0, 1>(buf, npen, tw, sign); return;
+        default: break;
+        }
+    }
     int L = 1;
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
@@ -203,11 +280,21 @@ __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__
     }
 }
 
+// Waves per SIMD targeted for the z passes (0: the compiler's choice).
+#ifndef SRX_ZPASS_WPE
+#define SRX_ZPASS_WPE 0
+#endif
+#if SRX_ZPASS_WPE > 0
+#define SRX_ZPASS_ATTR __attribute__((amdgpu_waves_per_eu(SRX_ZPASS_WPE, 8)))
+#else
+#define SRX_ZPASS_ATTR
+#endif
+
 // ---------------------------------------------------------------------------
 // forward z: real comps -> half spectrum, layout B [c][kz][x][y]
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(FFT_BLOCK) k_fwd_z(const double *__restrict__ in, cx<T> *__restrict__ specB,
+__global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double *__restrict__ in, cx<T> *__restrict__ specB,
                                                      int ncomp, int n0, int n1, int n2, FftPlan pz,
                                                      const cx<T> *__restrict__ tw, int TY, double m0, double m1,
                                                      double m2, double m3, double m4, double m5) {
@@ -234,7 +321,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fwd_z(const double *__restrict__ 
             buf[yy * n2 + z] = cx<T>{T(va), T(vb)};
         }
         __syncthreads();
-        lds_fft<T, 2>(buf, ty, pz, stw, -1);
+        lds_fft<T, 2, false>(buf, ty, pz, stw, -1);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
             const int kz = i / ty, yy = i - kz * ty;
             const cx<T> z1 = buf[yy * n2 + kz];
@@ -526,7 +613,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
 // grid: (n0, ceil(n1/TY)); `dg` may be an output (scale mode) when accumulate=0
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(FFT_BLOCK) k_inv_z(const cx<T> *__restrict__ specB, double *__restrict__ dg,
+__global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> *__restrict__ specB, double *__restrict__ dg,
                                                      int ncomp, int n0, int n1, int n2, FftPlan pz,
                                                      const cx<T> *__restrict__ tw, int TY, int accumulate,
                                                      double divisor, double a0, double a1, double a2, double a3,
@@ -554,7 +641,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_inv_z(const cx<T> *__restrict__ s
             if (kz >= 1 && kz <= n2 - H) buf[yy * n2 + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
         }
         __syncthreads();
-        lds_fft<T, 2>(buf, ty, pz, stw, +1);
+        lds_fft<T, 2, false>(buf, ty, pz, stw, +1);
         for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
             const int yy = i / n2, z = i - yy * n2;
             const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
