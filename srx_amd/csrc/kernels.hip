@@ -141,6 +141,15 @@ struct Dft<T, 8> {
 
 constexpr int FFT_BLOCK = 256;
 
+// n / d for 0 <= n with n * d < 2^32, as one high multiply by ceil(2^32 / d) (exact in that
+// range); replaces the long integer-division sequence in the index arithmetic of the passes.
+struct FastDiv {
+    unsigned m;
+    int d;
+    __device__ __forceinline__ explicit FastDiv(int dd) : m(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u), d(dd) {}
+    __device__ __forceinline__ int div(int n) const { return d > 1 ? (int)__umulhi((unsigned)n, m) : n; }
+};
+
 // One Stockham stage (radix R, sub-transform span L) over `npen` pencils of
 // length N stored pencil-major in LDS.  Every butterfly of the stage is held
 // in registers between the load and the store, so the stage runs in place.
@@ -150,12 +159,13 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
     const int nb = N / R;
     const int total = npen * nb;
     const int tstep = N / (L * R);
+    const FastDiv dnb(nb), dL(L);
     cx<T> v[MAXB][R];
 #pragma unroll
     for (int m = 0; m < MAXB; ++m) {
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
-            const int p = t / nb, j = t - p * nb;
+            const int p = dnb.div(t), j = t - p * nb;
             const cx<T> *src = buf + p * N + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
@@ -166,8 +176,8 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
     for (int m = 0; m < MAXB; ++m) {
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
-            const int p = t / nb, j = t - p * nb;
-            const int k = j % L;
+            const int p = dnb.div(t), j = t - p * nb;
+            const int jl = dL.div(j), k = j - jl * L;
             if (k) {
 #pragma unroll
                 for (int r = 1; r < R; ++r) {
@@ -177,7 +187,7 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
                 }
             }
             Dft<T, R>::run(v[m], sign);
-            cx<T> *dst = buf + p * N + (j / L) * L * R + k;
+            cx<T> *dst = buf + p * N + jl * L * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
         }
@@ -308,13 +318,14 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
     const size_t N = (size_t)n0 * n1 * n2;
     const size_t HS = (size_t)H * n0 * n1;
     const double mean[6] = {m0, m1, m2, m3, m4, m5};
+    const FastDiv dn2(n2), dty(ty);
     for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     for (int ca = 0; ca < ncomp; ca += 2) {
         const int cb = ca + 1;
         const bool two = cb < ncomp;
         const double ma = ca < 6 ? mean[ca] : 0.0, mb = (two && cb < 6) ? mean[cb] : 0.0;
         for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
-            const int yy = i / n2, z = i - yy * n2;
+            const int yy = dn2.div(i), z = i - yy * n2;
             const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
             const double va = in[ca * N + g] - ma;
             const double vb = two ? in[cb * N + g] - mb : 0.0;
@@ -323,7 +334,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
         __syncthreads();
         lds_fft<T, 2, false>(buf, ty, pz, stw, -1);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
-            const int kz = i / ty, yy = i - kz * ty;
+            const int kz = dty.div(i), yy = i - kz * ty;
             const cx<T> z1 = buf[yy * n2 + kz];
             const cx<T> z2 = buf[yy * n2 + (kz == 0 ? 0 : n2 - kz)];
             const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
@@ -354,14 +365,15 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     const int c = blockIdx.z;
     const size_t plane = (size_t)n0 * n1;
     const size_t base = ((size_t)c * H + kz) * plane;
+    const FastDiv dn1(n1), dtx(tx);
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
-            const int xx = i / n1, y = i - xx * n1;
+            const int xx = dn1.div(i), y = i - xx * n1;
             buf[xx * n1 + y] = in[base + (size_t)(x0 + xx) * n1 + y];
         }
     } else {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
-            const int y = i / tx, xx = i - y * tx;
+            const int y = dtx.div(i), xx = i - y * tx;
             buf[xx * n1 + y] = in[base + (size_t)y * n0 + x0 + xx];
         }
     }
@@ -369,12 +381,12 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     lds_fft<T, 2>(buf, tx, py, stw, b_to_c ? -1 : +1);
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
-            const int y = i / tx, xx = i - y * tx;
+            const int y = dtx.div(i), xx = i - y * tx;
             out[base + (size_t)y * n0 + x0 + xx] = buf[xx * n1 + y];
         }
     } else {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
-            const int xx = i / n1, y = i - xx * n1;
+            const int xx = dn1.div(i), y = i - xx * n1;
             out[base + (size_t)(x0 + xx) * n1 + y] = buf[xx * n1 + y];
         }
     }
@@ -513,10 +525,11 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
     const int slot = ty * n0;
     const int cin = mode == 1 ? 6 : ncomp;
     const int cout = mode == 1 ? 9 : ncomp;
+    const FastDiv dslot(slot), dn0(n0);
     cx<T> *stw = buf + (mode == 1 ? 9 : ncomp) * slot;
     for (int i = threadIdx.x; i < n0; i += FFT_BLOCK) stw[i] = tw[i];
     for (int i = threadIdx.x; i < cin * slot; i += FFT_BLOCK) {
-        const int c = i / slot, r = i - c * slot, yy = r / n0, x = r - yy * n0;
+        const int c = dslot.div(i), r = i - c * slot, yy = dn0.div(r), x = r - yy * n0;
         buf[c * slot + yy * n0 + x] = spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x];
     }
     __syncthreads();
@@ -526,7 +539,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
         unit_k(kz, n2, tzv);
         const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
         for (int i = threadIdx.x; i < slot; i += FFT_BLOCK) {
-            const int yy = i / n0, kx = i - yy * n0, ky = y0 + yy;
+            const int yy = dn0.div(i), kx = i - yy * n0, ky = y0 + yy;
             unit_k(ky, n1, tyv);
             double txv;
             unit_k(kx, n0, txv);
@@ -603,7 +616,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
     }
     if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, stw, +1);
     for (int i = threadIdx.x; i < cout * slot; i += FFT_BLOCK) {
-        const int c = i / slot, r = i - c * slot, yy = r / n0, x = r - yy * n0;
+        const int c = dslot.div(i), r = i - c * slot, yy = dn0.div(r), x = r - yy * n0;
         spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x] = buf[c * slot + yy * n0 + x];
     }
 }
@@ -629,11 +642,14 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
     const size_t N = (size_t)n0 * n1 * n2;
     const size_t HS = (size_t)H * n0 * n1;
     const double add[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+    const FastDiv dn2(n2), dty(ty);
+    // 1/Nxyz: a power of two for the usual grids, where the product equals the division
+    const double rdiv = 1.0 / divisor;
     for (int ca = 0; ca < ncomp; ca += 2) {
         const int cb = ca + 1;
         const bool two = cb < ncomp;
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
-            const int kz = i / ty, yy = i - kz * ty;
+            const int kz = dty.div(i), yy = i - kz * ty;
             const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
             const cx<T> A = specB[ca * HS + o];
             const cx<T> B = two ? specB[cb * HS + o] : cx<T>{T(0), T(0)};
@@ -643,15 +659,15 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
         __syncthreads();
         lds_fft<T, 2, false>(buf, ty, pz, stw, +1);
         for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
-            const int yy = i / n2, z = i - yy * n2;
+            const int yy = dn2.div(i), z = i - yy * n2;
             const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
             const cx<T> v = buf[yy * n2 + z];
             if (accumulate) {
-                dg[ca * N + g] += add[ca] + (double)v.x / divisor;
-                if (two) dg[cb * N + g] += add[cb] + (double)v.y / divisor;
+                dg[ca * N + g] += add[ca] + (double)v.x * rdiv;
+                if (two) dg[cb * N + g] += add[cb] + (double)v.y * rdiv;
             } else {
-                dg[ca * N + g] = (double)v.x / divisor;
-                if (two) dg[cb * N + g] = (double)v.y / divisor;
+                dg[ca * N + g] = (double)v.x * rdiv;
+                if (two) dg[cb * N + g] = (double)v.y * rdiv;
             }
         }
         __syncthreads();
