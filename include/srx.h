@@ -137,7 +137,8 @@ const char *srx_strerror(int code);
 void srx_default_options(srx_options *opt);
 
 /* Per-kernel timing accumulated since srx_open, from HIP events recorded on
-   the solver's stream around each launch (enabled by options.reserved[0] = 1):
+   the solver's stream around each launch (options.reserved[0] = 1: every kernel;
+   2: the Newton kernel only, which keeps the event overhead off the other launches):
    names[i], total milliseconds ms[i] and launch count calls[i], i < *count. */
 int srx_kernel_times(srx_ctx *ctx, const char **names, double *ms, int *calls, int *count);
 

@@ -371,7 +371,7 @@ struct srx_ctx {
     std::vector<int> grain;
     FILE *fvm = nullptr, *fdd = nullptr, *ferr = nullptr, *fdrx = nullptr;
     // timing
-    bool timing = false;
+    int timing = 0;   // 0 off, 1 every kernel, 2 the Newton kernel only
     std::vector<hipEvent_t> evpool;
     std::vector<int> evslot;
     size_t evused = 0;
@@ -753,8 +753,9 @@ struct Timer {
     srx_ctx *c;
     int slot;
     size_t ia = 0;
-    Timer(srx_ctx *cc, int s) : c(cc), slot(s) {
-        if (!c->timing) return;
+    bool on;
+    Timer(srx_ctx *cc, int s) : c(cc), slot(s), on(cc->timing == 1 || (cc->timing == 2 && s == 5)) {
+        if (!on) return;
         ia = c->evused;
         if (c->evused + 2 > c->evpool.size()) {
             for (int k = 0; k < 256; ++k) {
@@ -767,7 +768,7 @@ struct Timer {
         set_launch_events(c->evpool[ia], c->evpool[ia + 1]);
     }
     ~Timer() {
-        if (!c->timing) return;
+        if (!on) return;
         set_launch_events(nullptr, nullptr);
         c->evslot.push_back(slot);
     }
@@ -846,7 +847,7 @@ int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_
         const size_t p = d.find_last_of('/');
         c->workdir = p == std::string::npos ? std::string(".") : d.substr(0, p);
     }
-    c->timing = c->opt.reserved[0] != 0;
+    c->timing = c->opt.reserved[0] == 2 ? 2 : (c->opt.reserved[0] != 0 ? 1 : 0);
     for (int k = 0; k < 8; ++k) c->kt[k] = {KT_NAMES[k], 0.0, 0};
     c->nkt = 8;
     int rc = read_deck(deck, c->dk);
