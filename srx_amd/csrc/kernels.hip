@@ -773,17 +773,40 @@ __device__ __forceinline__ void rho_pf(const DevPhase &P, const double rs[NSYS],
 }
 
 // mobile density (constitutive.c:632)
+// sqrt(x) and 1/sqrt(x) for normal x > 0: hardware reciprocal square root refined by two
+// coupled Newton steps and a final residual correction of the root (within an ulp); the
+// dislocation densities here are O(1e-3..1e4).  x = 0 gives (0, inf) as sqrt and 1/sqrt do.
+__device__ __forceinline__ void sqrt_rsqrt(double x, double &sq, double &rs) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y, h = 0.5 * y;
+    double e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    e = fma(-g, h, 0.5);
+    g = fma(g, e, g);
+    h = fma(h, e, h);
+    const double d = fma(-g, g, x);
+    sq = x == 0.0 ? 0.0 : fma(d, h, g);
+    rs = x == 0.0 ? y : 2.0 * h;
+}
+__device__ __forceinline__ double fsqrt(double x) {
+    double sq, rs;
+    sqrt_rsqrt(x, sq, rs);
+    return sq;
+}
+
 __device__ __forceinline__ double rho_mobile(const DevPhase &P, double rP, double rF) {
-    return P.rhom_k * sqrt(rP * rF) * 160.2177;
+    return P.rhom_k * fsqrt(rP * rF) * 160.2177;
 }
 
 // tau_pass with a given mobile density (587), 1/(tau_0 + tau_cut) (630, 639) and gamma_0 (633)
 __device__ __forceinline__ void sys_const(const DevPhase &P, double rP, double rF, double rm_pass, double rmn,
                                           double &tp, double &inv, double &g0) {
-    const double sF = sqrt(rF);
-    tp = P.c1Gb3 * sqrt(fabs(rP + rm_pass));
+    double sF, rsF;
+    sqrt_rsqrt(rF, sF, rsF);
+    tp = P.c1Gb3 * fsqrt(fabs(rP + rm_pass));
     inv = rcp(P.tau0 + P.k_tcut * sF);
-    g0 = P.k_g0 * rmn * rcp(sF);
+    g0 = P.k_g0 * rmn * rsF;
 }
 
 // sinh(x) and cosh(x) for x >= 0 from one reduction x = k ln2 + r, |r| <= ln2/2 (Cody-Waite
@@ -997,7 +1020,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
                 g0[s * NR_BLOCK] = g;
                 if (FIRST) {
                     const double rmo = f.rho_m[s * N + v];
-                    tpo[s * NR_BLOCK] = P.c1Gb3 * sqrt(fabs(rP + rmo));
+                    tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo));
                     f.rho_m_old[s * N + v] = rmo;
                     f.rho_m[s * N + v] = rmn;
                 }
@@ -1101,25 +1124,42 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         double a[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
-        // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre
-        double Sp[15];
+        // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre, with
+        // S' pre = Q2 (sinv o (Q2^T pre)) (S' = Q2 diag(sinv) Q2^T, as sprime builds it)
+        double spp[5];
         if (ROT) {
-            double a0[9];
+            double a0[9], R[9];
 #pragma unroll
             for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
-            sprime(a, a0, P.sinv, Sp);
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) R[3 * i + j] = a[i] * a0[j] + a[3 + i] * a0[3 + j] + a[6 + i] * a0[6 + j];
+            double Q2[5][5];
+            q5_from(R, Q2);
+            double t[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 5; ++j) acc += Q2[j][k] * pre[j];
+                t[k] = P.sinv[k] * acc;
+            }
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) acc += Q2[i][k] * t[k];
+                spp[i] = acc;
+            }
         } else {
 #pragma unroll
-            for (int k = 0; k < 15; ++k) Sp[k] = 0.0;
-#pragma unroll
-            for (int i = 0; i < 5; ++i) Sp[ps(i, i)] = P.sinv[i];
+            for (int i = 0; i < 5; ++i) spp[i] = P.sinv[i] * pre[i];
         }
         double ce2 = 0.0;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            double acc = q[i] + ed[i] * dt;
-#pragma unroll
-            for (int j = 0; j < 5; ++j) acc += Sp[ps(i, j)] * pre[j];
+            const double acc = q[i] + ed[i] * dt + spp[i];
             ce2 += acc * acc;
         }
         const double eh = qh + P.sinv[5] * preh;
@@ -1251,25 +1291,28 @@ __attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WA
         f.nr_it[v] = r.it;
     }
     // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan, wavefront nr_max
-    double vals[NR_NPART];
-    vals[0] = r.cs;
-    vals[1] = r.ce;
-#pragma unroll
-    for (int k = 0; k < 6; ++k) {
-        vals[2 + k] = r.sig[k];
-        vals[8 + k] = (valid && ph == 1) ? r.sig[k] : 0.0;
-    }
-    vals[14] = (double)r.it;
-    vals[15] = (double)r.it;
-    vals[16] = (double)r.bad;
     const int lane = threadIdx.x & 63;
-    // the wavefront's largest iteration count (the lane-iterations it executed / 64)
-    const double wmax = wave_max((double)r.it);
-    vals[17] = lane == 0 ? wmax : 0.0;
+    double red[NR_NPART];
+    red[0] = wave_sum(r.cs);
+    red[1] = wave_sum(r.ce);
 #pragma unroll
-    for (int k = 0; k < NR_NPART; ++k) {
-        const double sred = (k == 15) ? wave_max(vals[k]) : wave_sum(vals[k]);
-        if (lane == 0) partials[(size_t)k * gridDim.x + blockIdx.x] = sred;
+    for (int k = 0; k < 6; ++k) red[2 + k] = wave_sum(r.sig[k]);
+    // when every voxel of the wavefront is phase 1 the phase-1 sums are the full sums
+    if (__any(valid && ph != 1)) {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) red[8 + k] = wave_sum((valid && ph == 1) ? r.sig[k] : 0.0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < 6; ++k) red[8 + k] = red[2 + k];
+    }
+    red[14] = wave_sum((double)r.it);
+    // the wavefront's largest iteration count (the lane-iterations it executed / 64)
+    red[15] = wave_max((double)r.it);
+    red[16] = __any(r.bad) ? 1.0 : 0.0;
+    red[17] = red[15];
+    if (lane == 0) {
+#pragma unroll
+        for (int k = 0; k < NR_NPART; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = red[k];
     }
 }
 
