@@ -155,7 +155,7 @@ struct FastDiv {
 // in registers between the load and the store, so the stage runs in place.
 template <typename T, int R, int MAXB>
 __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int L, const cx<T> *__restrict__ tw,
-                                               int sign) {
+                                               int sign, int pitch) {
     const int nb = N / R;
     const int total = npen * nb;
     const int tstep = N / (L * R);
@@ -166,7 +166,7 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
             const int p = dnb.div(t), j = t - p * nb;
-            const cx<T> *src = buf + p * N + j;
+            const cx<T> *src = buf + p * pitch + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
         }
@@ -187,7 +187,7 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
                 }
             }
             Dft<T, R>::run(v[m], sign);
-            cx<T> *dst = buf + p * N + jl * L * R + k;
+            cx<T> *dst = buf + p * pitch + jl * L * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
         }
@@ -198,7 +198,8 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
 // The same stage with the pencil length and span known at compile time (powers of two):
 // the index arithmetic reduces to shifts and masks.
 template <typename T, int R, int MAXB, int N, int L>
-__device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign) {
+__device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign,
+                                                  int pitch) {
     constexpr int nb = N / R;
     constexpr int tstep = N / (L * R);
     const int total = npen * nb;
@@ -208,7 +209,7 @@ __device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
             const int p = t / nb, j = t % nb;
-            const cx<T> *src = buf + p * N + j;
+            const cx<T> *src = buf + p * pitch + j;
 #pragma unroll
             for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
         }
@@ -229,7 +230,7 @@ __device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx
                 }
             }
             Dft<T, R>::run(v[m], sign);
-            cx<T> *dst = buf + p * N + (j / L) * L * R + k;
+            cx<T> *dst = buf + p * pitch + (j / L) * L * R + k;
 #pragma unroll
             for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
         }
@@ -252,11 +253,12 @@ __host__ __device__ constexpr int pow2_radix(int N, int st) {
 }
 
 template <typename T, int MAXB, int N, int ST, int L>
-__device__ __forceinline__ void lds_fft_pow2(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign) {
+__device__ __forceinline__ void lds_fft_pow2(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign,
+                                             int pitch) {
     constexpr int R = pow2_radix(N, ST);
     if constexpr (R != 0) {
-        stockham_stage_ct<T, R, MAXB, N, L>(buf, npen, tw, sign);
-        lds_fft_pow2<T, MAXB, N, ST + 1, L * R>(buf, npen, tw, sign);
+        stockham_stage_ct<T, R, MAXB, N, L>(buf, npen, tw, sign, pitch);
+        lds_fft_pow2<T, MAXB, N, ST + 1, L * R>(buf, npen, tw, sign, pitch);
     }
 }
 
@@ -266,13 +268,14 @@ __device__ __forceinline__ void lds_fft_pow2(cx<T> *buf, int npen, const cx<T> *
 // in the z passes, which interleave the transform with the real/complex packing, the extra
 // registers cost an occupancy step and the runtime stages are faster).
 template <typename T, int MAXB, bool POW2 = true>
-__device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign) {
+__device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign,
+                        int pitch) {
     if constexpr (POW2) {
         switch (pl.n) {
-        case 32: lds_fft_pow2<T, MAXB, 32, 0, 1>(buf, npen, tw, sign); return;
-        case 64: lds_fft_pow2<T, MAXB, 64, 0, 1>(buf, npen, tw, sign); return;
-        case 128: lds_fft_pow2<T, MAXB, 128, 0, 1>(buf, npen, tw, sign); return;
-        case 256: lds_fft_pow2<T, MAXB, 256, 0, 1>(buf, npen, tw, sign); return;
+        case 32: lds_fft_pow2<T, MAXB, 32, 0, 1>(buf, npen, tw, sign, pitch); return;
+        case 64: lds_fft_pow2<T, MAXB, 64, 0, 1>(buf, npen, tw, sign, pitch); return;
+        case 128: lds_fft_pow2<T, MAXB, 128, 0, 1>(buf, npen, tw, sign, pitch); return;
+        case 256: lds_fft_pow2<T, MAXB, 256, 0, 1>(buf, npen, tw, sign, pitch); return;
         default: break;
         }
     }
@@ -280,11 +283,11 @@ __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
         switch (R) {
-        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign); break;
-        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign); break;
-        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign); break;
-        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign); break;
-        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign); break;
+        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
         }
         L *= R;
     }
@@ -310,7 +313,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
                                                      double m2, double m3, double m4, double m5) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *stw = buf + TY * n2;   // twiddle table staged in LDS
+    const int P = n2 + 1;           // padded pencil pitch: rows fall on different LDS banks
+    cx<T> *stw = buf + TY * P;      // twiddle table staged in LDS
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
     const int ty = min(TY, n1 - y0);
@@ -329,14 +333,14 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
             const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
             const double va = in[ca * N + g] - ma;
             const double vb = two ? in[cb * N + g] - mb : 0.0;
-            buf[yy * n2 + z] = cx<T>{T(va), T(vb)};
+            buf[yy * P + z] = cx<T>{T(va), T(vb)};
         }
         __syncthreads();
-        lds_fft<T, 2, false>(buf, ty, pz, stw, -1);
+        lds_fft<T, 2, false>(buf, ty, pz, stw, -1, P);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
             const int kz = dty.div(i), yy = i - kz * ty;
-            const cx<T> z1 = buf[yy * n2 + kz];
-            const cx<T> z2 = buf[yy * n2 + (kz == 0 ? 0 : n2 - kz)];
+            const cx<T> z1 = buf[yy * P + kz];
+            const cx<T> z2 = buf[yy * P + (kz == 0 ? 0 : n2 - kz)];
             const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
             specB[ca * HS + o] = cx<T>{T(0.5) * (z1.x + z2.x), T(0.5) * (z1.y - z2.y)};
             if (two) specB[cb * HS + o] = cx<T>{T(0.5) * (z1.y + z2.y), T(-0.5) * (z1.x - z2.x)};
@@ -356,7 +360,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
                                                      const cx<T> *__restrict__ tw, int TX) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *stw = buf + TX * n1;
+    const int P = n1 + 1;   // padded pencil pitch
+    cx<T> *stw = buf + TX * P;
     for (int i = threadIdx.x; i < n1; i += FFT_BLOCK) stw[i] = tw[i];
     const int H = n2 / 2 + 1;
     const int x0 = blockIdx.x * TX;
@@ -369,25 +374,25 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int xx = dn1.div(i), y = i - xx * n1;
-            buf[xx * n1 + y] = in[base + (size_t)(x0 + xx) * n1 + y];
+            buf[xx * P + y] = in[base + (size_t)(x0 + xx) * n1 + y];
         }
     } else {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int y = dtx.div(i), xx = i - y * tx;
-            buf[xx * n1 + y] = in[base + (size_t)y * n0 + x0 + xx];
+            buf[xx * P + y] = in[base + (size_t)y * n0 + x0 + xx];
         }
     }
     __syncthreads();
-    lds_fft<T, 2>(buf, tx, py, stw, b_to_c ? -1 : +1);
+    lds_fft<T, 2>(buf, tx, py, stw, b_to_c ? -1 : +1, P);
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int y = dtx.div(i), xx = i - y * tx;
-            out[base + (size_t)y * n0 + x0 + xx] = buf[xx * n1 + y];
+            out[base + (size_t)y * n0 + x0 + xx] = buf[xx * P + y];
         }
     } else {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int xx = dn1.div(i), y = i - xx * n1;
-            out[base + (size_t)(x0 + xx) * n1 + y] = buf[xx * n1 + y];
+            out[base + (size_t)(x0 + xx) * n1 + y] = buf[xx * P + y];
         }
     }
 }
@@ -533,7 +538,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
         buf[c * slot + yy * n0 + x] = spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x];
     }
     __syncthreads();
-    if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, stw, -1);
+    if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, stw, -1, n0);
     if (mode == 1) {
         double tzv, tyv;
         unit_k(kz, n2, tzv);
@@ -614,7 +619,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
         }
         __syncthreads();
     }
-    if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, stw, +1);
+    if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, stw, +1, n0);
     for (int i = threadIdx.x; i < cout * slot; i += FFT_BLOCK) {
         const int c = dslot.div(i), r = i - c * slot, yy = dn0.div(r), x = r - yy * n0;
         spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x] = buf[c * slot + yy * n0 + x];
@@ -633,7 +638,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
                                                      double a4, double a5, double a6, double a7, double a8) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    cx<T> *stw = buf + TY * n2;
+    const int P = n2 + 1;   // padded pencil pitch
+    cx<T> *stw = buf + TY * P;
     for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
@@ -653,15 +659,15 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
             const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
             const cx<T> A = specB[ca * HS + o];
             const cx<T> B = two ? specB[cb * HS + o] : cx<T>{T(0), T(0)};
-            buf[yy * n2 + kz] = cx<T>{A.x - B.y, A.y + B.x};
-            if (kz >= 1 && kz <= n2 - H) buf[yy * n2 + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
+            buf[yy * P + kz] = cx<T>{A.x - B.y, A.y + B.x};
+            if (kz >= 1 && kz <= n2 - H) buf[yy * P + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
         }
         __syncthreads();
-        lds_fft<T, 2, false>(buf, ty, pz, stw, +1);
+        lds_fft<T, 2, false>(buf, ty, pz, stw, +1, P);
         for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
             const int yy = dn2.div(i), z = i - yy * n2;
             const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
-            const cx<T> v = buf[yy * n2 + z];
+            const cx<T> v = buf[yy * P + z];
             if (accumulate) {
                 dg[ca * N + g] += add[ca] + (double)v.x * rdiv;
                 if (two) dg[cb * N + g] += add[cb] + (double)v.y * rdiv;
@@ -1635,13 +1641,16 @@ bool make_plan(int n, FftPlan &p) {
 // LDS budget of one FFT block (bytes)
 constexpr int FFT_LDS_MAX = 48 * 1024;
 
-// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*n complex values within
+// LDS of a y or z pass: t pencils at the padded pitch n + 1, then the twiddle table
+static size_t padded_lds(int t, int n, size_t csz) { return ((size_t)t * (n + 1) + n) * csz; }
+
+// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*(n+1) complex values within
 // FFT_LDS_MAX and t <= cap
 static int tile_for(const FftPlan &p, int per_tile_comps, int maxb, int cap, int prec) {
     const int mr = min_radix(p);
     int t = (maxb * FFT_BLOCK * mr) / (p.n * per_tile_comps);
     const int csz = prec == 0 ? 16 : 8;
-    const int tl = (FFT_LDS_MAX - p.n * csz) / (per_tile_comps * p.n * csz);
+    const int tl = (FFT_LDS_MAX - p.n * csz) / (per_tile_comps * (p.n + 1) * csz);
     if (t > tl) t = tl;
     if (t > cap) t = cap;
     if (t < 1) t = 1;
@@ -1661,11 +1670,11 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
     const double z6[6] = {0, 0, 0, 0, 0, 0};
     const double *m = mean6 ? mean6 : z6;
     if (prec == 0) {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     } else {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, m[0], m[1], m[2], m[3], m[4], m[5]);
     }
@@ -1679,11 +1688,11 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
-        size_t sm = (size_t)(TX + 1) * n[1] * sizeof(cx<double>);
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>));
         SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX);
     } else {
-        size_t sm = (size_t)(TX + 1) * n[1] * sizeof(cx<float>);
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>));
         SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX);
     }
@@ -1732,12 +1741,12 @@ void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double 
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     const double *a = ddg9;
     if (prec == 0) {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
                            a[6], a[7], a[8]);
     } else {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
                            a[7], a[8]);
@@ -1790,11 +1799,11 @@ void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     } else {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
     }
@@ -1811,12 +1820,12 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<double>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
                            0., 0.);
     } else {
-        size_t sm = (size_t)(TY + 1) * n[2] * sizeof(cx<float>);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
                            0.);
