@@ -101,6 +101,55 @@ def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: 
                                                                gpu_nr_per_iter, iters_per_increment)}
 
 
+def ensemble_bench(args) -> int:
+    """RVE ensemble: members of uneven size, each with its own grain count, texture and strain rate
+    (0.01% strain per increment), run by srx_run_ensemble `concurrency` at a time and once one at
+    a time.  Prints one JSON line; not the headline metric."""
+    import srx_amd
+    from srx_amd import inputs
+    rng = np.random.default_rng(args.seed)
+    work = args.workdir or tempfile.mkdtemp(prefix="srx_ens_")
+    decks, nvox = [], []
+    for spec in args.ensemble.split(","):
+        count, n = (int(x) for x in spec.split("x"))
+        for _ in range(count):
+            k = len(decks)
+            grains = int(rng.integers(20, 1001))
+            rate = float(10 ** rng.uniform(-4, -2))
+            tex = "random" if rng.random() < 0.5 else "cube"
+            d = os.path.join(work, "rve%02d_%d" % (k, n))
+            deck = os.path.join(d, "deck.in")
+            if not os.path.exists(deck):
+                inputs.make_case(d, n=(n, n, n), grains=grains, seed=args.seed + k, nsteps=args.steps,
+                                 timestep=1e-4 / rate, rate=rate, texture=tex)
+            decks.append(deck)
+            nvox.append(n ** 3)
+    print("bench: ensemble of %d RVEs, %d voxels, in %s" % (len(decks), sum(nvox), work), file=sys.stderr, flush=True)
+    prec = srx_amd.PREC_MIXED if args.precisions.split(",")[-1] == "mixed" else srx_amd.PREC_FP64
+    out = {}
+    for conc in (args.concurrency, 1):
+        t0 = time.perf_counter()
+        res = srx_amd.run_ensemble(decks, concurrency=conc, precision=prec, nsteps_override=args.steps)
+        wall = time.perf_counter() - t0
+        work_vi = sum(r["voxels"] * r["increments"] for r in res)
+        out[conc] = {"wall_seconds": wall, "voxel_increments": work_vi, "value": work_vi / wall,
+                     "setup_seconds_sum": sum(r["setup_seconds"] for r in res),
+                     "run_seconds_sum": sum(r["run_seconds"] for r in res)}
+        print("bench: concurrency %d: %.2f s wall, %.3g voxel-increments/s" % (conc, wall, work_vi / wall),
+              file=sys.stderr, flush=True)
+    line = {"metric": "converged voxel-increments/s, RVE ensemble (wall clock incl. setup)",
+            "value": out[args.concurrency]["value"], "unit": "voxel-increments/s", "n_gpus": 1,
+            "steps": args.steps, "higher_is_better": True, "data": "synthetic",
+            "dtype": "f64+f32" if prec == srx_amd.PREC_MIXED else "f64",
+            "config": {"workload": "RVE ensemble %s (count x edge), grains U{20..1000}, random or cube texture, "
+                                   "rates log-U[1e-4, 1e-2]/s, 0.01%% strain per increment" % args.ensemble,
+                       "concurrency": args.concurrency},
+            "sequential": out[1], "concurrent": out[args.concurrency],
+            "concurrency_speedup": out[args.concurrency]["value"] / out[1]["value"]}
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -114,12 +163,17 @@ def main() -> int:
     ap.add_argument("--cpu-iters", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=None)
+    ap.add_argument("--ensemble", default=None,
+                    help="run an RVE ensemble instead, e.g. 16x32,16x64,12x96,4x128 (count x edge)")
+    ap.add_argument("--concurrency", type=int, default=4, help="ensemble members run at once")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the whole grid is resident on one MI355X; "
               "ensembles would shard by RVE); --gpus %d is not supported" % args.gpus, file=sys.stderr)
         return 2
 
+    if args.ensemble:
+        return ensemble_bench(args)
     from srx_amd import inputs
     import srx_amd
 

@@ -131,6 +131,25 @@ int srx_dims(srx_ctx *ctx, int *n3);
 /* Replaces DestroyJob + CloseFiles (init.c:1218-1416, io.c:230-245). */
 void srx_close(srx_ctx *ctx);
 
+/* Result of one member of an ensemble run. */
+typedef struct {
+    int    index;            /* position in the decks array */
+    int    rc;               /* 0, or the error code srx_open/srx_step returned */
+    int    increments;       /* increments completed */
+    long   voxels;           /* CellDim product */
+    double setup_seconds;    /* srx_open */
+    double run_seconds;      /* all increments and srx_close */
+    srx_step_report last;    /* report of the last completed increment */
+} srx_ensemble_result;
+
+/* Runs every increment of n independent decks (an ensemble of RVEs, each the equivalent of one
+   `trial <deck>` job) on this GPU, `concurrency` of them at a time (0: 4), each on its own stream
+   from its own host thread, largest grids first.  Output files go to workdirs[i] (NULL: each deck's
+   directory).  results[i] is filled for every deck; returns 0 or the first member's error code.
+   No counterpart in the reference, which runs one RVE per mpirun job. */
+int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int n, const srx_options *opt,
+                     int concurrency, srx_ensemble_result *results);
+
 /* Text for an error code. */
 const char *srx_strerror(int code);
 

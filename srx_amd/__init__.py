@@ -65,7 +65,14 @@ FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velg
           "rho_tot": (18, 1), "phase": (19, 1), "grain": (20, 1)}
 
 EXPORTS = ["srx_open", "srx_step", "srx_run", "srx_get_field", "srx_set_field", "srx_dims", "srx_close",
-           "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse"]
+           "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse",
+           "srx_run_ensemble"]
+
+
+class EnsembleResult(C.Structure):
+    """srx_ensemble_result of include/srx.h."""
+    _fields_ = [("index", C.c_int), ("rc", C.c_int), ("increments", C.c_int), ("voxels", C.c_long),
+                ("setup_seconds", C.c_double), ("run_seconds", C.c_double), ("last", StepReport)]
 
 _lib = None
 
@@ -93,6 +100,8 @@ def lib():
         L.srx_fft_forward.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, P(C.c_double)]
         L.srx_fft_inverse.argtypes = [P(C.c_double), C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                       P(C.c_double)]
+        L.srx_run_ensemble.argtypes = [P(C.c_char_p), P(C.c_char_p), C.c_int, P(Options), C.c_int,
+                                       P(EnsembleResult)]
         _lib = L
     return _lib
 
@@ -212,3 +221,20 @@ def fft_inverse(X: np.ndarray, n2: int, precision: int = PREC_FP64, scale: float
     if rc:
         raise SrxError(rc, "srx_fft_inverse")
     return out.reshape(nc, n0, n1, n2)
+
+
+def run_ensemble(decks, workdirs=None, concurrency: int = 4, **opt) -> list[dict]:
+    """srx_run_ensemble: every increment of each deck, `concurrency` RVEs at a time on this GPU.
+    Returns one dict per deck (rc, increments, voxels, setup_seconds, run_seconds, last report);
+    raises SrxError if any member failed."""
+    n = len(decks)
+    arr = (C.c_char_p * n)(*[d.encode() for d in decks])
+    wds = (C.c_char_p * n)(*[w.encode() for w in workdirs]) if workdirs else None
+    res = (EnsembleResult * n)()
+    o = options(**opt)
+    rc = lib().srx_run_ensemble(arr, wds, n, C.byref(o), concurrency, res)
+    out = [{"index": r.index, "rc": r.rc, "increments": r.increments, "voxels": r.voxels,
+            "setup_seconds": r.setup_seconds, "run_seconds": r.run_seconds, "last": r.last.as_dict()} for r in res]
+    if rc:
+        raise SrxError(rc, "srx_run_ensemble")
+    return out

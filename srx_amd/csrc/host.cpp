@@ -17,6 +17,8 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <atomic>
+#include <thread>
 #include <vector>
 
 namespace srx {
@@ -1156,6 +1158,57 @@ int srx_run(srx_ctx *c) {
         if (rc == SRX_E_CREEP_STOP) break;
         if (rc) return rc;
     }
+    return 0;
+}
+
+int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int n, const srx_options *opt,
+                     int concurrency, srx_ensemble_result *results) {
+    if (!decks || n < 0 || (n > 0 && !results)) return SRX_E_ARG;
+    if (n == 0) return 0;
+    // largest grids first, so the last workers to finish hold the small ones
+    std::vector<long> size(n, 0);
+    for (int i = 0; i < n; ++i) {
+        Deck d;
+        if (decks[i] && read_deck(decks[i], d) == 0) size[i] = (long)d.CellDim[0] * d.CellDim[1] * d.CellDim[2];
+    }
+    std::vector<int> order(n);
+    for (int i = 0; i < n; ++i) order[i] = i;
+    std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return size[a] > size[b]; });
+    const int workers = std::max(1, std::min(concurrency > 0 ? concurrency : 4, n));
+    srx_options o;
+    if (opt) o = *opt; else srx_default_options(&o);
+    if (o.nthreads <= 0) o.nthreads = std::max(1, (int)std::thread::hardware_concurrency() / workers);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        for (;;) {
+            const int k = next.fetch_add(1);
+            if (k >= n) return;
+            const int i = order[k];
+            srx_ensemble_result &r = results[i];
+            std::memset(&r, 0, sizeof(r));
+            r.index = i;
+            r.voxels = size[i];
+            const auto t0 = std::chrono::steady_clock::now();
+            srx_ctx *c = nullptr;
+            int rc = srx_open(decks[i], workdirs ? workdirs[i] : nullptr, &o, &c);
+            const auto t1 = std::chrono::steady_clock::now();
+            while (rc == 0) {
+                rc = srx_step(c, &r.last);
+                if (rc == 0) ++r.increments;
+            }
+            if (rc == SRX_E_DONE || rc == SRX_E_CREEP_STOP) rc = 0;
+            if (c) srx_close(c);
+            const auto t2 = std::chrono::steady_clock::now();
+            r.rc = rc;
+            r.setup_seconds = std::chrono::duration<double>(t1 - t0).count();
+            r.run_seconds = std::chrono::duration<double>(t2 - t1).count();
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int w = 0; w < workers; ++w) pool.emplace_back(work);
+    for (auto &t : pool) t.join();
+    for (int i = 0; i < n; ++i)
+        if (results[i].rc) return results[i].rc;
     return 0;
 }
 

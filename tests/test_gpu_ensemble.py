@@ -1,0 +1,57 @@
+"""srx_run_ensemble: several RVEs of uneven size run concurrently on one GPU (one stream and host
+thread per member) give exactly the results of running each deck alone."""
+import os
+
+import numpy as np
+import pytest
+
+import srx_amd
+from srx_amd import inputs
+
+pytestmark = pytest.mark.gpu
+
+CASES = [  # (grid, grains, seed, strain rate, texture)
+    ((16, 16, 16), 10, 21, 1e-3, "random"),
+    ((8, 12, 10), 5, 22, 3e-3, "cube"),
+    ((12, 12, 12), 7, 23, 5e-4, "random"),
+    ((8, 8, 8), 3, 24, 1e-3, "cube"),
+]
+
+
+def make_cases(root):
+    decks = []
+    for k, (n, g, seed, rate, tex) in enumerate(CASES):
+        d = os.path.join(root, "rve%d" % k)
+        # 0.01% strain per increment at every rate
+        decks.append(inputs.make_case(d, n=n, grains=g, seed=seed, nsteps=3, timestep=1e-4 / rate, rate=rate,
+                                      texture=tex))
+    return decks
+
+
+def test_ensemble_matches_members_run_alone(tmp_path):
+    decks = make_cases(str(tmp_path / "ens"))
+    res = srx_amd.run_ensemble(decks, concurrency=3, write_files=1)
+    alone = make_cases(str(tmp_path / "alone"))
+    for k, deck in enumerate(alone):
+        s = srx_amd.Solver(deck, write_files=1)
+        last = None
+        for _ in range(3):
+            last = s.step()
+        s.close()
+        r = res[k]
+        assert r["rc"] == 0 and r["increments"] == 3 and r["index"] == k
+        assert r["voxels"] == int(np.prod(CASES[k][0]))
+        for key in ("iters", "nr_iters_total", "s_vm", "e_vm", "rho_s_avg"):
+            assert r["last"][key] == last[key], (k, key)
+        assert r["last"]["sig_avg"] == last["sig_avg"]
+        a = open(os.path.join(os.path.dirname(decks[k]), "vm.out")).read()
+        b = open(os.path.join(os.path.dirname(deck), "vm.out")).read()
+        assert a == b
+
+
+def test_ensemble_reports_a_failing_member(tmp_path):
+    decks = make_cases(str(tmp_path / "ens"))[:2]
+    os.remove(os.path.join(os.path.dirname(decks[1]), "mesh.txt"))
+    with pytest.raises(srx_amd.SrxError) as e:
+        srx_amd.run_ensemble(decks, concurrency=2)
+    assert e.value.code == 2001
