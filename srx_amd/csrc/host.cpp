@@ -9,6 +9,7 @@
 #include "../../include/srx.h"
 
 #include <hip/hip_runtime.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <chrono>
@@ -498,33 +499,87 @@ int setup(srx_ctx *c) {
 
     // microstructure (init.c:233-396)
     const long N = c->nvox;
+    const int nthr = c->opt.nthreads > 0 ? c->opt.nthreads : omp_get_max_threads();
     std::vector<double> tran(9 * N, 0.0), sig(6 * N, 0.0);
     std::vector<int> phase(N, 0);
     c->grain.assign(N, 0);
     {
+        // The reference reads N records with fgets(buf, 80) and sscanf (init.c:253-301).  When each
+        // of the first N lines is shorter than 79 characters (so fgets returns whole lines, as
+        // it does for every valid mesh) the lines are parsed in parallel from one read of the
+        // file; otherwise record by record with fgets, as the reference does.
         const std::string ms = resolve(c->workdir, d.initial_ms);
-        FILE *fp = std::fopen(ms.c_str(), "r");
+        FILE *fp = std::fopen(ms.c_str(), "rb");
         if (!fp) return SRX_E_FILE;
-        char buf[80];
-        long nph1 = 0;
-        for (long v = 0; v < N; ++v) {
+        std::vector<char> text;
+        {
+            char chunk[1 << 16];
+            size_t got;
+            while ((got = std::fread(chunk, 1, sizeof(chunk), fp)) > 0) text.insert(text.end(), chunk, chunk + got);
+        }
+        std::fclose(fp);
+        std::vector<size_t> start;
+        start.reserve(N + 1);
+        bool whole_lines = true;
+        for (size_t pos = 0; (long)start.size() < N && pos < text.size();) {
+            const char *nl = static_cast<const char *>(std::memchr(&text[pos], '\n', text.size() - pos));
+            const size_t end = nl ? (size_t)(nl - &text[0]) : text.size();
+            if (end - pos >= 79) whole_lines = false;
+            start.push_back(pos);
+            pos = end + 1;
+        }
+        if ((long)start.size() < N) return 1139;   // fgets hits EOF before N records
+        std::vector<double> eul(3 * N);
+        std::vector<int> ijk(3 * N);
+        auto parse = [&](const char *line, long v) {
             double t1 = 0, th = 0, om = 0;
             int ii = 0, jj = 0, kk = 0, jgr = 0, jph = 0;
-            if (!std::fgets(buf, 80, fp)) { std::fclose(fp); return 1139; }
-            std::sscanf(buf, "%lf %lf %lf %d %d %d %d %d", &t1, &th, &om, &ii, &jj, &kk, &jgr, &jph);
-            const long idx = ((long)(ii - 1) * c->n[1] + jj - 1) * c->n[2] + kk - 1;
-            if (idx != v || jph < 1 || jph > NPH) { std::fclose(fp); return 1139; }
-            if (jph == 1) ++nph1;
+            std::sscanf(line, "%lf %lf %lf %d %d %d %d %d", &t1, &th, &om, &ii, &jj, &kk, &jgr, &jph);
+            eul[3 * v] = t1; eul[3 * v + 1] = th; eul[3 * v + 2] = om;
+            ijk[3 * v] = ii; ijk[3 * v + 1] = jj; ijk[3 * v + 2] = kk;
             c->grain[v] = jgr;
             phase[v] = jph;
+        };
+        if (whole_lines) {
+#pragma omp parallel for schedule(static) num_threads(nthr)
+            for (long v = 0; v < N; ++v) {
+                char buf[80];
+                const size_t b0 = start[v];
+                size_t len = (v + 1 < (long)start.size() ? start[v + 1] : text.size()) - b0;
+                if (len > 79) len = 79;
+                std::memcpy(buf, &text[b0], len);
+                buf[len] = 0;
+                parse(buf, v);
+            }
+        } else {
+            FILE *f2 = std::fopen(ms.c_str(), "r");
+            if (!f2) return SRX_E_FILE;
+            char buf[80];
+            for (long v = 0; v < N; ++v) {
+                if (!std::fgets(buf, 80, f2)) { std::fclose(f2); return 1139; }
+                parse(buf, v);
+            }
+            std::fclose(f2);
+        }
+        // record order and phase checks, in voxel order as the reference meets them
+        long nph1 = 0;
+        for (long v = 0; v < N; ++v) {
+            const long idx = ((long)(ijk[3 * v] - 1) * c->n[1] + ijk[3 * v + 1] - 1) * c->n[2] + ijk[3 * v + 2] - 1;
+            const int jph = phase[v];
+            if (idx != v || jph < 1 || jph > NPH) return 1139;
+            if (jph == 1) ++nph1;
+        }
+#pragma omp parallel for schedule(static) num_threads(nthr)
+        for (long v = 0; v < N; ++v) {
+            const int jph = phase[v];
             if (!c->ph[jph - 1].xtal) continue;
-            t1 *= PI_REF / 180.; th *= PI_REF / 180.; om *= PI_REF / 180.;
+            const double t1 = eul[3 * v] * (PI_REF / 180.), th = eul[3 * v + 1] * (PI_REF / 180.);
+            const double om = eul[3 * v + 2] * (PI_REF / 180.);
             double s2x[9];
             euler_to_matrix(t1, th, om, s2x);
             for (int a = 0; a < 3; ++a)
                 for (int b = 0; b < 3; ++b) tran[9 * v + 3 * a + b] = s2x[3 * b + a];
         }
-        std::fclose(fp);
         c->wgt_ph1 = (1.0 * (double)nph1) / N;
     }
     // C066 = <C_gr> over the grid; C_gr = Q diag Q^T in the B basis.  Chunked sums
@@ -533,7 +588,7 @@ int setup(srx_ctx *c) {
         const long CH = 4096;
         const long nch = (N + CH - 1) / CH;
         std::vector<double> part(36 * nch, 0.0);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nthr)
         for (long ch = 0; ch < nch; ++ch) {
             double acc[36] = {0};
             for (long v = ch * CH; v < std::min(N, (ch + 1) * CH); ++v) {
@@ -611,7 +666,7 @@ int setup(srx_ctx *c) {
             for (int j = 0; j < 3; ++j) e9[3 * i + j] = (c->DisGradAvg[3 * i + j] + c->DisGradAvg[3 * j + i]) / 2.0;
         t9_to_b6(e9, e6);
         t9_to_b6(c->Scauchy, sc6);
-#pragma omp parallel for schedule(static)
+#pragma omp parallel for schedule(static) num_threads(nthr)
         for (long v = 0; v < N; ++v) {
             const int jph = phase[v];
             if (!c->ph[jph - 1].xtal) continue;

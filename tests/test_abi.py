@@ -77,6 +77,46 @@ def test_setup_error_codes(tmp_path):
     assert open_code(bad) == 3002
 
 
+def parse_ok(rc):
+    """srx_open got past the mesh reader: 0 on a GPU, SRX_E_HIP (2003) where there is none."""
+    return rc in (0, 2003)
+
+
+def test_mesh_reader_line_forms(tmp_path, oracle_lib):
+    """The mesh reader follows fgets(buf, 80) + sscanf (init.c:253-301) whether it parses in
+    parallel (every line shorter than 79 characters) or record by record: CRLF line ends parse,
+    an over-long line or a short file is error 1139, as in the oracle."""
+    inputs.make_case(str(tmp_path), n=(4, 4, 4), grains=2, seed=3, nsteps=1, timestep=0.1)
+    d = str(tmp_path)
+    lines = open(os.path.join(d, "mesh.txt")).read().splitlines()
+    variants = {
+        "crlf": ("\r\n".join(lines) + "\r\n", True),
+        "long": ("\n".join(lines[:5] + [lines[5] + " " * 90] + lines[6:]) + "\n", False),
+        "short": ("\n".join(lines[:-1]) + "\n", False),
+        "noeol": ("\n".join(lines), True),
+    }
+    for name, (text, ok) in variants.items():
+        open(os.path.join(d, "mesh_%s.txt" % name), "w", newline="").write(text)
+        deck = os.path.join(d, "deck_%s.in" % name)
+        inputs.write_deck(deck, n=(4, 4, 4), mesh="mesh_%s.txt" % name)
+        ro = oracle_lib.open_status(deck)
+        rp = open_code(deck) if not ok else _open_any(deck)
+        if ok:
+            assert ro == 0 and parse_ok(rp), (name, ro, rp)
+        else:
+            assert ro == 1139 and rp == 1139, (name, ro, rp)
+
+
+def _open_any(deck):
+    L = srx_amd.lib()
+    h = C.c_void_p()
+    o = srx_amd.options()
+    rc = L.srx_open(deck.encode(), os.path.dirname(deck).encode(), C.byref(o), C.byref(h))
+    if rc == 0:
+        L.srx_close(h)
+    return rc
+
+
 def test_bench_refuses_more_than_one_gpu():
     import subprocess
     import sys
