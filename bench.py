@@ -52,7 +52,7 @@ def fft_green_bytes_per_voxel(real_bytes: int, n2: int, c_out: int = 9) -> float
     return 48.0 + 16.0 * c_out + (24.0 + 4.0 * c_out) * 2.0 * real_bytes * h
 
 
-def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: int) -> dict:
+def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: int, keep_fields: bool = False) -> dict:
     """One solver run: `warmup` untimed increments, then `steps` timed ones.  timing=2 records
     HIP events around the Newton kernel only (the headline run: the other launches carry no
     event overhead); timing=1 around every kernel (the per-kernel breakdown run)."""
@@ -69,8 +69,13 @@ def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: in
     kt1 = s.kernel_times()
     kt = {k: (kt1[k][0] - kt0.get(k, (0, 0))[0], kt1[k][1] - kt0.get(k, (0, 0))[1]) for k in kt1}
     sig = np.array(reps[-1]["sig_avg"])
+    fields = {}
+    if keep_fields:   # outside the timed region
+        s9 = s.field("sig").reshape(-1, 3, 3)
+        dev = s9 - np.trace(s9, axis1=1, axis2=2)[:, None, None] * np.eye(3) / 3
+        fields = {"vm": np.sqrt(1.5 * (dev * dev).sum(axis=(1, 2))), "rho_s": s.field("rho_s")}
     s.close()
-    return {"seconds": t1 - t0, "reports": reps, "kernels": kt, "sig_avg": sig,
+    return {"seconds": t1 - t0, "reports": reps, "kernels": kt, "sig_avg": sig, "fields": fields,
             "iters": sum(r["iters"] for r in reps), "nr_total": sum(r["nr_iters_total"] for r in reps),
             "nr_wave": sum(r["nr_wave_iters"] for r in reps)}
 
@@ -191,7 +196,7 @@ def main() -> int:
         prec = srx_amd.PREC_MIXED if name == "mixed" else srx_amd.PREC_FP64
         print("bench: %s run (%d warmup + %d timed increments)" % (name, args.warmup, args.steps),
               file=sys.stderr, flush=True)
-        res[name] = run_precision(deck, prec, args.steps, args.warmup, timing=2)
+        res[name] = run_precision(deck, prec, args.steps, args.warmup, timing=2, keep_fields=True)
         r = res[name]
         print("bench: %s %.3f s, %d outer iterations, Newton kernel %.1f ms" % (
             name, r["seconds"], r["iters"], r["kernels"]["nr"][0]), file=sys.stderr, flush=True)
@@ -235,8 +240,12 @@ def main() -> int:
     out["mixed"] = per.get("mixed")
     if "fp64" in res and "mixed" in res:
         a, b = res["mixed"]["sig_avg"], res["fp64"]["sig_avg"]
+        fm, ff = res["mixed"]["fields"], res["fp64"]["fields"]
         out["mixed_vs_fp64"] = {"speedup": per["mixed"]["value"] / per["fp64"]["value"],
-                                "rel_macro_stress_error": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+                                "rel_macro_stress_error": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
+                                "max_voxel_von_mises_error_rel": float(np.abs(fm["vm"] - ff["vm"]).max() / ff["vm"].max()),
+                                "max_slip_system_rho_s_error_rel": float(np.abs(fm["rho_s"] - ff["rho_s"]).max()
+                                                                         / np.abs(ff["rho_s"]).max())}
     # roofline of the dominant kernel (largest share in the breakdown run); the Newton kernel is
     # timed with HIP events in the headline run's own timed region
     bk = brk[head]["kernels"]

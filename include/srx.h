@@ -60,7 +60,10 @@ typedef struct {
     int    write_fields;     /* 1: also the .iout field dumps at the deck's PrintControl steps (evolution.c:1456-1483) */
     int    nthreads;         /* host threads for setup (0: default) */
     int    verbose;          /* 1: per-iteration errors on stdout */
-    int    warm_start;       /* reserved */
+    int    warm_start;       /* 1: each Newton solve after the first of an increment starts from
+                                the stress extrapolated from the last two outer iterations instead of
+                                the last one (converges to the same tolerance in fewer iterations;
+                                results differ from the reference within that tolerance) */
     int    reserved[8];
 } srx_options;
 

@@ -479,6 +479,8 @@ struct orc_state {
     double *rho_s, *rho_m, *rho_F, *rho_P, *trial_rho_s, *gamdot, *gam_acum, *trial_gam_acum;
     double *rho_dipole, *rho_forest, *rho_inplane;          /* 12 each */
     double *rho_tot;
+    float *dsig;      /* warm_start 1: last NR call's stress change, B basis (float, as the GPU keeps it) */
+    int ws_use;       /* warm_start 1: this sweep starts from the extrapolated stress */
     double *nrit;
     double *work;         /* 2*nxyz complex scratch */
     double *ksig;         /* 9 comps x nxyz complex */
@@ -608,6 +610,10 @@ static int setup(orc_state *s)
     AL(velgrad, 9); AL(fluct, 9); AL(rho_s, 12); AL(rho_m, 12); AL(rho_F, 12); AL(rho_P, 12);
     AL(trial_rho_s, 12); AL(gamdot, 12); AL(gam_acum, 12); AL(trial_gam_acum, 12); AL(rho_dipole, 12);
     AL(rho_forest, 12); AL(rho_inplane, 12); AL(rho_tot, 1); AL(nrit, 1); AL(work, 2); AL(ksig, 18);
+    if (s->opt.warm_start == 1) {
+        s->dsig = (float *)calloc((size_t)6 * s->nxyz, sizeof(float));
+        if (!s->dsig) return 2005;
+    }
     AL(conv_s, 1); AL(conv_e, 1);
 #undef AL
     s->phase = (int *)calloc(s->nxyz, sizeof(int));
@@ -818,6 +824,9 @@ static int nr_voxel(orc_state *s, long p, double *cs, double *ce)
     }
     t_to_v6(lam, lam6, 6);
     t_to_v6(lam, sig6, 6);
+    /* warm_start 1 (no counterpart in the reference): start from lam + (lam - previous lam) */
+    if (s->dsig && s->ws_use)
+        for (i = 0; i < 6; i++) sig6[i] += (double)s->dsig[6 * p + i];
     t_to_v6(ep, eps6, 6);
     t_to_v6(st, str6, 6);
     for (i = 0; i < 3; i++) for (j = 0; j < 3; j++) signorm += lam[i][j] * lam[i][j];
@@ -854,6 +863,8 @@ static int nr_voxel(orc_state *s, long p, double *cs, double *ce)
         conv_s = sqrt(css);
         conv_e = sqrt(cee);
     }
+    if (s->dsig)
+        for (i = 0; i < 6; i++) s->dsig[6 * p + i] = (float)(sig6[i] - lam6[i]);
     {
         t33 so, eo;
         v6_to_t(sig6, 6, so);
@@ -1202,6 +1213,7 @@ int orc_step(orc_state *s, orc_report *rep)
             }
         }
         /* UpdateStress (evolution.c:1224) */
+        s->ws_use = iter > 1;
         #pragma omp parallel for schedule(dynamic, 64)
         for (p = 0; p < N; p++) s->nrit[p] = nr_voxel(s, p, s->conv_s + p, s->conv_e + p);
         for (p = 0; p < N; p++) {
@@ -1393,6 +1405,7 @@ void orc_close(orc_state *s)
     free(s->velgrad); free(s->fluct); free(s->rho_s); free(s->rho_m); free(s->rho_F); free(s->rho_P);
     free(s->trial_rho_s); free(s->gamdot); free(s->gam_acum); free(s->trial_gam_acum); free(s->rho_dipole);
     free(s->rho_forest); free(s->rho_inplane); free(s->rho_tot); free(s->nrit); free(s->work); free(s->ksig);
+    free(s->dsig);
     free(s->nvec); free(s->conv_s); free(s->conv_e); free(s->phase); free(s->grain);
     free(s);
 }

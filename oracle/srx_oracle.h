@@ -38,7 +38,7 @@ typedef struct {
     int    write_fields;   /* 1: also write the .iout field dumps at PrintControl steps */
     int    nthreads;       /* OpenMP threads (0 = default) */
     int    verbose;
-    int    warm_start;     /* ignored by the oracle */
+    int    warm_start;     /* 1: extrapolated Newton start (no counterpart in the reference) */
     int    reserved[8];
 } orc_options;
 

@@ -719,6 +719,10 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.rho_inplane, 12, 0.0)) || (rc = dalloc(&F.rho_tot, 1, 0.0)) ||
         (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
+    if (c->opt.warm_start == 1) {
+        HIPCK(hipMalloc(&F.dsig, sizeof(float) * 6 * N));
+        HIPCK(hipMemset(F.dsig, 0, sizeof(float) * 6 * N));
+    }
     HIPCK(hipMalloc(&F.phase, sizeof(int) * N));
     HIPCK(hipMalloc(&F.nr_it, sizeof(int) * N));
     HIPCK(hipMemset(F.nr_it, 0, sizeof(int) * N));
@@ -1414,6 +1418,7 @@ void srx_close(srx_ctx *c) {
                       c->partials, c->red};
     for (double *p : ptrs)
         if (p) hipFree(p);
+    if (c->f.dsig) hipFree(c->f.dsig);
     if (c->f.phase) hipFree(c->f.phase);
     if (c->f.nr_it) hipFree(c->f.nr_it);
     if (c->specB) hipFree(c->specB);

@@ -927,7 +927,7 @@ __device__ __forceinline__ double nr_in(const double *stg, int slot, const doubl
     return STAGED ? stg[(slot + k) * NR_BLOCK] : g[k * N + v];
 }
 
-template <bool FIRST, bool ROT, int PH>
+template <bool FIRST, bool ROT, int PH, bool WS>
 __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
                                          double *lds, const double *stg, NrResult &res) {
     constexpr bool STAGED = !FIRST;
@@ -936,6 +936,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     const double dt = cst->dt;
     double lam[5], q[5], lamh, qh, rsig;
     double A[15], r0[5];
+    double ws[5] = {0, 0, 0, 0, 0}, wsh = 0.0;   // warm start: crystal-frame offset of the first iterate
     {
         double a[9];
 #pragma unroll
@@ -979,6 +980,13 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         }
         lamh = lam6[5];
         qh = -ec[5];
+        if (!FIRST && WS) {
+            // extrapolated start: lam + (lam - lam of the previous outer iteration), the change
+            // kept in the crystal frame of this increment's orientation
+#pragma unroll
+            for (int i = 0; i < 5; ++i) ws[i] = (double)f.dsig[i * N + v];
+            wsh = (double)f.dsig[5 * N + v];
+        }
         sc_rotate(&cst->S066[0][0], Q, A);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -1037,9 +1045,9 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         }
     }
 
-    double sg[5], sh = lamh, pre[5], preh = lamh, ed[5];
+    double sg[5], sh = lamh + wsh, pre[5], preh = sh, ed[5];
 #pragma unroll
-    for (int i = 0; i < 5; ++i) { sg[i] = lam[i]; pre[i] = lam[i]; ed[i] = 0.0; }
+    for (int i = 0; i < 5; ++i) { sg[i] = lam[i] + ws[i]; pre[i] = sg[i]; ed[i] = 0.0; }
     int it = 0;
     double erral = 10 * 1E-10;
     while (it < 500 && erral > 1E-10) {
@@ -1181,6 +1189,12 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             for (int k = 0; k < 5; ++k) acc += Q[i][k] * sg[k];
             res.sig[i] = acc;
         }
+        if (WS) {
+            // this call's stress change (crystal frame) for the next call's extrapolated start
+#pragma unroll
+            for (int i = 0; i < 5; ++i) f.dsig[i * N + v] = (float)(sg[i] - lam[i]);
+            f.dsig[5 * N + v] = (float)(sh - lamh);
+        }
     }
     res.sig[5] = sh;
 #pragma unroll
@@ -1251,7 +1265,7 @@ __device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *
     __syncthreads();
 }
 
-template <bool FIRST, bool ROT>
+template <bool FIRST, bool ROT, bool WS>
 __global__ void __launch_bounds__(NR_BLOCK)
 __attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8))) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                  double *__restrict__ partials) {
@@ -1283,9 +1297,9 @@ __attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WA
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0>(cst, f, v, N, lds, stg, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS>(cst, f, v, N, lds, stg, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1>(cst, f, v, N, lds, stg, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS>(cst, f, v, N, lds, stg, r);
             }
         }
         todo &= ~__ballot(mine);
@@ -1760,10 +1774,17 @@ int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
                double *partials) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
-    if (first_call && rotated) SRX_LAUNCH((k_nr<true, true>), g, b, 0, s, cst, f, nvox, partials);
-    else if (first_call) SRX_LAUNCH((k_nr<true, false>), g, b, 0, s, cst, f, nvox, partials);
-    else if (rotated) SRX_LAUNCH((k_nr<false, true>), g, b, 0, s, cst, f, nvox, partials);
-    else SRX_LAUNCH((k_nr<false, false>), g, b, 0, s, cst, f, nvox, partials);
+    if (f.dsig) {   // extrapolated Newton start
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, true>), g, b, 0, s, cst, f, nvox, partials);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, true>), g, b, 0, s, cst, f, nvox, partials);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, true>), g, b, 0, s, cst, f, nvox, partials);
+        else SRX_LAUNCH((k_nr<false, false, true>), g, b, 0, s, cst, f, nvox, partials);
+    } else {
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, false>), g, b, 0, s, cst, f, nvox, partials);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, false>), g, b, 0, s, cst, f, nvox, partials);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, false>), g, b, 0, s, cst, f, nvox, partials);
+        else SRX_LAUNCH((k_nr<false, false, false>), g, b, 0, s, cst, f, nvox, partials);
+    }
     check_launch("nr");
 }
 

@@ -65,6 +65,7 @@ struct Fields {
     double *rho_dipole, *rho_forest, *rho_inplane;                          // [12][N]
     double *rho_tot;   // [N] rho_tot of the last hardening update (evolution.c:1030-1048)
     double *fl0;       // [9][N] fluctuation carried over DisGrad resets (creep runs only, else null)
+    float *dsig;       // [6][N] last Newton call's stress change, B basis, crystal frame (warm start only)
     int *phase;        // [N]
     int *nr_it;        // [N]
 };

@@ -167,3 +167,33 @@ def test_nan_is_reported_not_aborted(tmp_path):
     with pytest.raises(srx_amd.SrxError) as e:
         g.step()
     assert e.value.code == 3001
+
+
+def test_warm_start_matches_oracle(tmp_path, oracle_lib):
+    """warm_start 1 (extrapolated Newton start, no counterpart in the reference): the GPU and the
+    oracle run the same algorithm."""
+    deck = make(tmp_path, n=(16, 16, 16), grains=8, seed=12, nsteps=4)
+    o = oracle_lib.Oracle(deck, fixed_iters=6, warm_start=1)
+    g = srx_amd.Solver(deck, fixed_iters=6, warm_start=1)
+    for step in range(4):
+        ro, rg = o.step(), g.step()
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9)
+
+
+def test_warm_start_saves_newton_iterations(tmp_path):
+    """Converged increments with and without the extrapolated start: the same macroscopic response
+    (the Newton tolerance is 1e-10) with fewer Newton iterations."""
+    deck = make(tmp_path, n=(24, 24, 24), grains=12, seed=13, nsteps=5)
+    cold = srx_amd.Solver(deck)
+    warm = srx_amd.Solver(deck, warm_start=1)
+    nr_cold = nr_warm = 0
+    for step in range(5):
+        rc, rw = cold.step(), warm.step()
+        np.testing.assert_allclose(rw["sig_avg"], rc["sig_avg"], rtol=0, atol=1e-8 * np.abs(rc["sig_avg"]).max())
+        assert rw["iters"] == rc["iters"]
+        if step >= 2:
+            nr_cold += rc["nr_iters_total"]
+            nr_warm += rw["nr_iters_total"]
+    assert nr_warm < 0.9 * nr_cold, (nr_warm, nr_cold)
+    assert rel(warm.field("rho_s"), cold.field("rho_s")) < 1e-8
