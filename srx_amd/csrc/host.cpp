@@ -361,6 +361,8 @@ struct srx_ctx {
     void *tw[3] = {nullptr, nullptr, nullptr};
     FftPlan pl[3];
     double *partials = nullptr, *red = nullptr, *hred = nullptr;
+    IterCtl *dctl = nullptr, *hctl = nullptr;   // outer-iteration control block (device, pinned host copy)
+    int iters_last = 0;                        // outer iterations of the last increment
     int nr_blocks = 0, end_blocks = 0;
     hipStream_t stream = nullptr;
     // macro state
@@ -376,7 +378,7 @@ struct srx_ctx {
     // timing
     int timing = 0;   // 0 off, 1 every kernel, 2 the Newton kernel only
     std::vector<hipEvent_t> evpool;
-    std::vector<int> evslot;
+    std::vector<int> evslot, evit;
     size_t evused = 0;
     struct KT { const char *name; double ms; int calls; };
     KT kt[8];
@@ -751,6 +753,8 @@ int setup(srx_ctx *c) {
     HIPCK(hipMalloc(&c->partials, sizeof(double) * pb));
     HIPCK(hipMalloc(&c->red, sizeof(double) * 64));
     HIPCK(hipHostMalloc((void **)&c->hred, sizeof(double) * 64, hipHostMallocDefault));
+    HIPCK(hipMalloc(&c->dctl, sizeof(IterCtl)));
+    HIPCK(hipHostMalloc((void **)&c->hctl, sizeof(IterCtl), hipHostMallocDefault));
     return 0;
 }
 
@@ -769,53 +773,16 @@ void open_files(srx_ctx *c) {
 }
 
 // get_smacro (evolution.c:590-664) from the reduced sums
-void get_smacro(srx_ctx *c, const double sum6[6], const double sum6_ph1[6]) {
-    static const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
-    double t[9], t1[9];
-    b6_to_t9(sum6, t);
-    b6_to_t9(sum6_ph1, t1);
-    for (int k = 0; k < 9; ++k) {
-        c->SigAvg[k] = t[k] * c->wgt;
-        c->SigAvg1[k] = t1[k] * (c->wgt / c->wgt_ph1);
-    }
-    for (int i = 0; i < 6; ++i) {
-        const int ii = IJV[0][i], jj = IJV[1][i];
-        double acc = 0.0;
-        if (c->dk.VelGrad_BC_Flag[i] == 0) {
-            for (int k = 0; k < 6; ++k) {
-                const int kk = IJV[0][k], ll = IJV[1][k];
-                acc += c->hc.S0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * c->dk.Stress_BC_Flag[k] *
-                       (c->Scauchy[3 * kk + ll] - c->SigAvg[3 * kk + ll]);
-            }
-        }
-        c->dDisGradAvg[3 * ii + jj] = acc;
-    }
-    for (int k = 0; k < 9; ++k) c->acum[k] += c->dDisGradAvg[k];
-    double sav6[6], sda[9];
-    t9_to_b6(c->SigAvg, sav6);
-    b6_to_t9(sav6, sda, 5);
-    const double tr = c->opt.svm_mode == 1 ? (sda[0] + sda[4] + sda[8]) : (sda[4] + sda[8] + c->opt.svm_stray);
-    double s = 0.0;
-    for (int i = 0; i < 3; ++i)
-        for (int j = 0; j < 3; ++j) {
-            const double x = sda[3 * i + j];
-            s += i != j ? x * x : (x - tr / 3.0) * (x - tr / 3.0);
-        }
-    c->s_vm = std::sqrt(3. / 2. * s);
-    t9_to_b6(c->SigAvg1, sav6);
-    b6_to_t9(sav6, sda, 5);
-    s = 0.0;
-    for (int k = 0; k < 9; ++k) s += sda[k] * sda[k];
-    c->s_vm1 = std::sqrt(3. / 2. * s);
-}
-
 // event-timed launch helper: events come from a pool that is reused every increment
 struct Timer {
     srx_ctx *c;
-    int slot;
+    int slot, iter;
     size_t ia = 0;
     bool on;
-    Timer(srx_ctx *cc, int s) : c(cc), slot(s), on(cc->timing == 1 || (cc->timing == 2 && s == 5)) {
+    // iter: the outer iteration a launch belongs to (0: not an iteration kernel); launches of
+    // iterations past the last one run (they return at once) are left out of the totals
+    Timer(srx_ctx *cc, int s, int it = 0)
+        : c(cc), slot(s), iter(it), on(cc->timing == 1 || (cc->timing == 2 && s == 5)) {
         if (!on) return;
         ia = c->evused;
         if (c->evused + 2 > c->evpool.size()) {
@@ -832,6 +799,7 @@ struct Timer {
         if (!on) return;
         set_launch_events(nullptr, nullptr);
         c->evslot.push_back(slot);
+        c->evit.push_back(iter);
     }
 };
 
@@ -841,12 +809,14 @@ void collect_timing(srx_ctx *c) {
     if (!c->timing) return;
     hipStreamSynchronize(c->stream);
     for (size_t i = 0; i < c->evslot.size(); ++i) {
+        if (c->evit[i] > c->iters_last) continue;
         float ms = 0;
         hipEventElapsedTime(&ms, c->evpool[2 * i], c->evpool[2 * i + 1]);
         c->kt[c->evslot[i]].ms += ms;
         c->kt[c->evslot[i]].calls += 1;
     }
     c->evslot.clear();
+    c->evit.clear();
     c->evused = 0;
 }
 
@@ -1086,48 +1056,73 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     if (c->creep == 1) for (int k = 0; k < 9; ++k) c->uniform[k] = set[k];
     else for (int k = 0; k < 9; ++k) c->uniform[k] += add[k];
     for (int k = 0; k < 9; ++k) c->dDisGradAvg[k] = c->acum[k] = 0.0;
-    c->Err_e = 2. * c->err;
-    c->Err_s = 2. * c->err;
     const int rotated = c->istep > 2 ? 1 : 0;   // update_orient first runs at the end of istep 2
-    int iter = 0, nrmax = 0;
-    long nrtot = 0;
-    double nrwave = 0.0;
+    // outer fixed point (evolution.c:1150-1256), controlled on the device: the host enqueues
+    // iterations in batches and reads the control block once per batch
+    IterCtl &ctl = *c->hctl;
+    std::memset(&ctl, 0, sizeof(IterCtl));
+    ctl.wgt = c->wgt;
+    ctl.wgt_ph1 = c->wgt_ph1;
+    ctl.err = c->err;
+    ctl.e_vm_prev = c->e_vm;
+    ctl.svm_stray = c->opt.svm_stray;
+    ctl.svm_mode = c->opt.svm_mode;
+    ctl.nvox = (double)c->nvox;
+    for (int k = 0; k < 9; ++k) ctl.Scauchy[k] = c->Scauchy[k];
+    for (int k = 0; k < 6; ++k) {
+        ctl.vbc[k] = d.VelGrad_BC_Flag[k];
+        ctl.sbc[k] = d.Stress_BC_Flag[k];
+    }
+    ctl.itermax = c->itermax;
+    ctl.fixed_iters = c->opt.fixed_iters;
+    ctl.mixed = prec == SRX_PREC_MIXED;
+    ctl.Err_s = ctl.Err_e = 2. * c->err;
+    for (int k = 0; k < 6; ++k) {
+        ctl.sig_sum6[k] = c->sig_sum6[k];
+        ctl.mean6[k] = ctl.mixed ? c->sig_sum6[k] / ctl.nvox : 0.0;
+        ctl.dcs6[k] = ctl.mixed ? ctl.mean6[k] * ctl.nvox : 0.0;
+    }
+    for (int k = 0; k < 9; ++k) ctl.uniform[k] = c->uniform[k];
+    ctl.done = ctl_stop(ctl);
+    HIPCK(hipMemcpyAsync(c->dctl, &ctl, sizeof(IterCtl), hipMemcpyHostToDevice, c->stream));
     const double divisor = (double)c->nvox;
-    for (;;) {
-        if (c->opt.fixed_iters > 0) {
-            if (iter >= c->opt.fixed_iters) break;
-        } else if (!(iter < c->itermax && std::fmax(std::fabs(c->Err_s), std::fabs(c->Err_e)) > c->err)) {
-            break;
+    const int batch = c->opt.verbose ? 1 : 8;
+    const int most = std::max(c->itermax, c->opt.fixed_iters);
+    int launched = 0;
+    while (!ctl.done) {
+        if (launched > most) return SRX_E_HIP;   // the control kernel must have stopped the loop
+        for (int b = 0; b < batch; ++b) {
+            ++launched;
+            IterCtl *g = c->dctl;
+            { Timer t(c, 0, launched); launch_fft_fwd_z(c->stream, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], g); }
+            { Timer t(c, 1, launched); launch_fft_y(c->stream, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], g); }
+            { Timer t(c, 2, launched); launch_green_x(c->stream, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, g); }
+            { Timer t(c, 3, launched); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], g); }
+            { Timer t(c, 4, launched); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], g, divisor); }
+            { Timer t(c, 5, launched); launch_nr(c->stream, c->dc, c->f, N, launched == 1, rotated, c->partials, g); }
+            { Timer t(c, 7, launched); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red, g); }
+            launch_control(c->stream, c->dc, c->red, g);
         }
-        ++iter;
-        double mean6[6], dcs[6];
-        for (int k = 0; k < 6; ++k) {
-            mean6[k] = c->sig_sum6[k] / c->nvox;
-            dcs[k] = prec == SRX_PREC_MIXED ? mean6[k] * c->nvox : 0.0;
-        }
-        { Timer t(c, 0); launch_fft_fwd_z(c->stream, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], mean6); }
-        { Timer t(c, 1); launch_fft_y(c->stream, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1]); }
-        { Timer t(c, 2); launch_green_x(c->stream, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, dcs); }
-        { Timer t(c, 3); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1]); }
-        { Timer t(c, 4); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], c->dDisGradAvg, divisor); }
-        for (int k = 0; k < 9; ++k) c->uniform[k] += c->dDisGradAvg[k];
-        { Timer t(c, 5); launch_nr(c->stream, c->dc, c->f, N, iter == 1, rotated, c->partials); }
-        { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red); }
-        int rc = sync_read(c, NR_NPART);
-        if (rc) return rc;
-        const double *h = c->hred;
-        if (h[16] > 0) return SRX_E_NAN;
-        for (int k = 0; k < 6; ++k) c->sig_sum6[k] = h[2 + k];
-        nrtot += (long)h[14];
-        nrwave += h[17];
-        nrmax = std::max(nrmax, (int)h[15]);
-        c->Err_e = h[1] * c->wgt;
-        c->Err_s = h[0] * c->wgt;
-        get_smacro(c, h + 2, h + 8);
-        if (std::fabs(c->e_vm) < 1E-5) c->Err_e = 0.0;
-        else c->Err_e /= c->e_vm;
-        c->Err_s /= c->s_vm;
-        if (c->opt.verbose) std::printf("iter %d Err_e %e Err_s %e\n", iter, c->Err_e, c->Err_s);
+        HIPCK(hipMemcpyAsync(&ctl, c->dctl, sizeof(IterCtl), hipMemcpyDeviceToHost, c->stream));
+        HIPCK(hipStreamSynchronize(c->stream));
+        if (c->opt.verbose) std::printf("iter %d Err_e %e Err_s %e\n", ctl.iter, ctl.Err_e, ctl.Err_s);
+    }
+    if (ctl.nan) return SRX_E_NAN;
+    const int iter = ctl.iter, nrmax = ctl.nrmax;
+    const long nrtot = (long)ctl.nrtot;
+    const double nrwave = ctl.nrwave;
+    c->iters_last = iter;
+    c->Err_s = ctl.Err_s;
+    c->Err_e = ctl.Err_e;
+    c->s_vm = ctl.s_vm;
+    c->s_vm1 = ctl.s_vm1;
+    for (int k = 0; k < 6; ++k) c->sig_sum6[k] = ctl.sig_sum6[k];
+    for (int k = 0; k < 9; ++k) {
+        c->dDisGradAvg[k] = ctl.ddg[k];
+        c->acum[k] = ctl.acum[k];
+        c->uniform[k] = ctl.uniform[k];
+        c->SigAvg[k] = ctl.SigAvg[k];
+        c->SigAvg1[k] = ctl.SigAvg1[k];
     }
     r.converged = !(std::fmax(std::fabs(c->Err_s), std::fabs(c->Err_e)) > c->err);
     // evolution.c:1268-1290
@@ -1427,6 +1422,8 @@ void srx_close(srx_ctx *c) {
         if (c->tw[i]) hipFree(c->tw[i]);
     if (c->dc) hipFree(c->dc);
     if (c->hred) hipHostFree(c->hred);
+    if (c->dctl) hipFree(c->dctl);
+    if (c->hctl) hipHostFree(c->hctl);
     for (hipEvent_t e : c->evpool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;

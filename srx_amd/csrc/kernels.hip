@@ -310,7 +310,9 @@ template <typename T>
 __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double *__restrict__ in, cx<T> *__restrict__ specB,
                                                      int ncomp, int n0, int n1, int n2, FftPlan pz,
                                                      const cx<T> *__restrict__ tw, int TY, double m0, double m1,
-                                                     double m2, double m3, double m4, double m5) {
+                                                     double m2, double m3, double m4, double m5,
+                                                     const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int P = n2 + 1;           // padded pencil pitch: rows fall on different LDS banks
@@ -321,7 +323,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
     const int H = n2 / 2 + 1;
     const size_t N = (size_t)n0 * n1 * n2;
     const size_t HS = (size_t)H * n0 * n1;
-    const double mean[6] = {m0, m1, m2, m3, m4, m5};
+    double mean[6] = {m0, m1, m2, m3, m4, m5};
+    if (ctl)
+        for (int k = 0; k < 6; ++k) mean[k] = ctl->mean6[k];
     const FastDiv dn2(n2), dty(ty);
     for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     for (int ca = 0; ca < ncomp; ca += 2) {
@@ -357,7 +361,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
 template <typename T>
 __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                                      int n0, int n1, int n2, int b_to_c, FftPlan py,
-                                                     const cx<T> *__restrict__ tw, int TX) {
+                                                     const cx<T> *__restrict__ tw, int TX,
+                                                     const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int P = n1 + 1;   // padded pencil pitch
@@ -519,7 +525,9 @@ template <typename T, int MAXB>
 __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec, int n0, int n1, int n2, FftPlan px,
                                                        const cx<T> *__restrict__ tw, int TY, int mode, int ncomp,
                                                        const DevConst *__restrict__ cst, double dc0, double dc1,
-                                                       double dc2, double dc3, double dc4, double dc5) {
+                                                       double dc2, double dc3, double dc4, double dc5,
+                                                       const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int H = n2 / 2 + 1;
@@ -567,7 +575,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
                 // that a mean-subtracted (mixed precision) spectrum lacks, B basis.
                 double dr[9], di[9];
                 if (cst->elast_bc == 0) {
-                    const double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
+                    double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
+                    if (ctl)
+                        for (int k = 0; k < 6; ++k) dc6[k] = ctl->dcs6[k];
                     double dcs[6], t9r[9], t9i[9];
                     b6_to_sym(dc6, dcs);
                     sym_to_t9(sr, t9r);
@@ -635,7 +645,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
                                                      int ncomp, int n0, int n1, int n2, FftPlan pz,
                                                      const cx<T> *__restrict__ tw, int TY, int accumulate,
                                                      double divisor, double a0, double a1, double a2, double a3,
-                                                     double a4, double a5, double a6, double a7, double a8) {
+                                                     double a4, double a5, double a6, double a7, double a8,
+                                                     const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
     const int P = n2 + 1;   // padded pencil pitch
@@ -647,7 +659,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
     const int H = n2 / 2 + 1;
     const size_t N = (size_t)n0 * n1 * n2;
     const size_t HS = (size_t)H * n0 * n1;
-    const double add[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+    double add[9] = {a0, a1, a2, a3, a4, a5, a6, a7, a8};
+    if (ctl)
+        for (int k = 0; k < 9; ++k) add[k] = ctl->ddg[k];
     const FastDiv dn2(n2), dty(ty);
     // 1/Nxyz: a power of two for the usual grids, where the product equals the division
     const double rdiv = 1.0 / divisor;
@@ -1267,8 +1281,10 @@ __device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *
 
 template <bool FIRST, bool ROT, bool WS>
 __global__ void __launch_bounds__(NR_BLOCK)
-__attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8))) k_nr(const DevConst *__restrict__ cst, Fields f, int nvox,
-                                                 double *__restrict__ partials) {
+__attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8)))
+k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ partials,
+     const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
     const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
@@ -1342,7 +1358,9 @@ __attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WA
 // result is the same on every run.
 constexpr int RED_BLOCK = 1024;
 __global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
-                                                      unsigned long long maxmask, double *__restrict__ out) {
+                                                      unsigned long long maxmask, double *__restrict__ out,
+                                                      const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
     __shared__ double sh[RED_BLOCK];
     const int k = blockIdx.x;
     const bool ismax = (maxmask >> k) & 1ull;
@@ -1370,6 +1388,38 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__
     }
     if (threadIdx.x == 0) out[k] = sh[0];
     (void)nvals;
+}
+
+// One outer iteration's bookkeeping on the device (the host loop of evolution.c:1150-1256):
+// the dDisGradAvg this iteration's inverse-z pass applied joins the uniform part, the reduced
+// sums give the errors and get_smacro's averages and correction, and `done` decides whether the
+// next iteration runs.  One thread; the iteration's other kernels have finished (stream order).
+__global__ void k_control(const DevConst *__restrict__ cst, const double *__restrict__ red, IterCtl *ctl) {
+    if (threadIdx.x != 0) return;
+    IterCtl &c = *ctl;
+    if (c.done) return;
+    for (int k = 0; k < 9; ++k) c.uniform[k] += c.ddg[k];
+    c.iter += 1;
+    if (red[16] > 0) {   // a non-finite stress (the reference asserts, constitutive.c:683)
+        c.nan = 1;
+        c.done = 1;
+        return;
+    }
+    for (int k = 0; k < 6; ++k) c.sig_sum6[k] = red[2 + k];
+    c.nrtot += (long long)red[14];
+    c.nrmax = max(c.nrmax, (int)red[15]);
+    c.nrwave += red[17];
+    c.Err_e = red[1] * c.wgt;
+    c.Err_s = red[0] * c.wgt;
+    smacro_update(c, cst->S0, red + 2, red + 8);
+    if (fabs(c.e_vm_prev) < 1E-5) c.Err_e = 0.0;
+    else c.Err_e /= c.e_vm_prev;
+    c.Err_s /= c.s_vm;
+    for (int k = 0; k < 6; ++k) {
+        c.mean6[k] = c.mixed ? c.sig_sum6[k] / c.nvox : 0.0;
+        c.dcs6[k] = c.mixed ? c.mean6[k] * c.nvox : 0.0;
+    }
+    c.done = ctl_stop(c);
 }
 
 // increment prologue (evolution.c:1115-1128): dg0 = dg; dg += Udot dt (or = set9 under creep)
@@ -1677,26 +1727,23 @@ static void check_launch(const char *name) {
 }
 
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
-                      const void *twz, const double *mean6) {
-    // ncomp encoded in mean6 == nullptr? no: always the 6 stress comps here
+                      const void *twz, const IterCtl *ctl) {
     const int TY = tile_for(pz, 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
-    const double z6[6] = {0, 0, 0, 0, 0, 0};
-    const double *m = mean6 ? mean6 : z6;
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
-                           n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+                           n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     } else {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
-                           n[2], pz, (const cx<float> *)twz, TY, m[0], m[1], m[2], m[3], m[4], m[5]);
+                           n[2], pz, (const cx<float> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     }
     check_launch("fwd_z");
 }
 
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c, const int n[3],
-                  const FftPlan &py, const void *twy) {
+                  const FftPlan &py, const void *twy, const IterCtl *ctl) {
     (void)sign;
     const int TX = tile_for(py, 1, 2, n[0] < 16 ? n[0] : 16, prec);
     const int H = n[2] / 2 + 1;
@@ -1704,17 +1751,18 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
     if (prec == 0) {
         size_t sm = padded_lds(TX, n[1], sizeof(cx<double>));
         SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
-                           n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX);
+                           n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX, ctl);
     } else {
         size_t sm = padded_lds(TX, n[1], sizeof(cx<float>));
         SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
-                           n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX);
+                           n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX, ctl);
     }
     check_launch("fft_y");
 }
 
 static void launch_green_generic(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px,
-                                 const void *twx, const DevConst *cst, const double *dc6, int mode, int ncomp) {
+                                 const void *twx, const DevConst *cst, const double *dc6, int mode, int ncomp,
+                                 const IterCtl *ctl) {
     const int slots = mode == 1 ? 9 : ncomp;
     // MAXB 6 covers 9 slots of two 128-point pencils with a radix-2 stage
     // two butterflies per thread when one row of `slots` pencils allows it, four otherwise
@@ -1728,42 +1776,42 @@ static void launch_green_generic(hipStream_t s, int prec, void *specC, const int
         size_t sm = (size_t)(slots * TY + 1) * n[0] * sizeof(cx<double>);
         if (wide)
             SRX_LAUNCH((k_green_x<double, 4>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
-                       px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+                       px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5], ctl);
         else
             SRX_LAUNCH((k_green_x<double, 2>), grid, dim3(FFT_BLOCK), sm, s, (cx<double> *)specC, n[0], n[1], n[2],
-                       px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+                       px, (const cx<double> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5], ctl);
     } else {
         size_t sm = (size_t)(slots * TY + 1) * n[0] * sizeof(cx<float>);
         if (wide)
             SRX_LAUNCH((k_green_x<float, 4>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
-                       px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+                       px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5], ctl);
         else
             SRX_LAUNCH((k_green_x<float, 2>), grid, dim3(FFT_BLOCK), sm, s, (cx<float> *)specC, n[0], n[1], n[2],
-                       px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5]);
+                       px, (const cx<float> *)twx, TY, mode, ncomp, cst, d[0], d[1], d[2], d[3], d[4], d[5], ctl);
     }
     check_launch("green_x");
 }
 
 void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px, const void *twx,
-                    const DevConst *cst, const double *dc_sig6_sum) {
-    launch_green_generic(s, prec, specC, n, px, twx, cst, dc_sig6_sum, 1, 9);
+                    const DevConst *cst, const IterCtl *ctl) {
+    launch_green_generic(s, prec, specC, n, px, twx, cst, nullptr, 1, 9, ctl);
 }
 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
-                             const void *twz, const double *ddg9, double divisor) {
+                             const void *twz, const IterCtl *ctl, double divisor) {
     const int TY = tile_for(pz, 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
-    const double *a = ddg9;
+    const double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // dDisGradAvg comes from ctl
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
-                           a[6], a[7], a[8]);
+                           a[6], a[7], a[8], ctl);
     } else {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
-                           a[7], a[8]);
+                           a[7], a[8], ctl);
     }
     check_launch("inv_z");
 }
@@ -1772,26 +1820,31 @@ int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
 int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials) {
+               double *partials, const IterCtl *ctl) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
     if (f.dsig) {   // extrapolated Newton start
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, true>), g, b, 0, s, cst, f, nvox, partials);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, true>), g, b, 0, s, cst, f, nvox, partials);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, true>), g, b, 0, s, cst, f, nvox, partials);
-        else SRX_LAUNCH((k_nr<false, false, true>), g, b, 0, s, cst, f, nvox, partials);
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else SRX_LAUNCH((k_nr<false, false, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
     } else {
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, false>), g, b, 0, s, cst, f, nvox, partials);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, false>), g, b, 0, s, cst, f, nvox, partials);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, false>), g, b, 0, s, cst, f, nvox, partials);
-        else SRX_LAUNCH((k_nr<false, false, false>), g, b, 0, s, cst, f, nvox, partials);
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else SRX_LAUNCH((k_nr<false, false, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
     }
     check_launch("nr");
 }
 
-void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out) {
+void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out, const IterCtl *ctl) {
     const unsigned long long mask = nvals == NR_NPART ? (1ull << 15) : 0ull;
-    SRX_LAUNCH(k_reduce, dim3(nvals), dim3(RED_BLOCK), 0, s, partials, nblocks, nvals, mask, out);
+    SRX_LAUNCH(k_reduce, dim3(nvals), dim3(RED_BLOCK), 0, s, partials, nblocks, nvals, mask, out, ctl);
     check_launch("reduce");
+}
+
+void launch_control(hipStream_t s, const DevConst *cst, const double *red, IterCtl *ctl) {
+    SRX_LAUNCH(k_control, dim3(1), dim3(64), 0, s, cst, red, ctl);
+    check_launch("control");
 }
 
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int creep, const double *st,
@@ -1822,34 +1875,34 @@ void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
-                           n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+                           n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     } else {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
-                           n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0);
+                           n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     }
     check_launch("fwd_z(test)");
-    launch_fft_y(s, prec, specB, specC, ncomp, -1, 1, n, pl[1], tw[1]);
-    launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 0, ncomp);
+    launch_fft_y(s, prec, specB, specC, ncomp, -1, 1, n, pl[1], tw[1], nullptr);
+    launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 0, ncomp, nullptr);
 }
 
 // test path: inverse of the above for ncomp comps (layout C in, real out = IFFT * 1/divisor)
 void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *specB, int ncomp, double *out,
                                   const int n[3], const FftPlan *pl, void *const *tw, double divisor) {
-    launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp);
-    launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1]);
+    launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp, nullptr);
+    launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1], nullptr);
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
-                           0., 0.);
+                           0., 0., nullptr);
     } else {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
-                           0.);
+                           0., nullptr);
     }
     check_launch("inv_z(test)");
 }

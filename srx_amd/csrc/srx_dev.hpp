@@ -51,6 +51,111 @@ struct DevConst {
     int pad;
 };
 
+// Outer fixed-point control kept on the device.  The mixed boundary-condition algebra
+// (get_smacro, evolution.c:590-664) and the convergence test (evolution.c:1150, 1246-1252) run in
+// a one-thread kernel after each iteration's reduction, so the host enqueues several iterations
+// without reading anything back in between; every kernel of an iteration returns at once when
+// `done` is set.
+struct IterCtl {
+    // set by the host at the start of each increment
+    double wgt, wgt_ph1, err, e_vm_prev, svm_stray, nvox;
+    double Scauchy[9];
+    int vbc[6], sbc[6];
+    int svm_mode, itermax, fixed_iters, mixed;
+    // advanced by the control kernel
+    int iter, done, nan, pad0;
+    long long nrtot;
+    int nrmax, pad1;
+    double nrwave;
+    double Err_s, Err_e;
+    double sig_sum6[6];   // grid sums of the stress (B basis) after the last iteration
+    double mean6[6];      // their means: subtracted before the FP32 cast (mixed precision)
+    double dcs6[6];       // the sums reinjected at k = 0 (mixed precision), else 0
+    double ddg[9];        // dDisGradAvg, applied by the next inverse-z pass
+    double acum[9], uniform[9];
+    double SigAvg[9], SigAvg1[9], s_vm, s_vm1;
+};
+
+// B basis of kinematics.c:35-53: component a,b of basis tensor i
+__host__ __device__ constexpr double bbasis(int i, int a, int b) {
+    constexpr double R2 = 0.70710678118654744, R3 = 0.57735026918962584, R6 = 0.40824829046386304;
+    switch (i) {
+    case 0: return a == b ? (a == 0 ? -R2 : (a == 1 ? R2 : 0.0)) : 0.0;
+    case 1: return a == b ? (a == 2 ? 2. * R6 : -R6) : 0.0;
+    case 2: return (a == 1 && b == 2) || (a == 2 && b == 1) ? R2 : 0.0;
+    case 3: return (a == 0 && b == 2) || (a == 2 && b == 0) ? R2 : 0.0;
+    case 4: return (a == 0 && b == 1) || (a == 1 && b == 0) ? R2 : 0.0;
+    default: return a == b ? R3 : 0.0;
+    }
+}
+__host__ __device__ inline void ctl_b6_to_t9(const double v[6], double t[9], int dim) {
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b) {
+            double s = 0.0;
+            for (int i = 0; i < dim; ++i) s += v[i] * bbasis(i, a, b);
+            t[3 * a + b] = s;
+        }
+}
+__host__ __device__ inline void ctl_t9_to_b6(const double t[9], double v[6]) {
+    for (int i = 0; i < 6; ++i) {
+        double s = 0.0;
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) s += t[3 * a + b] * bbasis(i, a, b);
+        v[i] = s;
+    }
+}
+
+// get_smacro (evolution.c:590-664): volume averages, the strain correction dDisGradAvg of the
+// stress-controlled components (upper-triangle entries only), and the two von Mises measures
+// (the first with the out-of-bounds SigDevAvg[3][3] term of evolution.c:648 taken as svm_stray).
+__host__ __device__ inline void smacro_update(IterCtl &c, const double *S0, const double sum6[6],
+                                              const double sum6_ph1[6]) {
+    const int IJV[2][6] = {{0, 1, 2, 1, 0, 0}, {0, 1, 2, 2, 2, 1}};
+    double t[9], t1[9];
+    ctl_b6_to_t9(sum6, t, 6);
+    ctl_b6_to_t9(sum6_ph1, t1, 6);
+    for (int k = 0; k < 9; ++k) {
+        c.SigAvg[k] = t[k] * c.wgt;
+        c.SigAvg1[k] = t1[k] * (c.wgt / c.wgt_ph1);
+    }
+    for (int i = 0; i < 6; ++i) {
+        const int ii = IJV[0][i], jj = IJV[1][i];
+        double acc = 0.0;
+        if (c.vbc[i] == 0) {
+            for (int k = 0; k < 6; ++k) {
+                const int kk = IJV[0][k], ll = IJV[1][k];
+                acc += S0[((ii * 3 + jj) * 3 + kk) * 3 + ll] * c.sbc[k] * (c.Scauchy[3 * kk + ll] - c.SigAvg[3 * kk + ll]);
+            }
+        }
+        c.ddg[3 * ii + jj] = acc;
+    }
+    for (int k = 0; k < 9; ++k) c.acum[k] += c.ddg[k];
+    double sav6[6], sda[9];
+    ctl_t9_to_b6(c.SigAvg, sav6);
+    ctl_b6_to_t9(sav6, sda, 5);
+    const double tr = c.svm_mode == 1 ? (sda[0] + sda[4] + sda[8]) : (sda[4] + sda[8] + c.svm_stray);
+    double s = 0.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const double x = sda[3 * i + j];
+            s += i != j ? x * x : (x - tr / 3.0) * (x - tr / 3.0);
+        }
+    c.s_vm = sqrt(3. / 2. * s);
+    ctl_t9_to_b6(c.SigAvg1, sav6);
+    ctl_b6_to_t9(sav6, sda, 5);
+    s = 0.0;
+    for (int k = 0; k < 9; ++k) s += sda[k] * sda[k];
+    c.s_vm1 = sqrt(3. / 2. * s);
+}
+
+// whether the outer loop stops before iteration c.iter + 1 (evolution.c:1150; fixed_iters > 0:
+// exactly that many iterations)
+__host__ __device__ inline int ctl_stop(const IterCtl &c) {
+    if (c.fixed_iters > 0) return c.iter >= c.fixed_iters;
+    const double e = fabs(c.Err_s) > fabs(c.Err_e) ? fabs(c.Err_s) : fabs(c.Err_e);
+    return !(c.iter < c.itermax && e > c.err);
+}
+
 // Pointers to the per-voxel device arrays (component-major).
 struct Fields {
     double *sig;       // [6][N] stress, B basis, sample frame
@@ -87,17 +192,21 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 
 // ---- launch wrappers (kernels.hip) -------------------------------------
 // Spectrum buffers: specB [c][kz][x][y], specC [c][kz][y][x]; T = double or float.
+// With a non-null `ctl` the iteration's scalars (mean stress, k = 0 sums, dDisGradAvg) come from
+// the device control block and the kernels return at once when its `done` is set.
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3],
-                      const FftPlan &pz, const void *twz, const double *mean6);
+                      const FftPlan &pz, const void *twz, const IterCtl *ctl);
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c,
-                  const int n[3], const FftPlan &py, const void *twy);
+                  const int n[3], const FftPlan &py, const void *twy, const IterCtl *ctl = nullptr);
 void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px, const void *twx,
-                    const DevConst *cst, const double *dc_sig6_sum);
+                    const DevConst *cst, const IterCtl *ctl);
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3],
-                             const FftPlan &pz, const void *twz, const double *ddg9, double scale);
+                             const FftPlan &pz, const void *twz, const IterCtl *ctl, double scale);
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials);
-void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out);
+               double *partials, const IterCtl *ctl = nullptr);
+void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out,
+                   const IterCtl *ctl = nullptr);
+void launch_control(hipStream_t s, const DevConst *cst, const double *red, IterCtl *ctl);
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9,
                      const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,
