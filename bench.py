@@ -52,12 +52,15 @@ def fft_green_bytes_per_voxel(real_bytes: int, n2: int, c_out: int = 9) -> float
     return 48.0 + 16.0 * c_out + (24.0 + 4.0 * c_out) * 2.0 * real_bytes * h
 
 
-def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: int, keep_fields: bool = False) -> dict:
+def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: int, keep_fields: bool = False,
+                  warm_start: int = 0, blob=None) -> dict:
     """One solver run: `warmup` untimed increments, then `steps` timed ones.  timing=2 records
     HIP events around the Newton kernel only (the headline run: the other launches carry no
     event overhead); timing=1 around every kernel (the per-kernel breakdown run)."""
     import srx_amd
-    s = srx_amd.Solver(deck, precision=precision, reserved=[timing])
+    s = srx_amd.Solver(deck, precision=precision, reserved=[timing], warm_start=warm_start)
+    if blob is not None:
+        s.mlp_set(blob)
     for _ in range(warmup):
         s.step()
     kt0 = s.kernel_times()
@@ -155,6 +158,68 @@ def ensemble_bench(args) -> int:
     return 0
 
 
+def learned_bench(args) -> int:
+    """Learned Newton start: the network is trained on the GPU on states recorded from a 64^3,
+    125-grain run (extrapolated start), then the bench workload runs with the plain (reference),
+    extrapolated and learned starts.  Prints one JSON line; not the headline metric."""
+    import srx_amd
+    from srx_amd import inputs
+    work = args.workdir or tempfile.mkdtemp(prefix="srx_mlp_")
+    tdeck = os.path.join(work, "train64", "deck.in")
+    if not os.path.exists(tdeck):
+        inputs.make_case(os.path.dirname(tdeck), n=(64, 64, 64), grains=125, seed=args.seed + 1,
+                         nsteps=args.train_increments + 1, timestep=0.1)
+    t0 = time.perf_counter()
+    s = srx_amd.Solver(tdeck, precision=srx_amd.PREC_MIXED, warm_start=1)
+    s.mlp_record(4, 4000000)
+    for _ in range(args.train_increments):
+        s.step()
+    x, t = s.mlp_samples()
+    s.close()
+    t1 = time.perf_counter()
+    blob, hist = srx_amd.mlp_train(x, t, epochs=20)
+    t2 = time.perf_counter()
+    print("bench: %d training samples recorded in %.1f s, trained in %.1f s (loss %.4f -> %.4f)" % (
+        len(x), t1 - t0, t2 - t1, hist[0], hist[-1]), file=sys.stderr, flush=True)
+    n = (args.n, args.n, args.n)
+    nvox = n[0] * n[1] * n[2]
+    deck = os.path.join(work, "eval%d" % args.n, "deck.in")
+    if not os.path.exists(deck):
+        inputs.make_case(os.path.dirname(deck), n=n, grains=args.grains, seed=args.seed,
+                         nsteps=args.warmup + args.steps + 1, timestep=0.1)
+    out = {}
+    for name, ws in (("plain", 0), ("extrapolated", 1), ("learned", 2)):
+        r = run_precision(deck, srx_amd.PREC_MIXED, args.steps, args.warmup, timing=2, warm_start=ws,
+                          blob=blob if ws == 2 else None)
+        out[name] = {"seconds": r["seconds"], "value": args.steps * nvox / r["seconds"],
+                     "outer_iters": r["iters"], "nr_iters_per_voxel_per_iter": r["nr_total"] / nvox / r["iters"],
+                     "nr_wave_iters_per_iter": r["nr_wave"] / r["iters"], "nr_kernel_ms": r["kernels"]["nr"][0],
+                     "sig_avg": list(r["sig_avg"])}
+        print("bench: %s start: %.3f s, %.3f Newton iterations/voxel/outer iteration, Newton kernel %.1f ms" % (
+            name, r["seconds"], out[name]["nr_iters_per_voxel_per_iter"], r["kernels"]["nr"][0]),
+            file=sys.stderr, flush=True)
+    P, L = out["plain"], out["learned"]
+    ref = np.array(P["sig_avg"])
+    for v in out.values():
+        v["rel_macro_stress_diff_vs_plain"] = float(np.abs(np.array(v.pop("sig_avg")) - ref).max() /
+                                                    np.abs(ref).max())
+    line = {"metric": "converged voxel-increments/s with the learned Newton start at %d^3" % args.n,
+            "value": L["value"], "unit": "voxel-increments/s", "n_gpus": 1, "steps": args.steps,
+            "warmup": args.warmup, "higher_is_better": True, "dtype": "f64+f32+bf16", "data": "synthetic",
+            "config": {"workload": "EVP-FFT increment loop, %d^3 periodic Voronoi, %d grains, mixed precision, "
+                                   "TimeStep 0.1 s, Err 2e-5; network 24-64-64-6 bf16 trained on %d states of "
+                                   "a 64^3 125-grain run" % (args.n, args.grains, len(x))},
+            "train": {"samples": len(x), "record_seconds": t1 - t0, "train_seconds": t2 - t1, "epochs": 20,
+                      "loss_first_epoch": float(hist[0]), "loss_last_epoch": float(hist[-1])},
+            "plain": P, "extrapolated": out["extrapolated"], "learned": L,
+            "speedup_vs_plain": L["value"] / P["value"],
+            "speedup_vs_extrapolated": L["value"] / out["extrapolated"]["value"],
+            "nr_iterations_saved_vs_plain": 1.0 - L["nr_iters_per_voxel_per_iter"] / P["nr_iters_per_voxel_per_iter"],
+            "nr_wave_iterations_saved_vs_plain": 1.0 - L["nr_wave_iters_per_iter"] / P["nr_wave_iters_per_iter"]}
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -171,6 +236,9 @@ def main() -> int:
     ap.add_argument("--ensemble", default=None,
                     help="run an RVE ensemble instead, e.g. 16x32,16x64,12x96,4x128 (count x edge)")
     ap.add_argument("--concurrency", type=int, default=4, help="ensemble members run at once")
+    ap.add_argument("--learned", action="store_true",
+                    help="train the learned Newton start and compare plain, extrapolated and learned starts")
+    ap.add_argument("--train-increments", type=int, default=4, help="increments recorded for training")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU only (the whole grid is resident on one MI355X; "
@@ -179,6 +247,8 @@ def main() -> int:
 
     if args.ensemble:
         return ensemble_bench(args)
+    if args.learned:
+        return learned_bench(args)
     from srx_amd import inputs
     import srx_amd
 

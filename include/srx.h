@@ -63,7 +63,9 @@ typedef struct {
     int    warm_start;       /* 1: each Newton solve after the first of an increment starts from
                                 the stress extrapolated from the last two outer iterations instead of
                                 the last one (converges to the same tolerance in fewer iterations;
-                                results differ from the reference within that tolerance) */
+                                results differ from the reference within that tolerance).
+                                2: the start comes from the learned network installed by srx_mlp_set
+                                (same convergence test; no counterpart in the reference) */
     int    reserved[8];
 } srx_options;
 
@@ -152,6 +154,61 @@ typedef struct {
    No counterpart in the reference, which runs one RVE per mpirun job. */
 int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int n, const srx_options *opt,
                      int concurrency, srx_ensemble_result *results);
+
+/* ---- learned Newton start (no counterpart in the reference) ----------------------------
+   A 24-64-64-6 ReLU perceptron, evaluated in bf16 on the matrix cores with FP32 accumulation
+   inside the Newton kernel, corrects each voxel's extrapolated start: the solve starts at
+   lam + |d| (d / |d| + c), d the previous Newton call's stress change, c the network's output
+   (limited to |c| <= 1; c = 0 is the warm_start 1 start).  The solve still iterates to its own
+   tolerance (evolution.c:217-223) and is repeated from lam if it fails to converge, so the
+   network only changes how many iterations it takes.
+   Parameters travel as one FP32 blob of SRX_MLP_BLOB floats: W1[24][64], b1[64], W2[64][64],
+   b2[64], W3[64][6], b3[6] (W[k][i]: input k -> unit i), feature mean[24] and inverse spread[24],
+   target mean[6] and spread[6].  Features per voxel (raw): orientation quaternion (w >= 0),
+   direction of d (crystal frame, B basis), the start stress lam in the crystal frame / 100 MPa,
+   log10 rho_s of systems 1-7, log10(|d| / |lam|).  Target: the coming call's change over |d|,
+   minus d / |d|. */
+#define SRX_MLP_BLOB 6210
+#define SRX_MLP_NPARAM 6150
+
+typedef struct {
+    int epochs;              /* passes over the samples (default 20) */
+    int batch;               /* samples per Adam step, a multiple of 64 (default 4096) */
+    float lr;                /* initial learning rate (default 2e-3), cosine-decayed to lr_final */
+    float lr_final;          /* (default 1e-4) */
+    unsigned long long seed; /* shuffling and initial weights */
+    int init;                /* 1: compute the normalisation from the samples and draw initial weights;
+                                0: start from the blob passed in */
+    int reserved[7];
+} srx_mlp_train_opts;
+
+void srx_mlp_default_train_options(srx_mlp_train_opts *opt);
+
+/* Trains the network on the GPU (forward on the matrix cores, backward in FP32, Adam) on n samples
+   x[n][24] (raw features) -> t[n][6] (targets); blob is read (init 0) and written;
+   loss_hist[epochs] (may be NULL) receives the mean squared error of the normalised outputs per epoch. */
+int srx_mlp_train(const float *x, const float *t, size_t n, const srx_mlp_train_opts *opt, float *blob,
+                  float *loss_hist);
+
+/* Gradient of mean((y' - t')^2) over n samples and 6 outputs (normalised outputs and targets)
+   with respect to the SRX_MLP_NPARAM parameters, and that loss (one GPU forward/backward pass). */
+int srx_mlp_grad(const float *blob, const float *x, const float *t, size_t n, float *grad, double *loss);
+
+/* Network outputs y[n][6] (de-normalised) for raw features x[n][24]. */
+int srx_mlp_forward(const float *blob, const float *x, size_t n, float *y);
+
+/* Installs the network of a context opened with warm_start 2. */
+int srx_mlp_set(srx_ctx *ctx, const float *blob);
+
+/* Starts (stride > 0) or stops (stride 0) recording training samples: every Newton call after the
+   first of an increment stores, for voxels 0, stride, 2 stride, ..., the features before the call
+   and the target (the correction defined above) after it, until
+   `capacity` samples are held.  Needs warm_start 1 or 2. */
+int srx_mlp_record(srx_ctx *ctx, int stride, size_t capacity);
+
+/* Copies up to max recorded samples with a non-zero previous change (x[.][24], t[.][6]; NULL
+   pointers: count only) and returns their number in *n. */
+int srx_mlp_samples(srx_ctx *ctx, float *x, float *t, size_t max, size_t *n);
 
 /* Text for an error code. */
 const char *srx_strerror(int code);

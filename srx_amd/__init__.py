@@ -66,7 +66,17 @@ FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velg
 
 EXPORTS = ["srx_open", "srx_step", "srx_run", "srx_get_field", "srx_set_field", "srx_dims", "srx_close",
            "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse",
-           "srx_run_ensemble"]
+           "srx_run_ensemble", "srx_mlp_default_train_options", "srx_mlp_train", "srx_mlp_grad",
+           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples"]
+
+MLP_BLOB = 6210     # SRX_MLP_BLOB
+MLP_NPARAM = 6150   # SRX_MLP_NPARAM
+
+
+class MlpTrainOpts(C.Structure):
+    """srx_mlp_train_opts of include/srx.h."""
+    _fields_ = [("epochs", C.c_int), ("batch", C.c_int), ("lr", C.c_float), ("lr_final", C.c_float),
+                ("seed", C.c_ulonglong), ("init", C.c_int), ("reserved", C.c_int * 7)]
 
 
 class EnsembleResult(C.Structure):
@@ -102,6 +112,15 @@ def lib():
                                       P(C.c_double)]
         L.srx_run_ensemble.argtypes = [P(C.c_char_p), P(C.c_char_p), C.c_int, P(Options), C.c_int,
                                        P(EnsembleResult)]
+        F = P(C.c_float)
+        L.srx_mlp_default_train_options.argtypes = [P(MlpTrainOpts)]
+        L.srx_mlp_default_train_options.restype = None
+        L.srx_mlp_train.argtypes = [F, F, C.c_size_t, P(MlpTrainOpts), F, F]
+        L.srx_mlp_grad.argtypes = [F, F, F, C.c_size_t, F, P(C.c_double)]
+        L.srx_mlp_forward.argtypes = [F, F, C.c_size_t, F]
+        L.srx_mlp_set.argtypes = [C.c_void_p, F]
+        L.srx_mlp_record.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+        L.srx_mlp_samples.argtypes = [C.c_void_p, F, F, C.c_size_t, P(C.c_size_t)]
         _lib = L
     return _lib
 
@@ -109,6 +128,11 @@ def lib():
 def _dptr(a: np.ndarray):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _fptr(a: np.ndarray):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
 class SrxError(RuntimeError):
@@ -181,6 +205,34 @@ class Solver:
         self.L.srx_kernel_times(self.h, names, ms, calls, C.byref(n))
         return {names[i].decode(): (ms[i], calls[i]) for i in range(n.value)}
 
+    def mlp_set(self, blob: np.ndarray) -> None:
+        """Install the learned Newton start (context opened with warm_start=2)."""
+        b = np.ascontiguousarray(blob, dtype=np.float32)
+        assert b.size == MLP_BLOB
+        rc = self.L.srx_mlp_set(self.h, _fptr(b))
+        if rc:
+            raise SrxError(rc, "srx_mlp_set")
+
+    def mlp_record(self, stride: int, capacity: int) -> None:
+        """Record training samples of voxels 0, stride, ... (stride 0: stop and free)."""
+        rc = self.L.srx_mlp_record(self.h, int(stride), int(capacity))
+        if rc:
+            raise SrxError(rc, "srx_mlp_record")
+
+    def mlp_samples(self) -> tuple[np.ndarray, np.ndarray]:
+        """Recorded samples (features [n][24], targets [n][6])."""
+        n = C.c_size_t()
+        rc = self.L.srx_mlp_samples(self.h, None, None, 0, C.byref(n))
+        if rc:
+            raise SrxError(rc, "srx_mlp_samples")
+        x = np.zeros((n.value, 24), np.float32)
+        t = np.zeros((n.value, 6), np.float32)
+        m = C.c_size_t()
+        rc = self.L.srx_mlp_samples(self.h, _fptr(x), _fptr(t), n.value, C.byref(m))
+        if rc:
+            raise SrxError(rc, "srx_mlp_samples")
+        return x[:m.value], t[:m.value]
+
     def close(self) -> None:
         if self.h:
             self.L.srx_close(self.h)
@@ -221,6 +273,55 @@ def fft_inverse(X: np.ndarray, n2: int, precision: int = PREC_FP64, scale: float
     if rc:
         raise SrxError(rc, "srx_fft_inverse")
     return out.reshape(nc, n0, n1, n2)
+
+
+def mlp_train(x: np.ndarray, t: np.ndarray, blob: np.ndarray | None = None, **kw):
+    """Train the learned Newton start on the GPU.  Returns (blob, loss per epoch).  kw: fields of
+    srx_mlp_train_opts (epochs, batch, lr, lr_final, seed); with `blob` given, training continues
+    from it (init 0)."""
+    L = lib()
+    o = MlpTrainOpts()
+    L.srx_mlp_default_train_options(C.byref(o))
+    for k, v in kw.items():
+        setattr(o, k, v)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    assert x.ndim == 2 and x.shape[1] == 24 and t.shape == (x.shape[0], 6)
+    if blob is None:
+        b = np.zeros(MLP_BLOB, np.float32)
+        o.init = 1
+    else:
+        b = np.array(blob, dtype=np.float32).ravel()
+        o.init = 0
+    hist = np.zeros(max(o.epochs, 1), np.float32)
+    rc = L.srx_mlp_train(_fptr(x), _fptr(t), x.shape[0], C.byref(o), _fptr(b), _fptr(hist))
+    if rc:
+        raise SrxError(rc, "srx_mlp_train")
+    return b, hist
+
+
+def mlp_grad(blob: np.ndarray, x: np.ndarray, t: np.ndarray):
+    """(gradient [6150], loss) of the normalised mean squared error, one GPU pass."""
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    t = np.ascontiguousarray(t, dtype=np.float32)
+    g = np.zeros(MLP_NPARAM, np.float32)
+    loss = C.c_double()
+    rc = lib().srx_mlp_grad(_fptr(b), _fptr(x), _fptr(t), x.shape[0], _fptr(g), C.byref(loss))
+    if rc:
+        raise SrxError(rc, "srx_mlp_grad")
+    return g, loss.value
+
+
+def mlp_forward(blob: np.ndarray, x: np.ndarray) -> np.ndarray:
+    """Network outputs [n][6] for raw features [n][24] (matrix-core forward pass)."""
+    b = np.ascontiguousarray(blob, dtype=np.float32)
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    y = np.zeros((x.shape[0], 6), np.float32)
+    rc = lib().srx_mlp_forward(_fptr(b), _fptr(x), x.shape[0], _fptr(y))
+    if rc:
+        raise SrxError(rc, "srx_mlp_forward")
+    return y
 
 
 def run_ensemble(decks, workdirs=None, concurrency: int = 4, **opt) -> list[dict]:

@@ -28,10 +28,12 @@ def build(verbose_resources: bool = False) -> None:
     kobj = os.path.join(objdir, "kernels.o")
     hobj = os.path.join(objdir, "host.o")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
+    mobj = os.path.join(objdir, "mlp.o")
     run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "kernels.hip"), "-o", kobj])
+    run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "mlp.hip"), "-o", mobj])
     run([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
          os.path.join(CSRC, "host.cpp"), "-o", hobj])
-    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", LIB, kobj, hobj])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", LIB, kobj, mobj, hobj])
     run([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
          os.path.join(CSRC, "cli.cpp"), "-o", CLI, "-L" + HERE, "-lsrx", "-Wl,-rpath,$ORIGIN",
          "-L/opt/rocm/lib", "-lamdhip64"])

@@ -383,6 +383,12 @@ struct srx_ctx {
     struct KT { const char *name; double ms; int calls; };
     KT kt[8];
     int nkt = 0;
+    // learned Newton start: packed network, and the training-sample recorder
+    MlpPacked *mlp = nullptr;
+    float *mlp_blob = nullptr;
+    float *rec_x = nullptr, *rec_t = nullptr, *rec_nd = nullptr;
+    int *rec_count = nullptr;
+    int rec_stride = 0, rec_cap = 0;
 };
 
 namespace {
@@ -721,7 +727,7 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.rho_inplane, 12, 0.0)) || (rc = dalloc(&F.rho_tot, 1, 0.0)) ||
         (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
-    if (c->opt.warm_start == 1) {
+    if (c->opt.warm_start == 1 || c->opt.warm_start == 2) {
         HIPCK(hipMalloc(&F.dsig, sizeof(float) * 6 * N));
         HIPCK(hipMemset(F.dsig, 0, sizeof(float) * 6 * N));
     }
@@ -1038,6 +1044,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     if (!c) return SRX_E_ARG;
     if (c->istep >= c->nsteps) return SRX_E_DONE;
     if (c->creep == 1 && c->e_vm > 0.3) return SRX_E_CREEP_STOP;
+    if (c->opt.warm_start == 2 && !c->mlp) return SRX_E_ARG;   // srx_mlp_set first
     const Deck &d = c->dk;
     const double dt = d.TimeStep;
     const int prec = c->opt.precision;
@@ -1099,7 +1106,14 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
             { Timer t(c, 2, launched); launch_green_x(c->stream, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, g); }
             { Timer t(c, 3, launched); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], g); }
             { Timer t(c, 4, launched); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], g, divisor); }
+            const bool rec = c->rec_stride > 0 && launched > 1;
+            if (rec)
+                launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 0, c->rec_x, c->rec_t, c->rec_nd,
+                                  c->rec_count, c->rec_cap, g);
             { Timer t(c, 5, launched); launch_nr(c->stream, c->dc, c->f, N, launched == 1, rotated, c->partials, g); }
+            if (rec)
+                launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 1, c->rec_x, c->rec_t, c->rec_nd,
+                                  c->rec_count, c->rec_cap, g);
             { Timer t(c, 7, launched); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red, g); }
             launch_control(c->stream, c->dc, c->red, g);
         }
@@ -1414,6 +1428,10 @@ void srx_close(srx_ctx *c) {
     for (double *p : ptrs)
         if (p) hipFree(p);
     if (c->f.dsig) hipFree(c->f.dsig);
+    if (c->mlp) hipFree(c->mlp);
+    if (c->mlp_blob) hipFree(c->mlp_blob);
+    for (void *p : {(void *)c->rec_x, (void *)c->rec_t, (void *)c->rec_nd, (void *)c->rec_count})
+        if (p) hipFree(p);
     if (c->f.phase) hipFree(c->f.phase);
     if (c->f.nr_it) hipFree(c->f.nr_it);
     if (c->specB) hipFree(c->specB);
@@ -1427,6 +1445,79 @@ void srx_close(srx_ctx *c) {
     for (hipEvent_t e : c->evpool) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+// ---- learned Newton start --------------------------------------------------
+int srx_mlp_set(srx_ctx *c, const float *blob) {
+    if (!c || !blob || c->opt.warm_start != 2) return SRX_E_ARG;
+    for (int k = 0; k < 6; ++k)
+        if (!(blob[SRX_MLP_NPARAM + 48 + 6 + k] != 0.0f)) return SRX_E_ARG;   // target spread
+    if (!c->mlp) {
+        HIPCK(hipMalloc(&c->mlp, mlp_packed_bytes()));
+        HIPCK(hipMalloc(&c->mlp_blob, sizeof(float) * SRX_MLP_BLOB));
+    }
+    HIPCK(hipMemcpyAsync(c->mlp_blob, blob, sizeof(float) * SRX_MLP_BLOB, hipMemcpyHostToDevice, c->stream));
+    mlp_pack(c->stream, c->mlp_blob, c->mlp);
+    HIPCK(hipGetLastError());
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->f.mlp = c->mlp;
+    return 0;
+}
+
+static void free_recorder(srx_ctx *c) {
+    for (void *p : {(void *)c->rec_x, (void *)c->rec_t, (void *)c->rec_nd, (void *)c->rec_count})
+        if (p) hipFree(p);
+    c->rec_x = c->rec_t = c->rec_nd = nullptr;
+    c->rec_count = nullptr;
+    c->rec_stride = c->rec_cap = 0;
+}
+
+int srx_mlp_record(srx_ctx *c, int stride, size_t capacity) {
+    if (!c || stride < 0) return SRX_E_ARG;
+    HIPCK(hipStreamSynchronize(c->stream));
+    free_recorder(c);
+    if (stride == 0) return 0;
+    if (!c->f.dsig || capacity == 0 || capacity > ((size_t)1 << 28)) return SRX_E_ARG;
+    const size_t cap = capacity;
+    HIPCK(hipMalloc(&c->rec_x, sizeof(float) * 24 * cap));
+    HIPCK(hipMalloc(&c->rec_t, sizeof(float) * 6 * cap));
+    HIPCK(hipMalloc(&c->rec_nd, sizeof(float) * cap));
+    HIPCK(hipMalloc(&c->rec_count, sizeof(int)));
+    HIPCK(hipMemsetAsync(c->rec_count, 0, sizeof(int), c->stream));
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->rec_stride = stride;
+    c->rec_cap = (int)cap;
+    return 0;
+}
+
+int srx_mlp_samples(srx_ctx *c, float *x, float *t, size_t max, size_t *n) {
+    if (!c || !n) return SRX_E_ARG;
+    *n = 0;
+    if (!c->rec_count) return 0;
+    int cnt = 0;
+    HIPCK(hipStreamSynchronize(c->stream));
+    HIPCK(hipMemcpy(&cnt, c->rec_count, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<float> hx((size_t)cnt * 24), ht((size_t)cnt * 6), hn(cnt);
+    if (cnt) {
+        HIPCK(hipMemcpy(hx.data(), c->rec_x, hx.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(ht.data(), c->rec_t, ht.size() * sizeof(float), hipMemcpyDeviceToHost));
+        HIPCK(hipMemcpy(hn.data(), c->rec_nd, hn.size() * sizeof(float), hipMemcpyDeviceToHost));
+    }
+    size_t k = 0;
+    for (int i = 0; i < cnt; ++i) {
+        bool ok = hn[i] > 0.0f && std::isfinite(hn[i]);
+        for (int j = 0; j < 24 && ok; ++j) ok = std::isfinite(hx[(size_t)i * 24 + j]);
+        for (int j = 0; j < 6 && ok; ++j) ok = std::isfinite(ht[(size_t)i * 6 + j]);
+        if (!ok) continue;
+        if (x && t) {
+            if (k >= max) break;
+            std::memcpy(x + k * 24, &hx[(size_t)i * 24], 24 * sizeof(float));
+            std::memcpy(t + k * 6, &ht[(size_t)i * 6], 6 * sizeof(float));
+        }
+        ++k;
+    }
+    *n = k;
+    return 0;
 }
 
 // ---- single-kernel entry points -------------------------------------------

@@ -12,6 +12,7 @@
 // Block partial sums are reduced by reduce_partials in a fixed order, so every
 // sum is bitwise repeatable from run to run.
 #include "srx_dev.hpp"
+#include "mlp_dev.hpp"
 
 #include <hip/hip_runtime.h>
 #include <hip/hip_ext.h>
@@ -742,6 +743,20 @@ __device__ __forceinline__ void q5_from(const double a[9], double Q[5][5]) {
 #undef SRX_OUTER
 }
 
+// sample-frame stress (B basis) -> crystal frame of a: deviatoric part Q^T lam, hydrostatic unchanged
+__device__ __forceinline__ void crystal_stress(const double a[9], const double lam6[6], double lamc[6]) {
+    double Q[5][5];
+    q5_from(a, Q);
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double al = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) al += Q[k][i] * lam6[k];
+        lamc[i] = al;
+    }
+    lamc[5] = lam6[5];
+}
+
 // packed symmetric 5x5: index of (i,j), i<=j
 __device__ __forceinline__ constexpr int ps(int i, int j) {
     return i <= j ? (i * (9 - i)) / 2 + j : (j * (9 - j)) / 2 + i;
@@ -903,7 +918,8 @@ __device__ __forceinline__ double wave_max(double v) {
 struct NrResult {
     double cs, ce;
     double sig[6];
-    int it;
+    int it;     // iterations of the solve that produced sig
+    int itot;   // iterations executed (both attempts of a repeated solve)
     int bad;
 };
 
@@ -941,9 +957,11 @@ __device__ __forceinline__ double nr_in(const double *stg, int slot, const doubl
     return STAGED ? stg[(slot + k) * NR_BLOCK] : g[k * N + v];
 }
 
-template <bool FIRST, bool ROT, int PH, bool WS>
+// WS: Newton start 0 = lam (the reference), 1 = lam + the previous call's change (extrapolated),
+// 2 = lam + the learned prediction `wsp` (crystal frame)
+template <bool FIRST, bool ROT, int PH, int WS>
 __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                                         double *lds, const double *stg, NrResult &res) {
+                                         double *lds, const double *stg, const float *wsp, NrResult &res) {
     constexpr bool STAGED = !FIRST;
     // compile-time phase index: every material constant is a scalar load at a fixed offset
     const DevPhase &P = cst->ph[PH];
@@ -994,13 +1012,6 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         }
         lamh = lam6[5];
         qh = -ec[5];
-        if (!FIRST && WS) {
-            // extrapolated start: lam + (lam - lam of the previous outer iteration), the change
-            // kept in the crystal frame of this increment's orientation
-#pragma unroll
-            for (int i = 0; i < 5; ++i) ws[i] = (double)f.dsig[i * N + v];
-            wsh = (double)f.dsig[5 * N + v];
-        }
         sc_rotate(&cst->S066[0][0], Q, A);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -1059,13 +1070,38 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         }
     }
 
+    if (!FIRST && WS == 1) {
+        // extrapolated start: lam + (lam - lam of the previous outer iteration), the change
+        // kept in the crystal frame of this increment's orientation
+#pragma unroll
+        for (int i = 0; i < 5; ++i) ws[i] = (double)f.dsig[i * N + v];
+        wsh = (double)f.dsig[5 * N + v];
+    } else if (!FIRST && WS == 2) {
+#pragma unroll
+        for (int i = 0; i < 5; ++i) ws[i] = (double)wsp[i];
+        wsh = (double)wsp[5];
+    }
     double sg[5], sh = lamh + wsh, pre[5], preh = sh, ed[5];
 #pragma unroll
     for (int i = 0; i < 5; ++i) { sg[i] = lam[i] + ws[i]; pre[i] = sg[i]; ed[i] = 0.0; }
-    int it = 0;
+    int it = 0, itot = 0;
     double erral = 10 * 1E-10;
+    // With the learned start (WS 2), a solve that does not converge (non-finite iterate, or 500
+    // iterations) is repeated from lam, the reference's start; `itot` counts both attempts.  (The
+    // extrapolated start has not needed it; the second attempt costs it registers.)
+    for (int att = 0; att < (WS == 2 ? 2 : 1); ++att) {
+    if (att == 1) {
+        if (erral <= 1E-10) break;
+#pragma unroll
+        for (int i = 0; i < 5; ++i) { sg[i] = lam[i]; pre[i] = sg[i]; }
+        sh = lamh;
+        preh = sh;
+        it = 0;
+        erral = 10 * 1E-10;
+    }
     while (it < 500 && erral > 1E-10) {
         ++it;
+        ++itot;
         double K[15];
 #pragma unroll
         for (int k = 0; k < 15; ++k) K[k] = A[k];
@@ -1142,6 +1178,13 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         sh -= dh;
         erral = sqrt(cnr) * rsig;
     }
+    }
+    if (WS) {
+        // this call's stress change (crystal frame) for the next call's extrapolated start
+#pragma unroll
+        for (int i = 0; i < 5; ++i) f.dsig[i * N + v] = (float)(sg[i] - lam[i]);
+        f.dsig[5 * N + v] = (float)(sh - lamh);
+    }
     // errors of evolution.c:381-391 (rotation invariant)
     double cs2 = (sh - lamh) * (sh - lamh);
 #pragma unroll
@@ -1203,17 +1246,12 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             for (int k = 0; k < 5; ++k) acc += Q[i][k] * sg[k];
             res.sig[i] = acc;
         }
-        if (WS) {
-            // this call's stress change (crystal frame) for the next call's extrapolated start
-#pragma unroll
-            for (int i = 0; i < 5; ++i) f.dsig[i * N + v] = (float)(sg[i] - lam[i]);
-            f.dsig[5 * N + v] = (float)(sh - lamh);
-        }
     }
     res.sig[5] = sh;
 #pragma unroll
     for (int k = 0; k < 5; ++k) f.sig_pre[k * N + v] = pre[k];
     res.it = it;
+    res.itot = itot;
     bool bad = false;
 #pragma unroll
     for (int k = 0; k < 6; ++k) bad |= !isfinite(res.sig[k]);
@@ -1279,7 +1317,42 @@ __device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *
     __syncthreads();
 }
 
-template <bool FIRST, bool ROT, bool WS>
+// Learned start of the wavefront's voxels (every lane calls; non-crystalline lanes get zero).
+// Features from the staged orientation and stress, the previous change and rho_s.
+__device__ __forceinline__ void nr_learned_start(const DevConst *__restrict__ cst, const Fields &f, size_t N, int v,
+                                                 bool xt, const double *stg, float wsp[6]) {
+    float xf[MLP_IN], nd = 0.0f;
+    if (xt) {
+        double a[9], lam6[6], lamc[6], rho[7];
+        float d[6];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) a[k] = stg[(ST_TRAN + k) * NR_BLOCK];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) lam6[k] = stg[(ST_SIG + k) * NR_BLOCK];
+        crystal_stress(a, lam6, lamc);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) d[k] = f.dsig[k * N + v];
+#pragma unroll
+        for (int k = 0; k < 7; ++k) rho[k] = f.rho_s[k * N + v];
+        mlp_features(a, lamc, d, rho, xf, nd);
+    } else {
+#pragma unroll
+        for (int k = 0; k < MLP_IN; ++k) xf[k] = 0.0f;
+    }
+    float u[MLP_OUT];
+#pragma unroll
+    for (int k = 0; k < MLP_OUT; ++k) u[k] = xf[4 + k];
+    mlp_normalise(f.mlp, xf);
+    float y[MLP_OUT];
+    MlpAcc unused;
+    mlp_forward_wave<false>(f.mlp, xf, y, unused);
+    double w6[6] = {0, 0, 0, 0, 0, 0};
+    if (xt) mlp_start(f.mlp, y, u, nd, w6);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) wsp[k] = (float)w6[k];
+}
+
+template <bool FIRST, bool ROT, int WS>
 __global__ void __launch_bounds__(NR_BLOCK)
 __attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8)))
 k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ partials,
@@ -1301,8 +1374,11 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
     __shared__ __align__(16) double lds[NSLOT * NR_BLOCK];
     if constexpr (!FIRST) stage_inputs<NST>(f, nvox, lds);
     const double *stg = lds + threadIdx.x;
+    float wsp[6] = {0, 0, 0, 0, 0, 0};
+    if constexpr (!FIRST && WS == 2)
+        nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, stg, wsp);
     NrResult r;
-    r.cs = 0.0; r.ce = 0.0; r.it = 0; r.bad = 0;
+    r.cs = 0.0; r.ce = 0.0; r.it = 0; r.itot = 0; r.bad = 0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) r.sig[k] = 0.0;
     // one pass per distinct phase present in the wavefront, with the phase index uniform
@@ -1313,9 +1389,9 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS>(cst, f, v, N, lds, stg, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS>(cst, f, v, N, lds, stg, wsp, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS>(cst, f, v, N, lds, stg, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS>(cst, f, v, N, lds, stg, wsp, r);
             }
         }
         todo &= ~__ballot(mine);
@@ -1341,9 +1417,9 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
 #pragma unroll
         for (int k = 0; k < 6; ++k) red[8 + k] = red[2 + k];
     }
-    red[14] = wave_sum((double)r.it);
+    red[14] = wave_sum((double)r.itot);
     // the wavefront's largest iteration count (the lane-iterations it executed / 64)
-    red[15] = wave_max((double)r.it);
+    red[15] = wave_max((double)r.itot);
     red[16] = __any(r.bad) ? 1.0 : 0.0;
     red[17] = red[15];
     if (lane == 0) {
@@ -1673,6 +1749,60 @@ __global__ void __launch_bounds__(END_BLOCK) k_end(const DevConst *__restrict__ 
 }
 
 // ---------------------------------------------------------------------------
+// Training samples for the learned Newton start: voxel j * stride (j < nsamp).  Before a Newton
+// call: features and the scale |d| of the previous change, at slot *count + j.  After it: the
+// target, this call's change over that scale minus the previous change's direction (the
+// correction to the extrapolated start), and the count advanced by nsamp (recording stops when
+// the buffer is full).
+// ---------------------------------------------------------------------------
+__global__ void k_mlp_rec_x(const DevConst *__restrict__ cst, Fields f, int nvox, int stride, int nsamp,
+                            float *__restrict__ X, float *__restrict__ ND, const int *__restrict__ count, int cap,
+                            const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int base = *count;
+    if (j >= nsamp || base + nsamp > cap) return;
+    const int v = j * stride;
+    const size_t N = (size_t)nvox;
+    const int ph = f.phase[v];
+    float x[MLP_IN], nd = 0.0f;
+    if (ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal) {
+        double a[9], lam6[6], lamc[6], rho[7];
+        float d[6];
+        for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+        for (int k = 0; k < 6; ++k) lam6[k] = f.sig[k * N + v];
+        crystal_stress(a, lam6, lamc);
+        for (int k = 0; k < 6; ++k) d[k] = f.dsig[k * N + v];
+        for (int k = 0; k < 7; ++k) rho[k] = f.rho_s[k * N + v];
+        mlp_features(a, lamc, d, rho, x, nd);
+    } else {
+        for (int k = 0; k < MLP_IN; ++k) x[k] = 0.0f;
+    }
+    for (int k = 0; k < MLP_IN; ++k) X[(size_t)(base + j) * MLP_IN + k] = x[k];
+    ND[base + j] = nd;
+}
+
+__global__ void k_mlp_rec_t(Fields f, int nvox, int stride, int nsamp, const float *__restrict__ X,
+                            float *__restrict__ T, const float *__restrict__ ND, const int *__restrict__ count, int cap,
+                            const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    const int base = *count;
+    if (j >= nsamp || base + nsamp > cap) return;
+    const int v = j * stride;
+    const size_t N = (size_t)nvox;
+    const float nd = ND[base + j];
+    const float *u = X + (size_t)(base + j) * MLP_IN + 4;
+    for (int k = 0; k < MLP_OUT; ++k)
+        T[(size_t)(base + j) * MLP_OUT + k] = nd > 0.0f ? f.dsig[k * N + v] / nd - u[k] : 0.0f;
+}
+
+__global__ void k_mlp_rec_adv(int *__restrict__ count, int cap, int nsamp, const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    if (threadIdx.x == 0 && *count + nsamp <= cap) *count += nsamp;
+}
+
+// ---------------------------------------------------------------------------
 // host launch wrappers
 // ---------------------------------------------------------------------------
 static int min_radix(const FftPlan &p) {
@@ -1816,22 +1946,40 @@ void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double 
     check_launch("inv_z");
 }
 
+void launch_mlp_record(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int stride, int after,
+                       float *X, float *T, float *ND, int *count, int cap, const IterCtl *ctl) {
+    const int nsamp = (nvox + stride - 1) / stride;
+    const dim3 g((nsamp + 255) / 256), b(256);
+    if (!after) {
+        hipLaunchKernelGGL(k_mlp_rec_x, g, b, 0, s, cst, f, nvox, stride, nsamp, X, ND, count, cap, ctl);
+    } else {
+        hipLaunchKernelGGL(k_mlp_rec_t, g, b, 0, s, f, nvox, stride, nsamp, X, T, ND, count, cap, ctl);
+        hipLaunchKernelGGL(k_mlp_rec_adv, dim3(1), dim3(64), 0, s, count, cap, nsamp, ctl);
+    }
+    check_launch("mlp_record");
+}
+
 int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
 int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
                double *partials, const IterCtl *ctl) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
-    if (f.dsig) {   // extrapolated Newton start
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else SRX_LAUNCH((k_nr<false, false, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+    if (f.dsig && f.mlp) {   // learned Newton start (the first call of an increment starts at lam)
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, 2>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else SRX_LAUNCH((k_nr<false, false, 2>), g, b, 0, s, cst, f, nvox, partials, ctl);
+    } else if (f.dsig) {   // extrapolated Newton start
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else SRX_LAUNCH((k_nr<false, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
     } else {
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else SRX_LAUNCH((k_nr<false, false, false>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else SRX_LAUNCH((k_nr<false, false, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
     }
     check_launch("nr");
 }

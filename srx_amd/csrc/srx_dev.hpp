@@ -157,6 +157,8 @@ __host__ __device__ inline int ctl_stop(const IterCtl &c) {
 }
 
 // Pointers to the per-voxel device arrays (component-major).
+struct MlpPacked;   // bf16 operands of the learned Newton start (mlp_dev.hpp)
+
 struct Fields {
     double *sig;       // [6][N] stress, B basis, sample frame
     double *sig_pre;   // [5][N] last NR pre-update iterate, crystal frame of tran (deviatoric)
@@ -171,6 +173,7 @@ struct Fields {
     double *rho_tot;   // [N] rho_tot of the last hardening update (evolution.c:1030-1048)
     double *fl0;       // [9][N] fluctuation carried over DisGrad resets (creep runs only, else null)
     float *dsig;       // [6][N] last Newton call's stress change, B basis, crystal frame (warm start only)
+    const MlpPacked *mlp;   // learned Newton start (non-null: the start comes from the network)
     int *phase;        // [N]
     int *nr_it;        // [N]
 };
@@ -211,6 +214,13 @@ void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int 
                      const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,
                 int do_orient, int do_hardening, int istep, double *partials);
+// Learned Newton start: operand packing (mlp.hip) and the recording of training samples around a
+// Newton call (kernels.hip): `after` = 0 writes the features of the sampled voxels before the
+// call, 1 their targets after it and advances the sample count.
+void mlp_pack(hipStream_t s, const float *dblob, MlpPacked *M);
+size_t mlp_packed_bytes();
+void launch_mlp_record(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int stride, int after,
+                       float *X, float *T, float *ND, int *count, int cap, const IterCtl *ctl);
 int nr_block_count(int nvox);
 int end_block_count(int nvox);
 // Field conversion helpers used by srx_get_field (device -> AoS tensor on device scratch)
