@@ -66,7 +66,9 @@ typedef struct {
                                 results differ from the reference within that tolerance).
                                 2: the start comes from the learned network installed by srx_mlp_set
                                 (same convergence test; no counterpart in the reference) */
-    int    reserved[8];
+    int    reserved[8];      /* [0]: kernel timing (srx_kernel_times); [1] = 1: enqueue the outer
+                                iterations launch by launch instead of replaying each batch of 8
+                                from a HIP graph captured on first use (the default) */
 } srx_options;
 
 /* Everything the reference prints or writes per increment, plus counters it

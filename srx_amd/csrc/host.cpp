@@ -34,6 +34,8 @@ namespace {
 // ---------------------------------------------------------------------------
 // constants of evp.h:84-94
 // ---------------------------------------------------------------------------
+// outer iterations enqueued between two read-backs of the control block
+constexpr int ITER_BATCH = 8;
 constexpr double PI_REF = 3.14159265359;
 constexpr double EV_IN_J = 1.60217733e-19;
 constexpr double J_IN_EV = (1.0 / EV_IN_J);
@@ -389,6 +391,9 @@ struct srx_ctx {
     float *rec_x = nullptr, *rec_t = nullptr, *rec_nd = nullptr;
     int *rec_count = nullptr;
     int rec_stride = 0, rec_cap = 0;
+    // outer-iteration batches captured as HIP graphs: [first batch of an increment][rotated]
+    hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
+    bool graphs = true;
 };
 
 namespace {
@@ -423,7 +428,7 @@ bool grid_supported(const FftPlan pl[3], const int n[3]) {
 }
 
 template <typename T>
-int upload_twiddles(int n, void **dst) {
+int upload_twiddles(int n, void **dst, hipStream_t st = nullptr) {
     std::vector<T> h(2 * std::max(n, 1));
     for (int m = 0; m < n; ++m) {
         const long double ang = -2.0L * 3.14159265358979323846264338327950288L * (long double)m / (long double)n;
@@ -431,7 +436,22 @@ int upload_twiddles(int n, void **dst) {
         h[2 * m + 1] = (T)std::sin(ang);
     }
     HIPCK(hipMalloc(dst, h.size() * sizeof(T)));
-    HIPCK(hipMemcpy(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice));
+    HIPCK(hipMemcpyAsync(*dst, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    HIPCK(hipStreamSynchronize(st));
+    return 0;
+}
+
+// Copies and fills of a context's buffers, ordered on its stream and complete on return.  The
+// context streams do not synchronise with the legacy stream, so one context's setup never waits
+// for (or breaks the graph capture of) another context working concurrently.
+int ctx_copy(hipStream_t st, void *dst, const void *src, size_t bytes, hipMemcpyKind kind) {
+    HIPCK(hipMemcpyAsync(dst, src, bytes, kind, st));
+    HIPCK(hipStreamSynchronize(st));
+    return 0;
+}
+int ctx_zero(hipStream_t st, void *dst, size_t bytes) {
+    HIPCK(hipMemsetAsync(dst, 0, bytes, st));
+    HIPCK(hipStreamSynchronize(st));
     return 0;
 }
 
@@ -702,18 +722,15 @@ int setup(srx_ctx *c) {
     c->nsteps = c->opt.nsteps_override > 0 ? c->opt.nsteps_override : d.N_steps;
 
     // device allocation
-    HIPCK(hipStreamCreate(&c->stream));
+    HIPCK(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+    hipStream_t st = c->stream;
     HIPCK(hipMalloc(&c->dc, sizeof(DevConst)));
-    HIPCK(hipMemcpy(c->dc, &c->hc, sizeof(DevConst), hipMemcpyHostToDevice));
+    if (int e = ctx_copy(st, c->dc, &c->hc, sizeof(DevConst), hipMemcpyHostToDevice)) return e;
     auto dalloc = [&](double **p, int comps, double init) -> int {
         HIPCK(hipMalloc(p, sizeof(double) * comps * N));
-        if (init == 0.0) {
-            HIPCK(hipMemset(*p, 0, sizeof(double) * comps * N));
-        } else {
-            std::vector<double> h((size_t)comps * N, init);
-            HIPCK(hipMemcpy(*p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice));
-        }
-        return 0;
+        if (init == 0.0) return ctx_zero(st, *p, sizeof(double) * comps * N);
+        std::vector<double> h((size_t)comps * N, init);
+        return ctx_copy(st, *p, h.data(), h.size() * sizeof(double), hipMemcpyHostToDevice);
     };
     Fields &F = c->f;
     int rc = 0;
@@ -729,19 +746,21 @@ int setup(srx_ctx *c) {
         return rc;
     if (c->opt.warm_start == 1 || c->opt.warm_start == 2) {
         HIPCK(hipMalloc(&F.dsig, sizeof(float) * 6 * N));
-        HIPCK(hipMemset(F.dsig, 0, sizeof(float) * 6 * N));
+        if ((rc = ctx_zero(st, F.dsig, sizeof(float) * 6 * N))) return rc;
     }
     HIPCK(hipMalloc(&F.phase, sizeof(int) * N));
     HIPCK(hipMalloc(&F.nr_it, sizeof(int) * N));
-    HIPCK(hipMemset(F.nr_it, 0, sizeof(int) * N));
-    HIPCK(hipMemcpy(F.phase, phase.data(), sizeof(int) * N, hipMemcpyHostToDevice));
-    HIPCK(hipMemcpy(F.sig, sig.data(), sizeof(double) * 6 * N, hipMemcpyHostToDevice));
+    if ((rc = ctx_zero(st, F.nr_it, sizeof(int) * N)) ||
+        (rc = ctx_copy(st, F.phase, phase.data(), sizeof(int) * N, hipMemcpyHostToDevice)) ||
+        (rc = ctx_copy(st, F.sig, sig.data(), sizeof(double) * 6 * N, hipMemcpyHostToDevice)))
+        return rc;
     {
         std::vector<double> soa(9 * N);
         for (long v = 0; v < N; ++v)
             for (int k = 0; k < 9; ++k) soa[k * N + v] = tran[9 * v + k];
-        HIPCK(hipMemcpy(F.tran, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice));
-        HIPCK(hipMemcpy(F.tran0, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+        if ((rc = ctx_copy(st, F.tran, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice)) ||
+            (rc = ctx_copy(st, F.tran0, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice)))
+            return rc;
     }
     const size_t H = (size_t)(c->n[2] / 2 + 1);
     const size_t spec_elems = 9 * H * c->n[0] * c->n[1];
@@ -749,8 +768,8 @@ int setup(srx_ctx *c) {
     HIPCK(hipMalloc(&c->specB, spec_elems * csz));
     HIPCK(hipMalloc(&c->specC, spec_elems * csz));
     for (int ax = 0; ax < 3; ++ax) {
-        int r = c->opt.precision == SRX_PREC_MIXED ? upload_twiddles<float>(c->n[ax], &c->tw[ax])
-                                                   : upload_twiddles<double>(c->n[ax], &c->tw[ax]);
+        int r = c->opt.precision == SRX_PREC_MIXED ? upload_twiddles<float>(c->n[ax], &c->tw[ax], st)
+                                                   : upload_twiddles<double>(c->n[ax], &c->tw[ax], st);
         if (r) return r;
     }
     c->nr_blocks = nr_block_count((int)N);
@@ -885,6 +904,7 @@ int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_
         c->workdir = p == std::string::npos ? std::string(".") : d.substr(0, p);
     }
     c->timing = c->opt.reserved[0] == 2 ? 2 : (c->opt.reserved[0] != 0 ? 1 : 0);
+    c->graphs = c->opt.reserved[1] == 0;
     for (int k = 0; k < 8; ++k) c->kt[k] = {KT_NAMES[k], 0.0, 0};
     c->nkt = 8;
     int rc = read_deck(deck, c->dk);
@@ -1093,11 +1113,20 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     ctl.done = ctl_stop(ctl);
     HIPCK(hipMemcpyAsync(c->dctl, &ctl, sizeof(IterCtl), hipMemcpyHostToDevice, c->stream));
     const double divisor = (double)c->nvox;
-    const int batch = c->opt.verbose ? 1 : 8;
+    const int batch = c->opt.verbose ? 1 : ITER_BATCH;
     const int most = std::max(c->itermax, c->opt.fixed_iters);
+    // a batch is replayed from a HIP graph unless kernels are timed or samples recorded (those
+    // batches differ from launch to launch)
+    const bool use_graph = c->graphs && batch == ITER_BATCH && !c->timing && c->rec_stride == 0;
     int launched = 0;
     while (!ctl.done) {
         if (launched > most) return SRX_E_HIP;   // the control kernel must have stopped the loop
+        hipGraphExec_t *gx = use_graph ? &c->gexec[launched == 0 ? 1 : 0][rotated] : nullptr;
+        if (gx && *gx) {
+            HIPCK(hipGraphLaunch(*gx, c->stream));
+            launched += batch;
+        } else {
+        if (gx) HIPCK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
         for (int b = 0; b < batch; ++b) {
             ++launched;
             IterCtl *g = c->dctl;
@@ -1116,6 +1145,19 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
                                   c->rec_count, c->rec_cap, g);
             { Timer t(c, 7, launched); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red, g); }
             launch_control(c->stream, c->dc, c->red, g);
+        }
+        if (gx) {
+            hipGraph_t graph = nullptr;
+            HIPCK(hipStreamEndCapture(c->stream, &graph));
+            const hipError_t ie = hipGraphInstantiate(gx, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            if (ie != hipSuccess) {
+                *gx = nullptr;
+                std::fprintf(stderr, "srx: hipGraphInstantiate failed: %s\n", hipGetErrorString(ie));
+                return SRX_E_HIP;
+            }
+            HIPCK(hipGraphLaunch(*gx, c->stream));
+        }
         }
         HIPCK(hipMemcpyAsync(&ctl, c->dctl, sizeof(IterCtl), hipMemcpyDeviceToHost, c->stream));
         HIPCK(hipStreamSynchronize(c->stream));
@@ -1311,7 +1353,7 @@ int srx_get_field(srx_ctx *c, int id, double *out, size_t n) {
     if (n < (size_t)nc * N) return SRX_E_SIZE;
     HIPCK(hipStreamSynchronize(c->stream));
     auto fetch = [&](const void *src, size_t bytes, void *dst) -> int {
-        HIPCK(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+        if (int e = ctx_copy(c->stream, dst, src, bytes, hipMemcpyDeviceToHost)) return e;
         return 0;
     };
     std::vector<double> tmp;
@@ -1386,7 +1428,9 @@ int srx_set_field(srx_ctx *c, int id, const double *in, size_t n) {
             t9_to_b6(in + 9 * v, b6);
             for (int k = 0; k < m; ++k) tmp[k * N + v] = b6[k];
         }
-        HIPCK(hipMemcpy(id == SRX_F_SIG ? c->f.sig : c->f.eps, tmp.data(), sizeof(double) * m * N, hipMemcpyHostToDevice));
+        if (int e = ctx_copy(c->stream, id == SRX_F_SIG ? c->f.sig : c->f.eps, tmp.data(), sizeof(double) * m * N,
+                             hipMemcpyHostToDevice))
+            return e;
         if (id == SRX_F_SIG) {
             for (int k = 0; k < 6; ++k) {
                 double s = 0;
@@ -1400,7 +1444,9 @@ int srx_set_field(srx_ctx *c, int id, const double *in, size_t n) {
         tmp.resize((size_t)9 * N);
         for (long v = 0; v < N; ++v)
             for (int k = 0; k < 9; ++k) tmp[k * N + v] = in[9 * v + k];
-        HIPCK(hipMemcpy(id == SRX_F_DISGRAD ? c->f.dg : c->f.tran, tmp.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice));
+        if (int e = ctx_copy(c->stream, id == SRX_F_DISGRAD ? c->f.dg : c->f.tran, tmp.data(), sizeof(double) * 9 * N,
+                             hipMemcpyHostToDevice))
+            return e;
         return 0;
     }
     if (id == SRX_F_RHO_S || id == SRX_F_RHO_M || id == SRX_F_GAM_ACUM) {
@@ -1408,10 +1454,19 @@ int srx_set_field(srx_ctx *c, int id, const double *in, size_t n) {
         for (long v = 0; v < N; ++v)
             for (int k = 0; k < 12; ++k) tmp[k * N + v] = in[12 * v + k];
         double *dst = id == SRX_F_RHO_S ? c->f.rho_s : id == SRX_F_RHO_M ? c->f.rho_m : c->f.gam_acum;
-        HIPCK(hipMemcpy(dst, tmp.data(), sizeof(double) * 12 * N, hipMemcpyHostToDevice));
+        if (int e = ctx_copy(c->stream, dst, tmp.data(), sizeof(double) * 12 * N, hipMemcpyHostToDevice)) return e;
         return 0;
     }
     return SRX_E_FIELD;
+}
+
+static void drop_graphs(srx_ctx *c) {
+    for (auto &row : c->gexec)
+        for (auto &g : row)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
 }
 
 void srx_close(srx_ctx *c) {
@@ -1443,6 +1498,7 @@ void srx_close(srx_ctx *c) {
     if (c->dctl) hipFree(c->dctl);
     if (c->hctl) hipHostFree(c->hctl);
     for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+    drop_graphs(c);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -1461,6 +1517,7 @@ int srx_mlp_set(srx_ctx *c, const float *blob) {
     HIPCK(hipGetLastError());
     HIPCK(hipStreamSynchronize(c->stream));
     c->f.mlp = c->mlp;
+    drop_graphs(c);   // captured batches hold the previous kernel arguments
     return 0;
 }
 
@@ -1496,12 +1553,14 @@ int srx_mlp_samples(srx_ctx *c, float *x, float *t, size_t max, size_t *n) {
     if (!c->rec_count) return 0;
     int cnt = 0;
     HIPCK(hipStreamSynchronize(c->stream));
-    HIPCK(hipMemcpy(&cnt, c->rec_count, sizeof(int), hipMemcpyDeviceToHost));
+    if (int e = ctx_copy(c->stream, &cnt, c->rec_count, sizeof(int), hipMemcpyDeviceToHost)) return e;
     std::vector<float> hx((size_t)cnt * 24), ht((size_t)cnt * 6), hn(cnt);
     if (cnt) {
-        HIPCK(hipMemcpy(hx.data(), c->rec_x, hx.size() * sizeof(float), hipMemcpyDeviceToHost));
-        HIPCK(hipMemcpy(ht.data(), c->rec_t, ht.size() * sizeof(float), hipMemcpyDeviceToHost));
-        HIPCK(hipMemcpy(hn.data(), c->rec_nd, hn.size() * sizeof(float), hipMemcpyDeviceToHost));
+        int e = 0;
+        if ((e = ctx_copy(c->stream, hx.data(), c->rec_x, hx.size() * sizeof(float), hipMemcpyDeviceToHost)) ||
+            (e = ctx_copy(c->stream, ht.data(), c->rec_t, ht.size() * sizeof(float), hipMemcpyDeviceToHost)) ||
+            (e = ctx_copy(c->stream, hn.data(), c->rec_nd, hn.size() * sizeof(float), hipMemcpyDeviceToHost)))
+            return e;
     }
     size_t k = 0;
     for (int i = 0; i < cnt; ++i) {
