@@ -68,7 +68,8 @@ typedef struct {
                                 (same convergence test; no counterpart in the reference) */
     int    reserved[8];      /* [0]: kernel timing (srx_kernel_times); [1] = 1: enqueue the outer
                                 iterations launch by launch instead of replaying each batch of 8
-                                from a HIP graph captured on first use (the default) */
+                                from a HIP graph captured on first use (the default); [2] = 1: keep
+                                the default FFT tiles instead of timing the candidates at srx_open */
 } srx_options;
 
 /* Everything the reference prints or writes per increment, plus counters it
@@ -211,6 +212,10 @@ int srx_mlp_record(srx_ctx *ctx, int stride, size_t capacity);
 /* Copies up to max recorded samples with a non-zero previous change (x[.][24], t[.][6]; NULL
    pointers: count only) and returns their number in *n. */
 int srx_mlp_samples(srx_ctx *ctx, float *x, float *t, size_t max, size_t *n);
+
+/* FFT tiles chosen at srx_open (rows per block of the forward z, forward y, Green x, inverse y
+   and inverse z passes; 0: default) and their timed launch durations in microseconds. */
+int srx_fft_tiles(srx_ctx *ctx, int *tiles, double *us);
 
 /* Text for an error code. */
 const char *srx_strerror(int code);

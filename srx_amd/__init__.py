@@ -67,7 +67,7 @@ FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velg
 EXPORTS = ["srx_open", "srx_step", "srx_run", "srx_get_field", "srx_set_field", "srx_dims", "srx_close",
            "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse",
            "srx_run_ensemble", "srx_mlp_default_train_options", "srx_mlp_train", "srx_mlp_grad",
-           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples"]
+           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples", "srx_fft_tiles"]
 
 MLP_BLOB = 6210     # SRX_MLP_BLOB
 MLP_NPARAM = 6150   # SRX_MLP_NPARAM
@@ -121,6 +121,7 @@ def lib():
         L.srx_mlp_set.argtypes = [C.c_void_p, F]
         L.srx_mlp_record.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.srx_mlp_samples.argtypes = [C.c_void_p, F, F, C.c_size_t, P(C.c_size_t)]
+        L.srx_fft_tiles.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -204,6 +205,15 @@ class Solver:
         calls = (C.c_int * 16)()
         self.L.srx_kernel_times(self.h, names, ms, calls, C.byref(n))
         return {names[i].decode(): (ms[i], calls[i]) for i in range(n.value)}
+
+    def fft_tiles(self) -> tuple[list, list]:
+        """Rows per block chosen by the tile autotuner for the five FFT passes, and their times (us)."""
+        t = (C.c_int * 5)()
+        us = (C.c_double * 5)()
+        rc = self.L.srx_fft_tiles(self.h, t, us)
+        if rc:
+            raise SrxError(rc, "srx_fft_tiles")
+        return list(t), list(us)
 
     def mlp_set(self, blob: np.ndarray) -> None:
         """Install the learned Newton start (context opened with warm_start=2)."""

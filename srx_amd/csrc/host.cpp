@@ -394,6 +394,9 @@ struct srx_ctx {
     // outer-iteration batches captured as HIP graphs: [first batch of an increment][rotated]
     hipGraphExec_t gexec[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
     bool graphs = true;
+    // rows per block of the five FFT passes (0: the default) and their tuned times
+    int tiles[5] = {0, 0, 0, 0, 0};
+    double tile_us[5] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -893,6 +896,72 @@ const char *srx_strerror(int code) {
     }
 }
 
+// FFT tile autotuner: each pass runs on the context's own buffers (the inverse z pass into a
+// scratch DisGrad) with every candidate tile (powers of two up to the largest valid one, and that
+// one), three timed launches after a warm-up, and keeps the fastest.
+static int autotune_tiles(srx_ctx *c) {
+    const int prec = c->opt.precision;
+    hipStream_t st = c->stream;
+    double *scratch = nullptr;
+    HIPCK(hipMalloc(&scratch, sizeof(double) * 9 * (size_t)c->nvox));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SRX_E_HIP;
+    for (int pass = 0; pass < 5 && !rc; ++pass) {
+        const FftPlan &p = c->pl[pass == 2 ? 0 : (pass == 1 || pass == 3) ? 1 : 2];
+        const int tmax = fft_tile_max(pass, c->n, p, prec);
+        std::vector<int> cand;
+        for (int t = 1; t < tmax; t *= 2) cand.push_back(t);
+        cand.push_back(tmax);
+        double best = 1e300;
+        for (int t : cand) {
+            auto run = [&]() {
+                switch (pass) {
+                case 0: launch_fft_fwd_z(st, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], nullptr, t); break;
+                case 1: launch_fft_y(st, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], nullptr, t); break;
+                case 2: launch_green_x(st, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, nullptr, t); break;
+                case 3: launch_fft_y(st, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], nullptr, t); break;
+                default:
+                    launch_fft_inv_z_update(st, prec, c->specB, scratch, c->n, c->pl[2], c->tw[2], nullptr,
+                                            (double)c->nvox, t);
+                }
+            };
+            run();
+            float ms = 0.0f;
+            if (hipEventRecord(e0, st) != hipSuccess) { rc = SRX_E_HIP; break; }
+            for (int r = 0; r < 3; ++r) run();
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess) {
+                rc = SRX_E_HIP;
+                break;
+            }
+            const double us = 1000.0 * ms / 3.0;
+            if (us < best) {
+                best = us;
+                c->tiles[pass] = t;
+                c->tile_us[pass] = us;
+            }
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(scratch);
+    if (c->opt.verbose && !rc)
+        std::printf("fft tiles (fwd z, fwd y, green x, inv y, inv z): %d %d %d %d %d  (%.1f %.1f %.1f %.1f %.1f us)\n",
+                    c->tiles[0], c->tiles[1], c->tiles[2], c->tiles[3], c->tiles[4], c->tile_us[0], c->tile_us[1],
+                    c->tile_us[2], c->tile_us[3], c->tile_us[4]);
+    return rc;
+}
+
+int srx_fft_tiles(srx_ctx *c, int *tiles, double *us) {
+    if (!c || !tiles) return SRX_E_ARG;
+    for (int k = 0; k < 5; ++k) {
+        tiles[k] = c->tiles[k];
+        if (us) us[k] = c->tile_us[k];
+    }
+    return 0;
+}
+
 int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_ctx **out) {
     if (!deck || !out) return SRX_E_ARG;
     srx_ctx *c = new srx_ctx();
@@ -909,6 +978,7 @@ int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_
     c->nkt = 8;
     int rc = read_deck(deck, c->dk);
     if (!rc) rc = setup(c);
+    if (!rc && c->opt.reserved[2] == 0) rc = autotune_tiles(c);
     if (rc) {
         srx_close(c);
         return rc;
@@ -1130,11 +1200,12 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
         for (int b = 0; b < batch; ++b) {
             ++launched;
             IterCtl *g = c->dctl;
-            { Timer t(c, 0, launched); launch_fft_fwd_z(c->stream, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], g); }
-            { Timer t(c, 1, launched); launch_fft_y(c->stream, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], g); }
-            { Timer t(c, 2, launched); launch_green_x(c->stream, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, g); }
-            { Timer t(c, 3, launched); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], g); }
-            { Timer t(c, 4, launched); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], g, divisor); }
+            const int *tl = c->tiles;
+            { Timer t(c, 0, launched); launch_fft_fwd_z(c->stream, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], g, tl[0]); }
+            { Timer t(c, 1, launched); launch_fft_y(c->stream, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], g, tl[1]); }
+            { Timer t(c, 2, launched); launch_green_x(c->stream, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, g, tl[2]); }
+            { Timer t(c, 3, launched); launch_fft_y(c->stream, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], g, tl[3]); }
+            { Timer t(c, 4, launched); launch_fft_inv_z_update(c->stream, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], g, divisor, tl[4]); }
             const bool rec = c->rec_stride > 0 && launched > 1;
             if (rec)
                 launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 0, c->rec_x, c->rec_t, c->rec_nd,

@@ -1856,9 +1856,32 @@ static void check_launch(const char *name) {
     if (e != hipSuccess) fprintf(stderr, "srx: launch of %s failed: %s\n", name, hipGetErrorString(e));
 }
 
+// Rows (pencils) per block of a z or y pass: `tile` if it is valid, else the largest that fits.
+static int zy_tile(const FftPlan &p, int rows, int tile, int prec) {
+    const int t = tile_for(p, 1, 2, rows < 16 ? rows : 16, prec);
+    return tile >= 1 && tile <= t ? tile : t;
+}
+
+// Largest valid tile of pass `pass` (0 forward z, 1 forward y, 2 Green x, 3 inverse y, 4 inverse
+// z); p is the plan of the pass's axis.
+int fft_tile_max(int pass, const int n[3], const FftPlan &p, int prec) {
+    switch (pass) {
+    case 0: case 4: return zy_tile(p, n[1], 0, prec);
+    case 1: case 3: return zy_tile(p, n[0], 0, prec);
+    default: {   // Green x pass: 9 slots of t rows, up to 4 butterflies per thread
+        int t = (4 * FFT_BLOCK * min_radix(p)) / (9 * p.n);
+        const int csz = prec == 0 ? 16 : 8;
+        const int tl = (FFT_LDS_MAX / csz - p.n) / (9 * p.n);
+        t = t < tl ? t : tl;
+        t = t < n[1] ? t : n[1];
+        return t < 1 ? 1 : (t > 4 ? 4 : t);
+    }
+    }
+}
+
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
-                      const void *twz, const IterCtl *ctl) {
-    const int TY = tile_for(pz, 1, 2, n[1] < 16 ? n[1] : 16, prec);
+                      const void *twz, const IterCtl *ctl, int tile) {
+    const int TY = zy_tile(pz, n[1], tile, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
@@ -1873,9 +1896,9 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
 }
 
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c, const int n[3],
-                  const FftPlan &py, const void *twy, const IterCtl *ctl) {
+                  const FftPlan &py, const void *twy, const IterCtl *ctl, int tile) {
     (void)sign;
-    const int TX = tile_for(py, 1, 2, n[0] < 16 ? n[0] : 16, prec);
+    const int TX = zy_tile(py, n[0], tile, prec);
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
@@ -1892,12 +1915,18 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
 
 static void launch_green_generic(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px,
                                  const void *twx, const DevConst *cst, const double *dc6, int mode, int ncomp,
-                                 const IterCtl *ctl) {
+                                 const IterCtl *ctl, int tile = 0) {
     const int slots = mode == 1 ? 9 : ncomp;
-    // MAXB 6 covers 9 slots of two 128-point pencils with a radix-2 stage
-    // two butterflies per thread when one row of `slots` pencils allows it, four otherwise
-    const bool wide = (long)slots * n[0] / min_radix(px) > 2 * FFT_BLOCK;
-    const int TY = tile_for(px, slots, wide ? 4 : 2, n[1] < 2 ? n[1] : 2, prec);
+    int TY;
+    bool wide;
+    if (mode == 1 && tile >= 1 && tile <= fft_tile_max(2, n, px, prec)) {
+        TY = tile;
+        wide = (long)slots * TY * n[0] / min_radix(px) > 2 * FFT_BLOCK;
+    } else {
+        // two butterflies per thread when one row of `slots` pencils allows it, four otherwise
+        wide = (long)slots * n[0] / min_radix(px) > 2 * FFT_BLOCK;
+        TY = tile_for(px, slots, wide ? 4 : 2, n[1] < 2 ? n[1] : 2, prec);
+    }
     const int H = n[2] / 2 + 1;
     dim3 grid((n[1] + TY - 1) / TY, H);
     const double z6[6] = {0, 0, 0, 0, 0, 0};
@@ -1923,13 +1952,13 @@ static void launch_green_generic(hipStream_t s, int prec, void *specC, const int
 }
 
 void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px, const void *twx,
-                    const DevConst *cst, const IterCtl *ctl) {
-    launch_green_generic(s, prec, specC, n, px, twx, cst, nullptr, 1, 9, ctl);
+                    const DevConst *cst, const IterCtl *ctl, int tile) {
+    launch_green_generic(s, prec, specC, n, px, twx, cst, nullptr, 1, 9, ctl, tile);
 }
 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
-                             const void *twz, const IterCtl *ctl, double divisor) {
-    const int TY = tile_for(pz, 1, 2, n[1] < 16 ? n[1] : 16, prec);
+                             const void *twz, const IterCtl *ctl, double divisor, int tile) {
+    const int TY = zy_tile(pz, n[1], tile, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY);
     const double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // dDisGradAvg comes from ctl
     if (prec == 0) {

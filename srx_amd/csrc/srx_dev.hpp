@@ -197,14 +197,17 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 // Spectrum buffers: specB [c][kz][x][y], specC [c][kz][y][x]; T = double or float.
 // With a non-null `ctl` the iteration's scalars (mean stress, k = 0 sums, dDisGradAvg) come from
 // the device control block and the kernels return at once when its `done` is set.
+// `tile`: rows (pencils) per block, 0 for the default; fft_tile_max gives the largest valid one of
+// a pass (0 forward z, 1 forward y, 2 Green x, 3 inverse y, 4 inverse z; p: the pass axis's plan).
+int fft_tile_max(int pass, const int n[3], const FftPlan &p, int prec);
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3],
-                      const FftPlan &pz, const void *twz, const IterCtl *ctl);
+                      const FftPlan &pz, const void *twz, const IterCtl *ctl, int tile = 0);
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c,
-                  const int n[3], const FftPlan &py, const void *twy, const IterCtl *ctl = nullptr);
+                  const int n[3], const FftPlan &py, const void *twy, const IterCtl *ctl = nullptr, int tile = 0);
 void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px, const void *twx,
-                    const DevConst *cst, const IterCtl *ctl);
+                    const DevConst *cst, const IterCtl *ctl, int tile = 0);
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3],
-                             const FftPlan &pz, const void *twz, const IterCtl *ctl, double scale);
+                             const FftPlan &pz, const void *twz, const IterCtl *ctl, double scale, int tile = 0);
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
                double *partials, const IterCtl *ctl = nullptr);
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out,

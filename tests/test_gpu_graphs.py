@@ -50,3 +50,21 @@ def test_graphs_rebuilt_after_new_weights(tmp_path):
         assert ra["sig_avg"] == rb["sig_avg"]
     a.close()
     b.close()
+
+
+@pytest.mark.parametrize("precision", [srx_amd.PREC_FP64, srx_amd.PREC_MIXED])
+def test_tuned_fft_tiles_are_valid_and_bit_identical(tmp_path, precision):
+    """The tile autotuner only changes how many pencils a block transforms: each pencil's
+    arithmetic is the same, so results are bit-identical to the default tiles."""
+    deck = inputs.make_case(str(tmp_path), n=(32, 24, 20), grains=6, seed=33, nsteps=2, timestep=0.1)
+    a = srx_amd.Solver(deck, precision=precision)
+    b = srx_amd.Solver(deck, precision=precision, reserved=[0, 0, 1])
+    tiles, us = a.fft_tiles()
+    assert all(t >= 1 for t in tiles) and all(u > 0 for u in us)
+    assert b.fft_tiles()[0] == [0, 0, 0, 0, 0]
+    for _ in range(2):
+        ra, rb = a.step(), b.step()
+        assert ra["sig_avg"] == rb["sig_avg"] and ra["nr_iters_total"] == rb["nr_iters_total"]
+    np.testing.assert_array_equal(a.field("disgrad"), b.field("disgrad"))
+    a.close()
+    b.close()
