@@ -1525,6 +1525,10 @@ __global__ void k_prologue(Fields f, int nvox, Vec9 add, int creep, Vec9 set, Ve
 // and the partial sums of AvgDislocation and the Eps/Edot averages.
 // ---------------------------------------------------------------------------
 constexpr int END_BLOCK = 128;
+// slip systems per trip of the end-of-increment kernel's slip-rate loop
+#ifndef SRX_END_UNROLL
+#define SRX_END_UNROLL 1
+#endif
 
 template <int PH>
 __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
@@ -1534,8 +1538,6 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
     double a[9];
 #pragma unroll
     for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
-    double Q[5][5];
-    q5_from(a, Q);
     const bool lagged = lagged_possible && f.nr_it[v] == 1;
     double rs[NSYS];
 #pragma unroll
@@ -1546,6 +1548,8 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
     // final stress in the crystal frame (Schmid tensors of this increment)
     double sf[5];
     {
+        double Q[5][5];
+        q5_from(a, Q);
         double s6[5];
 #pragma unroll
         for (int k = 0; k < 5; ++k) s6[k] = f.sig[k * N + v];
@@ -1557,19 +1561,20 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             sf[i] = acc;
         }
     }
-    double gam[NSYS], edc[5] = {0, 0, 0, 0, 0};
-    double rFs[NSYS];
+    double edc[5] = {0, 0, 0, 0, 0};
     double sumF = 0, sumP = 0, sumM = 0, rho_tot = 0.0;
+    // plastic spin sum_s b_s (x) n_s gamdot_s of update_orient (constitutive.c:215-250), with the
+    // pre-update TranMat, accumulated with the slip rates so they need not stay in registers
+    double lp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+    bool bad = false;
     const int nsys = P.nsys;
     // rho_tot of evolution.c:1040-1046, from the densities at the start of the increment
 #pragma unroll
     for (int s = 0; s < NSYS; ++s)
         if (s < nsys) rho_tot += rs[s] + f.rho_m_old[s * N + v] + sqrt(0.0);
     f.rho_tot[v] = rho_tot;
-#pragma unroll
+#pragma unroll SRX_END_UNROLL
     for (int s = 0; s < NSYS; ++s) {
-        gam[s] = 0.0;
-        rFs[s] = 0.0;
         if (s < nsys) {
             double rP, rF;
             rho_pf(P, rs, s, rP, rF);
@@ -1581,10 +1586,18 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             const double rss = p[0] * pre[0] + p[1] * pre[1] + p[2] * pre[2] + p[3] * pre[3] + p[4] * pre[4];
             double g, g2;
             slip_rate(rss, tp, inv, g0, g, g2);
-            gam[s] = g;
+            bad |= !isfinite(g);
 #pragma unroll
             for (int i = 0; i < 5; ++i) edc[i] += p[i] * g;
-            rFs[s] = rF;
+            if (do_orient) {
+                double bs[3], ns[3];
+                for (int i = 0; i < 3; ++i) {
+                    bs[i] = a[3 * i] * P.bb[s][0] + a[3 * i + 1] * P.bb[s][1] + a[3 * i + 2] * P.bb[s][2];
+                    ns[i] = a[3 * i] * P.bn[s][0] + a[3 * i + 1] * P.bn[s][1] + a[3 * i + 2] * P.bn[s][2];
+                }
+                for (int i = 0; i < 3; ++i)
+                    for (int j = 0; j < 3; ++j) lp[i][j] += bs[i] * ns[j] * g;
+            }
             f.rho_F[s * N + v] = rF;
             f.rho_P[s * N + v] = rP;
             f.gamdot[s * N + v] = g;
@@ -1593,8 +1606,11 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             sumM += rmn;
         }
     }
-    // Edot in the sample frame and Eps += Edot dt
+    // Edot in the sample frame and Eps += Edot dt (the rotation is rebuilt rather than kept in
+    // registers through the slip-rate loop)
     double eds[5];
+    double Q[5][5];
+    q5_from(a, Q);
 #pragma unroll
     for (int i = 0; i < 5; ++i) {
         double acc = 0.0;
@@ -1612,23 +1628,12 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
     (void)eds;
     // reorientation (update_orient), with the pre-update TranMat for b and n
     if (do_orient) {
-        double rl[3][3], lp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        double rl[3][3];
         double vg[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) vg[k] = (f.dg[k * N + v] - f.dg0[k * N + v]) / dt;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) rl[i][j] = (vg[3 * i + j] - vg[3 * j + i]) / 2.;
-#pragma unroll
-        for (int s = 0; s < NSYS; ++s) {
-            if (s >= nsys) continue;
-            double bs[3], ns[3];
-            for (int i = 0; i < 3; ++i) {
-                bs[i] = a[3 * i] * P.bb[s][0] + a[3 * i + 1] * P.bb[s][1] + a[3 * i + 2] * P.bb[s][2];
-                ns[i] = a[3 * i] * P.bn[s][0] + a[3 * i + 1] * P.bn[s][1] + a[3 * i + 2] * P.bn[s][2];
-            }
-            for (int i = 0; i < 3; ++i)
-                for (int j = 0; j < 3; ++j) lp[i][j] += bs[i] * ns[j] * gam[s];
-        }
         double rot[3][3], rsl[3][3];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) rsl[i][j] = (lp[i][j] - lp[j][i]) / 2.;
@@ -1653,27 +1658,29 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             for (int j = 0; j < 3; ++j)
                 f.tran[(3 * i + j) * N + v] = rot[i][0] * a[j] + rot[i][1] * a[3 + j] + rot[i][2] * a[6 + j];
     }
-    // DislocationEvolution, istep <= 2e6 branch
+    // DislocationEvolution, istep <= 2e6 branch; the slip rates and rho_F of this increment are
+    // read back from the arrays written above (one system at a time keeps the kernel's registers
+    // within two wavefronts per SIMD)
     const double G = P.G, nu = cst->nu, T = cst->T, D0 = cst->D0;
     double sR = 0, sD = 0, sFo = 0, sI = 0;
-#pragma unroll
-    for (int s = 0; s < NSYS; ++s) {
-        if (s >= nsys) continue;
+#pragma unroll 1
+    for (int s = 0; s < nsys; ++s) {
         const double *p = P.p[s];
         const double rss = p[0] * sf[0] + p[1] * sf[1] + p[2] * sf[2] + p[3] * sf[3] + p[4] * sf[4];
         const double bb = P.b;
         const double d_dip = sqrt(3.0) * G * bb / 16.0 / 3.14159265359 / (1 - nu) / (fabs(rss) + 1.0E-4);
         const double hydro = G * bb / 3.14159265359 / (1 - nu) / d_dip;
-        double r = rs[s];
+        double r = f.rho_s[s * N + v];
         double rdip = f.rho_dipole[s * N + v], rfor = f.rho_forest[s * N + v], rinp = f.rho_inplane[s * N + v];
-        const double gd = gam[s];
+        const double gd = f.gamdot[s * N + v];
+        const double rFs = f.rho_F[s * N + v];
         if (istep <= 2000000) {
             const double climb = 1E-17 * rho_tot * 0.5 * 1E24 * (d_dip) * (D0 / bb) * pow(r, 2.0) *
                                  (exp(fabs(hydro) * bb * bb * bb * 100 / (1.38 * T)) - 1.0) * P.expmqb;
             rdip += climb * dt;
-            rfor += fabs(gd) * (P.c4 * sqrt(rFs[s])) * dt;
+            rfor += fabs(gd) * (P.c4 * sqrt(rFs)) * dt;
             rinp += fabs(gd) * (P.c5 * r) * dt;
-            const double dot = fabs(gd) * (P.c4 * sqrt(rFs[s]) + P.c6 * (d_dip) * f.rho_m[s * N + v] - P.c5 * r) - climb;
+            const double dot = fabs(gd) * (P.c4 * sqrt(rFs) + P.c6 * (d_dip) * f.rho_m[s * N + v] - P.c5 * r) - climb;
             r += dot * dt;
         }
         f.rho_s[s * N + v] = r;
@@ -1693,13 +1700,13 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
     vals[16] = sD;
     vals[17] = sFo;
     vals[20] = sI;
-    bool bad = false;
-#pragma unroll
-    for (int s = 0; s < NSYS; ++s) bad |= !isfinite(gam[s]);
     vals[21] = bad ? 1.0 : 0.0;
 }
 
-__global__ void __launch_bounds__(END_BLOCK) k_end(const DevConst *__restrict__ cst, Fields f, int nvox,
+#ifndef SRX_END_WAVES_PER_EU
+#define SRX_END_WAVES_PER_EU 2
+#endif
+__global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(SRX_END_WAVES_PER_EU, 8))) k_end(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                    int lagged_possible, int do_orient, int istep,
                                                    double *__restrict__ partials) {
     const int v = blockIdx.x * END_BLOCK + threadIdx.x;
