@@ -214,7 +214,9 @@ int srx_mlp_record(srx_ctx *ctx, int stride, size_t capacity);
 int srx_mlp_samples(srx_ctx *ctx, float *x, float *t, size_t max, size_t *n);
 
 /* FFT tiles chosen at srx_open (rows per block of the forward z, forward y, Green x, inverse y
-   and inverse z passes; 0: default) and their timed launch durations in microseconds. */
+   and inverse z passes; 0: default; -1: the Green x pass runs one wavefront per row, possible for
+   n0 = 64, 128 and 256, where it is also the default) and their timed launch durations in
+   microseconds. */
 int srx_fft_tiles(srx_ctx *ctx, int *tiles, double *us);
 
 /* Text for an error code. */

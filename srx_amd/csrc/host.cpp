@@ -911,6 +911,7 @@ static int autotune_tiles(srx_ctx *c) {
         const FftPlan &p = c->pl[pass == 2 ? 0 : (pass == 1 || pass == 3) ? 1 : 2];
         const int tmax = fft_tile_max(pass, c->n, p, prec);
         std::vector<int> cand;
+        if (pass == 2 && green_wave_ok(c->n)) cand.push_back(-1);   // one wavefront per row
         for (int t = 1; t < tmax; t *= 2) cand.push_back(t);
         cand.push_back(tmax);
         double best = 1e300;

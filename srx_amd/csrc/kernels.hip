@@ -519,6 +519,82 @@ __device__ __forceinline__ void green_w6(const double *__restrict__ G6, const do
     wi[2] = (c02 * t0i + c12 * t1i + c22 * t2i) * rdet;
 }
 
+// Green operator at one k point (kx, ky, kz): B-basis spectrum q[6] of the stress -> du[9]
+// (row-major).  k = 0: -S0 (sig + dc6) for ElastBC 0, else 0 (init.c:558-569); dc6 is the grid sum
+// of sig that a mean-subtracted (mixed precision) spectrum lacks, B basis.
+template <typename T>
+__device__ __forceinline__ void green_point(const DevConst *__restrict__ cst, const double dc6[6], int kx, int ky,
+                                            int kz, int n0, int n1, int n2, const cx<T> q[6], cx<T> out[9]) {
+    double tzv, tyv, txv;
+    unit_k(kz, n2, tzv);
+    unit_k(ky, n1, tyv);
+    unit_k(kx, n0, txv);
+    const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
+    const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
+    const bool nyy = (n1 % 2 == 0) && (ky == n1 / 2);
+    double sr[6], si[6];
+    {
+        double vr[6], vi[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            vr[c] = (double)q[c].x;
+            vi[c] = (double)q[c].y;
+        }
+        b6_to_sym(vr, sr);
+        b6_to_sym(vi, si);
+    }
+    if (kx == 0 && ky == 0 && kz == 0) {
+        double dr[9], di[9];
+        if (cst->elast_bc == 0) {
+            double dcs[6], t9r[9], t9i[9];
+            b6_to_sym(dc6, dcs);
+            sym_to_t9(sr, t9r);
+            sym_to_t9(si, t9i);
+            double d9[9];
+            sym_to_t9(dcs, d9);
+            for (int qq = 0; qq < 9; ++qq) {
+                double accr = 0.0, acci = 0.0;
+                for (int r = 0; r < 9; ++r) {
+                    accr += -cst->S0[qq * 9 + r] * (t9r[r] + d9[r]);
+                    acci += -cst->S0[qq * 9 + r] * t9i[r];
+                }
+                dr[qq] = accr;
+                di[qq] = acci;
+            }
+        } else {
+#pragma unroll
+            for (int qq = 0; qq < 9; ++qq) dr[qq] = di[qq] = 0.0;
+        }
+#pragma unroll
+        for (int qq = 0; qq < 9; ++qq) out[qq] = cx<T>{T(dr[qq]), T(di[qq])};
+        return;
+    }
+    double nv[3] = {txv, tyv, tzv};
+    const double rg = 1. / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
+    double wr[3], wi[3];
+    green_w6(cst->G6, nv, sr, si, wr, wi);
+    if (!(nyx || nyy || nyz)) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) out[3 * a + b] = cx<T>{T(-wr[a] * nv[b]), T(-wi[a] * nv[b])};
+    } else {
+        // Taking Re() of the inverse transform (evolution.c:1197) keeps the Hermitian part of du; at
+        // a Nyquist index that part averages Gamma(n) with Gamma(n') where n' has the Nyquist
+        // components negated.
+        const double np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
+        double wr2[3], wi2[3];
+        green_w6(cst->G6, np, sr, si, wr2, wi2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b)
+                out[3 * a + b] = cx<T>{T(0.5 * (-wr[a] * nv[b] - wr2[a] * np[b])),
+                                       T(0.5 * (-wi[a] * nv[b] - wi2[a] * np[b]))};
+    }
+}
+
 // x pass with the Green operator.  grid: (ceil(n1/TY), H).  LDS: 9 slots of TY*n0.
 // mode 1: forward x of 6 comps, Green, inverse x of 9 comps (the solver pass)
 // mode 0: forward x only (ncomp comps, testing)   mode 2: inverse x only (testing)
@@ -549,84 +625,17 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
     __syncthreads();
     if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, stw, -1, n0);
     if (mode == 1) {
-        double tzv, tyv;
-        unit_k(kz, n2, tzv);
-        const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
+        double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
+        if (ctl)
+            for (int k = 0; k < 6; ++k) dc6[k] = ctl->dcs6[k];
         for (int i = threadIdx.x; i < slot; i += FFT_BLOCK) {
             const int yy = dn0.div(i), kx = i - yy * n0, ky = y0 + yy;
-            unit_k(ky, n1, tyv);
-            double txv;
-            unit_k(kx, n0, txv);
-            const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
-            const bool nyy = (n1 % 2 == 0) && (ky == n1 / 2);
-            double sr[6], si[6];
-            {
-                double vr[6], vi[6];
+            cx<T> q[6], o[9];
 #pragma unroll
-                for (int c = 0; c < 6; ++c) {
-                    const cx<T> q = buf[c * slot + i];
-                    vr[c] = (double)q.x;
-                    vi[c] = (double)q.y;
-                }
-                b6_to_sym(vr, sr);
-                b6_to_sym(vi, si);
-            }
-            if (kx == 0 && ky == 0 && kz == 0) {
-                // k = 0: Gamma = -S0 (ElastBC 0) or 0 (init.c:558-569).  dc*: the grid sum of sig
-                // that a mean-subtracted (mixed precision) spectrum lacks, B basis.
-                double dr[9], di[9];
-                if (cst->elast_bc == 0) {
-                    double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
-                    if (ctl)
-                        for (int k = 0; k < 6; ++k) dc6[k] = ctl->dcs6[k];
-                    double dcs[6], t9r[9], t9i[9];
-                    b6_to_sym(dc6, dcs);
-                    sym_to_t9(sr, t9r);
-                    sym_to_t9(si, t9i);
-                    double d9[9];
-                    sym_to_t9(dcs, d9);
-                    for (int q = 0; q < 9; ++q) {
-                        double accr = 0.0, acci = 0.0;
-                        for (int r = 0; r < 9; ++r) {
-                            accr += -cst->S0[q * 9 + r] * (t9r[r] + d9[r]);
-                            acci += -cst->S0[q * 9 + r] * t9i[r];
-                        }
-                        dr[q] = accr;
-                        di[q] = acci;
-                    }
-                } else {
+            for (int c = 0; c < 6; ++c) q[c] = buf[c * slot + i];
+            green_point(cst, dc6, kx, ky, kz, n0, n1, n2, q, o);
 #pragma unroll
-                    for (int q = 0; q < 9; ++q) dr[q] = di[q] = 0.0;
-                }
-#pragma unroll
-                for (int q = 0; q < 9; ++q) buf[q * slot + i] = cx<T>{T(dr[q]), T(di[q])};
-            } else {
-                double nv[3] = {txv, tyv, tzv};
-                const double rg = 1. / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
-                nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
-                double wr[3], wi[3];
-                green_w6(cst->G6, nv, sr, si, wr, wi);
-                if (!(nyx || nyy || nyz)) {
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-#pragma unroll
-                        for (int b = 0; b < 3; ++b)
-                            buf[(3 * a + b) * slot + i] = cx<T>{T(-wr[a] * nv[b]), T(-wi[a] * nv[b])};
-                } else {
-                    // Taking Re() of the inverse transform (evolution.c:1197) keeps the Hermitian part
-                    // of du; at a Nyquist index that part averages Gamma(n) with Gamma(n') where n'
-                    // has the Nyquist components negated.
-                    const double np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
-                    double wr2[3], wi2[3];
-                    green_w6(cst->G6, np, sr, si, wr2, wi2);
-#pragma unroll
-                    for (int a = 0; a < 3; ++a)
-#pragma unroll
-                        for (int b = 0; b < 3; ++b)
-                            buf[(3 * a + b) * slot + i] = cx<T>{T(0.5 * (-wr[a] * nv[b] - wr2[a] * np[b])),
-                                                                T(0.5 * (-wi[a] * nv[b] - wi2[a] * np[b]))};
-                }
-            }
+            for (int c = 0; c < 9; ++c) buf[c * slot + i] = o[c];
         }
         __syncthreads();
     }
@@ -635,6 +644,160 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
         const int c = dslot.div(i), r = i - c * slot, yy = dn0.div(r), x = r - yy * n0;
         spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x] = buf[c * slot + yy * n0 + x];
     }
+}
+
+// ---------------------------------------------------------------------------
+// The solver's x pass with one wavefront per (ky, kz) row and no LDS or barriers, for
+// n0 = 64 NPL.  Lane l holds points l + 64 r (r < NPL) of each component.  Forward transform as
+// four steps: an in-lane NPL-point DFT, the twiddle w_N^(l k1), then a 64-point radix-2
+// decimation-in-frequency transform across lanes (partners l ^ h, h = 32 .. 1), which leaves
+// X[k1 + NPL bitrev6(l)] in lane l, register k1.  The Green operator is applied in that order and
+// the inverse runs the conjugate-transposed stages in reverse (decimation in time), which returns
+// natural order: an unnormalised inverse DFT, as lds_fft with sign +1.
+// grid: (ceil(n1 / 4), H), 256 threads (four rows per block).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int brev6(int l) { return (int)(__brev((unsigned)l) >> 26); }
+
+// value of lane ^ H: DPP quad permutations for H = 1, 2 (no LDS traffic), ds_swizzle's xor mode
+// within 32 lanes for H = 4, 8, 16, ds_bpermute for H = 32
+template <int H>
+__device__ __forceinline__ int xor_lane(int v) {
+    if constexpr (H == 1) return __builtin_amdgcn_mov_dpp(v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
+    else if constexpr (H == 2) return __builtin_amdgcn_mov_dpp(v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
+    else if constexpr (H < 32) return __builtin_amdgcn_ds_swizzle(v, (H << 10) | 0x1F);
+    else return __shfl_xor(v, 32, 64);
+}
+template <int H>
+__device__ __forceinline__ float xor_lane(float v) { return __int_as_float(xor_lane<H>(__float_as_int(v))); }
+template <int H>
+__device__ __forceinline__ double xor_lane(double v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = xor_lane<H>((int)(b & 0xffffffffll)), hi = xor_lane<H>((int)(b >> 32));
+    return __longlong_as_double(((long long)hi << 32) | (unsigned)lo);
+}
+template <int H, typename T>
+__device__ __forceinline__ cx<T> shfl_xor_cx(cx<T> v) {
+    return cx<T>{xor_lane<H>(v.x), xor_lane<H>(v.y)};
+}
+template <typename T>
+__device__ __forceinline__ cx<T> conj_cx(cx<T> a) { return cx<T>{a.x, -a.y}; }
+
+// Cross-lane stage ST (partner lane ^ (32 >> ST)) and the ones after it, for C components at once:
+// all partner values of a stage are requested before any is used, so one wait covers them.
+template <typename T, int NPL, int C, int ST>
+__device__ __forceinline__ void wave_dif_stage(cx<T> (&v)[C][NPL], const cx<T> ws[6], int lane) {
+    if constexpr (ST < 6) {
+        constexpr int h = 32 >> ST;
+        const bool hi = lane & h;
+        cx<T> u[C][NPL];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) u[c][k] = shfl_xor_cx<h>(v[c][k]);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) v[c][k] = hi ? cmul(u[c][k] - v[c][k], ws[ST]) : v[c][k] + u[c][k];
+        wave_dif_stage<T, NPL, C, ST + 1>(v, ws, lane);
+    }
+}
+template <typename T, int NPL, int C, int ST>
+__device__ __forceinline__ void wave_dit_stage(cx<T> (&v)[C][NPL], const cx<T> ws[6], int lane) {
+    if constexpr (ST >= 0) {
+        constexpr int h = 32 >> ST;
+        const bool hi = lane & h;
+        const cx<T> w = conj_cx(ws[ST]);
+        cx<T> u[C][NPL];
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < NPL; ++k) u[c][k] = shfl_xor_cx<h>(v[c][k]);
+#pragma unroll
+        for (int c = 0; c < C; ++c)
+#pragma unroll
+            for (int k = 0; k < NPL; ++k)
+                v[c][k] = hi ? u[c][k] - cmul(v[c][k], w) : v[c][k] + cmul(u[c][k], w);
+        wave_dit_stage<T, NPL, C, ST - 1>(v, ws, lane);
+    }
+}
+
+template <typename T, int NPL, int C>
+__device__ __forceinline__ void wave_fft_fwd(cx<T> (&v)[C][NPL], const cx<T> wl[], const cx<T> ws[6], int lane) {
+    if constexpr (NPL > 1) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+            Dft<T, NPL>::run(v[c], -1);
+#pragma unroll
+            for (int k = 1; k < NPL; ++k) v[c][k] = cmul(v[c][k], wl[k - 1]);
+        }
+    }
+    wave_dif_stage<T, NPL, C, 0>(v, ws, lane);
+}
+
+template <typename T, int NPL, int C>
+__device__ __forceinline__ void wave_fft_inv(cx<T> (&v)[C][NPL], const cx<T> wl[], const cx<T> ws[6], int lane) {
+    wave_dit_stage<T, NPL, C, 5>(v, ws, lane);
+    if constexpr (NPL > 1) {
+#pragma unroll
+        for (int c = 0; c < C; ++c) {
+#pragma unroll
+            for (int k = 1; k < NPL; ++k) v[c][k] = cmul(v[c][k], conj_cx(wl[k - 1]));
+            Dft<T, NPL>::run(v[c], +1);
+        }
+    }
+}
+
+template <typename T, int NPL>
+__global__ void __launch_bounds__(256) k_green_wave(cx<T> *__restrict__ spec, int n1, int n2,
+                                                    const cx<T> *__restrict__ tw, const DevConst *__restrict__ cst,
+                                                    double dc0, double dc1, double dc2, double dc3, double dc4,
+                                                    double dc5, const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    constexpr int N = 64 * NPL;
+    const int lane = threadIdx.x & 63;
+    const int ky = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int kz = blockIdx.y;
+    if (ky >= n1) return;   // the whole wavefront: no barrier follows
+    const int H = n2 / 2 + 1;
+    const size_t plane = (size_t)N * n1;
+    double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
+    if (ctl)
+        for (int k = 0; k < 6; ++k) dc6[k] = ctl->dcs6[k];
+    // twiddles of this lane: w_N^(l k1) for the four-step split, and W_2h^(l mod h) = w_N^(..) of
+    // each cross-lane stage
+    cx<T> wl[NPL > 1 ? NPL - 1 : 1];
+#pragma unroll
+    for (int k = 1; k < NPL; ++k) wl[k - 1] = tw[lane * k];
+    cx<T> ws[6];
+#pragma unroll
+    for (int st = 0; st < 6; ++st) {
+        const int h = 32 >> st;
+        ws[st] = tw[(lane & (h - 1)) * (32 / h) * NPL];
+    }
+    const size_t row = (size_t)kz * plane + (size_t)ky * N + lane;
+    const size_t cstride = (size_t)H * plane;
+    cx<T> q[6][NPL];
+#pragma unroll
+    for (int c = 0; c < 6; ++c)
+#pragma unroll
+        for (int r = 0; r < NPL; ++r) q[c][r] = spec[c * cstride + row + 64 * r];
+    wave_fft_fwd<T, NPL, 6>(q, wl, ws, lane);
+    cx<T> o[9][NPL];
+    const int kb = NPL * brev6(lane);
+#pragma unroll
+    for (int r = 0; r < NPL; ++r) {
+        cx<T> qi[6], oi[9];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) qi[c] = q[c][r];
+        green_point(cst, dc6, r + kb, ky, kz, N, n1, n2, qi, oi);
+#pragma unroll
+        for (int c = 0; c < 9; ++c) o[c][r] = oi[c];
+    }
+    wave_fft_inv<T, NPL, 9>(o, wl, ws, lane);
+#pragma unroll
+    for (int c = 0; c < 9; ++c)
+#pragma unroll
+        for (int r = 0; r < NPL; ++r) spec[c * cstride + row + 64 * r] = o[c][r];
 }
 
 // ---------------------------------------------------------------------------
@@ -1958,8 +2121,29 @@ static void launch_green_generic(hipStream_t s, int prec, void *specC, const int
     check_launch("green_x");
 }
 
+// The wavefront-per-row x pass serves n0 = 64, 128, 256 (tile -1, and the default tile 0 there);
+// other extents, and tiles >= 1, use the LDS kernel.
+bool green_wave_ok(const int n[3]) { return n[0] == 64 || n[0] == 128 || n[0] == 256; }
+
+template <typename T>
+static void launch_green_wave(hipStream_t s, void *specC, const int n[3], const void *twx, const DevConst *cst,
+                              const IterCtl *ctl) {
+    const dim3 grid((n[1] + 3) / 4, n[2] / 2 + 1), block(256);
+    cx<T> *sp = (cx<T> *)specC;
+    const cx<T> *tw = (const cx<T> *)twx;
+    if (n[0] == 64) SRX_LAUNCH((k_green_wave<T, 1>), grid, block, 0, s, sp, n[1], n[2], tw, cst, 0., 0., 0., 0., 0., 0., ctl);
+    else if (n[0] == 128) SRX_LAUNCH((k_green_wave<T, 2>), grid, block, 0, s, sp, n[1], n[2], tw, cst, 0., 0., 0., 0., 0., 0., ctl);
+    else SRX_LAUNCH((k_green_wave<T, 4>), grid, block, 0, s, sp, n[1], n[2], tw, cst, 0., 0., 0., 0., 0., 0., ctl);
+}
+
 void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const FftPlan &px, const void *twx,
                     const DevConst *cst, const IterCtl *ctl, int tile) {
+    if (green_wave_ok(n) && tile <= 0) {
+        if (prec == 0) launch_green_wave<double>(s, specC, n, twx, cst, ctl);
+        else launch_green_wave<float>(s, specC, n, twx, cst, ctl);
+        check_launch("green_x");
+        return;
+    }
     launch_green_generic(s, prec, specC, n, px, twx, cst, nullptr, 1, 9, ctl, tile);
 }
 

@@ -200,6 +200,9 @@ void set_launch_events(hipEvent_t start, hipEvent_t stop);
 // `tile`: rows (pencils) per block, 0 for the default; fft_tile_max gives the largest valid one of
 // a pass (0 forward z, 1 forward y, 2 Green x, 3 inverse y, 4 inverse z; p: the pass axis's plan).
 int fft_tile_max(int pass, const int n[3], const FftPlan &p, int prec);
+// The solver's Green x pass can run one wavefront per row (tile -1; the default tile 0 selects it)
+// for n0 = 64, 128, 256.
+bool green_wave_ok(const int n[3]);
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3],
                       const FftPlan &pz, const void *twz, const IterCtl *ctl, int tile = 0);
 void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp, int sign, int b_to_c,

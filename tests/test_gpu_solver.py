@@ -197,3 +197,18 @@ def test_warm_start_saves_newton_iterations(tmp_path):
             nr_warm += rw["nr_iters_total"]
     assert nr_warm < 0.9 * nr_cold, (nr_warm, nr_cold)
     assert rel(warm.field("rho_s"), cold.field("rho_s")) < 1e-8
+
+
+@pytest.mark.parametrize("n", [(64, 6, 4), (128, 4, 6), (256, 2, 4)])
+def test_wavefront_green_pass_matches_oracle(tmp_path, oracle_lib, n):
+    """x extents of 64, 128 and 256 run the Green x pass one wavefront per row (1, 2 and 4 points
+    per lane, cross-lane radix-2 stages); FP64 parity with the oracle's full complex FFT, with an
+    elastic-BC-0 deck so the k = 0 term is exercised too."""
+    deck = make(tmp_path, n=n, grains=4, seed=15, nsteps=3, elast_bc=0)
+    o = oracle_lib.Oracle(deck, fixed_iters=5)
+    g = srx_amd.Solver(deck, fixed_iters=5, reserved=[0, 0, 1])   # default tiles: the wavefront pass
+    for step in range(3):
+        ro, rg = o.step(), g.step()
+        assert rg["nr_iters_total"] == ro["nr_iters_total"]
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9)
