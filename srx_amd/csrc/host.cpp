@@ -657,6 +657,7 @@ int setup(srx_ctx *c) {
                 double v = c->hc.C0[((i * 3 + j) * 3 + k) * 3 + l];
                 if (j != l) v += c->hc.C0[((i * 3 + l) * 3 + k) * 3 + j];
                 c->hc.G6[6 * a + b] = v;
+                c->hc.G6f[6 * a + b] = (float)v;
             }
     }
 

@@ -409,10 +409,11 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
 //   Omega = (C0_ijkl n_j n_l)^-1,  du_ij = -Omega_ik (sig_kl n_l) n_j
 // sig: 6 B-basis comps (complex); du: 9 comps (row-major tensor).
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void unit_k(int idx, int n, double &t) {
-    double v = (double)idx * (double)(1.0 / n);
+template <typename A>
+__device__ __forceinline__ void unit_k(int idx, int n, A &t) {
+    A v = (A)idx * (A)(1.0 / n);
     v -= round(v);
-    t = v * 6.28318530718;
+    t = v * (A)6.28318530718;
 }
 
 __device__ void green_w(const double *__restrict__ C0, const double nv[3], const double sr[9], const double si[9],
@@ -473,13 +474,14 @@ __device__ __forceinline__ void b6_to_t9(const V v[6], V t[9]) {
 }
 
 // B-basis 6-vector -> symmetric tensor entries (00, 11, 22, 12, 02, 01)
-__device__ __forceinline__ void b6_to_sym(const double v[6], double t[6]) {
-    t[0] = v[0] * (-SRX_R2) + v[1] * (-SRX_R6) + v[5] * SRX_R3;
-    t[1] = v[0] * SRX_R2 + v[1] * (-SRX_R6) + v[5] * SRX_R3;
-    t[2] = v[1] * (2. * SRX_R6) + v[5] * SRX_R3;
-    t[3] = v[2] * SRX_R2;
-    t[4] = v[3] * SRX_R2;
-    t[5] = v[4] * SRX_R2;
+template <typename A>
+__device__ __forceinline__ void b6_to_sym(const A v[6], A t[6]) {
+    t[0] = v[0] * A(-SRX_R2) + v[1] * A(-SRX_R6) + v[5] * A(SRX_R3);
+    t[1] = v[0] * A(SRX_R2) + v[1] * A(-SRX_R6) + v[5] * A(SRX_R3);
+    t[2] = v[1] * A(2. * SRX_R6) + v[5] * A(SRX_R3);
+    t[3] = v[2] * A(SRX_R2);
+    t[4] = v[3] * A(SRX_R2);
+    t[5] = v[4] * A(SRX_R2);
 }
 __device__ __forceinline__ void sym_to_t9(const double t[6], double m[9]) {
     m[0] = t[0]; m[4] = t[1]; m[8] = t[2];
@@ -492,25 +494,26 @@ __device__ __forceinline__ void sym_to_t9(const double t[6], double m[9]) {
 //   A_ik = sum_jl C0_ijkl n_j n_l = sum_b G6[(ik)][b] m_b,  m = (n0^2, n1^2, n2^2, n1n2, n0n2, n0n1)
 //   w = A^-1 (sig n)  (cofactors of the symmetric A, as M3Inverse, init.c:518-540)
 // and du_ij = -w_i n_j  (init.c:585 with evolution.c:1169-1183).
-__device__ __forceinline__ void green_w6(const double *__restrict__ G6, const double n[3], const double sr[6],
-                                         const double si[6], double wr[3], double wi[3]) {
-    const double m[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[1] * n[2], n[0] * n[2], n[0] * n[1]};
-    double A[6];
+template <typename A>
+__device__ __forceinline__ void green_w6(const A *__restrict__ G6, const A n[3], const A sr[6], const A si[6],
+                                         A wr[3], A wi[3]) {
+    const A m[6] = {n[0] * n[0], n[1] * n[1], n[2] * n[2], n[1] * n[2], n[0] * n[2], n[0] * n[1]};
+    A Am[6];
 #pragma unroll
     for (int a = 0; a < 6; ++a) {
-        double acc = 0.0;
+        A acc = A(0);
 #pragma unroll
         for (int b = 0; b < 6; ++b) acc += G6[6 * a + b] * m[b];
-        A[a] = acc;
+        Am[a] = acc;
     }
     // A = [[A0, A5, A4], [A5, A1, A3], [A4, A3, A2]]
-    const double c00 = A[1] * A[2] - A[3] * A[3], c11 = A[0] * A[2] - A[4] * A[4], c22 = A[0] * A[1] - A[5] * A[5];
-    const double c01 = A[4] * A[3] - A[5] * A[2], c02 = A[5] * A[3] - A[4] * A[1], c12 = A[5] * A[4] - A[0] * A[3];
-    const double rdet = 1.0 / (A[0] * c00 + A[5] * c01 + A[4] * c02);
+    const A c00 = Am[1] * Am[2] - Am[3] * Am[3], c11 = Am[0] * Am[2] - Am[4] * Am[4], c22 = Am[0] * Am[1] - Am[5] * Am[5];
+    const A c01 = Am[4] * Am[3] - Am[5] * Am[2], c02 = Am[5] * Am[3] - Am[4] * Am[1], c12 = Am[5] * Am[4] - Am[0] * Am[3];
+    const A rdet = A(1) / (Am[0] * c00 + Am[5] * c01 + Am[4] * c02);
     // t = sig n with sig = [[s0, s5, s4], [s5, s1, s3], [s4, s3, s2]]
-    const double t0r = sr[0] * n[0] + sr[5] * n[1] + sr[4] * n[2], t0i = si[0] * n[0] + si[5] * n[1] + si[4] * n[2];
-    const double t1r = sr[5] * n[0] + sr[1] * n[1] + sr[3] * n[2], t1i = si[5] * n[0] + si[1] * n[1] + si[3] * n[2];
-    const double t2r = sr[4] * n[0] + sr[3] * n[1] + sr[2] * n[2], t2i = si[4] * n[0] + si[3] * n[1] + si[2] * n[2];
+    const A t0r = sr[0] * n[0] + sr[5] * n[1] + sr[4] * n[2], t0i = si[0] * n[0] + si[5] * n[1] + si[4] * n[2];
+    const A t1r = sr[5] * n[0] + sr[1] * n[1] + sr[3] * n[2], t1i = si[5] * n[0] + si[1] * n[1] + si[3] * n[2];
+    const A t2r = sr[4] * n[0] + sr[3] * n[1] + sr[2] * n[2], t2i = si[4] * n[0] + si[3] * n[1] + si[2] * n[2];
     wr[0] = (c00 * t0r + c01 * t1r + c02 * t2r) * rdet;
     wi[0] = (c00 * t0i + c01 * t1i + c02 * t2i) * rdet;
     wr[1] = (c01 * t0r + c11 * t1r + c12 * t2r) * rdet;
@@ -520,21 +523,18 @@ __device__ __forceinline__ void green_w6(const double *__restrict__ G6, const do
 }
 
 // Green operator at one k point (kx, ky, kz): B-basis spectrum q[6] of the stress -> du[9]
-// (row-major).  k = 0: -S0 (sig + dc6) for ElastBC 0, else 0 (init.c:558-569); dc6 is the grid sum
-// of sig that a mean-subtracted (mixed precision) spectrum lacks, B basis.
+// (row-major), computed in the spectrum's precision T (FP32 in the mixed-precision path, whose
+// convolution is FP32 throughout).  k = 0: -S0 (sig + dc6) for ElastBC 0, else 0 (init.c:558-569);
+// dc6 is the grid sum of sig that a mean-subtracted (mixed precision) spectrum lacks, B basis.
 template <typename T>
 __device__ __forceinline__ void green_point(const DevConst *__restrict__ cst, const double dc6[6], int kx, int ky,
                                             int kz, int n0, int n1, int n2, const cx<T> q[6], cx<T> out[9]) {
-    double tzv, tyv, txv;
-    unit_k(kz, n2, tzv);
-    unit_k(ky, n1, tyv);
-    unit_k(kx, n0, txv);
-    const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
-    const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
-    const bool nyy = (n1 % 2 == 0) && (ky == n1 / 2);
-    double sr[6], si[6];
-    {
-        double vr[6], vi[6];
+    using A = T;
+    const A *G6;
+    if constexpr (sizeof(T) == 4) G6 = cst->G6f;
+    else G6 = cst->G6;
+    if (kx == 0 && ky == 0 && kz == 0) {
+        double sr[6], si[6], vr[6], vi[6];
 #pragma unroll
         for (int c = 0; c < 6; ++c) {
             vr[c] = (double)q[c].x;
@@ -542,8 +542,6 @@ __device__ __forceinline__ void green_point(const DevConst *__restrict__ cst, co
         }
         b6_to_sym(vr, sr);
         b6_to_sym(vi, si);
-    }
-    if (kx == 0 && ky == 0 && kz == 0) {
         double dr[9], di[9];
         if (cst->elast_bc == 0) {
             double dcs[6], t9r[9], t9i[9];
@@ -569,11 +567,29 @@ __device__ __forceinline__ void green_point(const DevConst *__restrict__ cst, co
         for (int qq = 0; qq < 9; ++qq) out[qq] = cx<T>{T(dr[qq]), T(di[qq])};
         return;
     }
-    double nv[3] = {txv, tyv, tzv};
-    const double rg = 1. / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    A tzv, tyv, txv;
+    unit_k(kz, n2, tzv);
+    unit_k(ky, n1, tyv);
+    unit_k(kx, n0, txv);
+    const bool nyz = (n2 % 2 == 0) && (kz == n2 / 2);
+    const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
+    const bool nyy = (n1 % 2 == 0) && (ky == n1 / 2);
+    A sr[6], si[6];
+    {
+        A vr[6], vi[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            vr[c] = (A)q[c].x;
+            vi[c] = (A)q[c].y;
+        }
+        b6_to_sym(vr, sr);
+        b6_to_sym(vi, si);
+    }
+    A nv[3] = {txv, tyv, tzv};
+    const A rg = A(1) / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
     nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
-    double wr[3], wi[3];
-    green_w6(cst->G6, nv, sr, si, wr, wi);
+    A wr[3], wi[3];
+    green_w6(G6, nv, sr, si, wr, wi);
     if (!(nyx || nyy || nyz)) {
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -583,15 +599,15 @@ __device__ __forceinline__ void green_point(const DevConst *__restrict__ cst, co
         // Taking Re() of the inverse transform (evolution.c:1197) keeps the Hermitian part of du; at
         // a Nyquist index that part averages Gamma(n) with Gamma(n') where n' has the Nyquist
         // components negated.
-        const double np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
-        double wr2[3], wi2[3];
-        green_w6(cst->G6, np, sr, si, wr2, wi2);
+        const A np[3] = {nyx ? -nv[0] : nv[0], nyy ? -nv[1] : nv[1], nyz ? -nv[2] : nv[2]};
+        A wr2[3], wi2[3];
+        green_w6(G6, np, sr, si, wr2, wi2);
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b)
-                out[3 * a + b] = cx<T>{T(0.5 * (-wr[a] * nv[b] - wr2[a] * np[b])),
-                                       T(0.5 * (-wi[a] * nv[b] - wi2[a] * np[b]))};
+                out[3 * a + b] = cx<T>{T(A(0.5) * (-wr[a] * nv[b] - wr2[a] * np[b])),
+                                       T(A(0.5) * (-wi[a] * nv[b] - wi2[a] * np[b]))};
     }
 }
 

@@ -42,6 +42,7 @@ struct DevConst {
     double C0[81];        // average stiffness, 4th order (init.c:370)
     double S0[81];        // its inverse, 4th order (init.c:371)
     double G6[36];        // C0 contracted over symmetric index pairs: A_ik = sum_b G6[(ik)][b] (n n)_b
+    float G6f[36];        // the same in FP32 (the mixed-precision Green operator)
     double dt;            // TimeStep
     double nu;            // ElastConst_PhaseI[3], used for every phase (constitutive.c:1383)
     double T, D0;
