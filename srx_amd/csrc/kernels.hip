@@ -329,7 +329,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
         for (int k = 0; k < 6; ++k) mean[k] = ctl->mean6[k];
     const FastDiv dn2(n2), dty(ty);
     for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
-    for (int ca = 0; ca < ncomp; ca += 2) {
+    // one pair of components per block along grid z
+    for (int ca = 2 * blockIdx.z; ca < ncomp && ca < 2 * blockIdx.z + 2; ca += 2) {
         const int cb = ca + 1;
         const bool two = cb < ncomp;
         const double ma = ca < 6 ? mean[ca] : 0.0, mb = (two && cb < 6) ? mean[cb] : 0.0;
@@ -845,7 +846,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
     const FastDiv dn2(n2), dty(ty);
     // 1/Nxyz: a power of two for the usual grids, where the product equals the division
     const double rdiv = 1.0 / divisor;
-    for (int ca = 0; ca < ncomp; ca += 2) {
+    // one pair of components per block along grid z
+    for (int ca = 2 * blockIdx.z; ca < ncomp && ca < 2 * blockIdx.z + 2; ca += 2) {
         const int cb = ca + 1;
         const bool two = cb < ncomp;
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
@@ -2068,7 +2070,7 @@ int fft_tile_max(int pass, const int n[3], const FftPlan &p, int prec) {
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
                       const void *twz, const IterCtl *ctl, int tile) {
     const int TY = zy_tile(pz, n[1], tile, prec);
-    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY, 3);
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
@@ -2166,7 +2168,7 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
                              const void *twz, const IterCtl *ctl, double divisor, int tile) {
     const int TY = zy_tile(pz, n[1], tile, prec);
-    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY, 5);
     const double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // dDisGradAvg comes from ctl
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
@@ -2255,7 +2257,7 @@ void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, i
 void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int ncomp, void *specB, void *specC,
                                   const int n[3], const FftPlan *pl, void *const *tw) {
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
-    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
@@ -2276,7 +2278,7 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
     launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp, nullptr);
     launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1], nullptr);
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
-    dim3 grid(n[0], (n[1] + TY - 1) / TY);
+    dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
         size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
