@@ -899,7 +899,8 @@ const char *srx_strerror(int code) {
 
 // FFT tile autotuner: each pass runs on the context's own buffers (the inverse z pass into a
 // scratch DisGrad) with every candidate tile (powers of two up to the largest valid one, and that
-// one), three timed launches after a warm-up, and keeps the fastest.
+// one), three timed launches after a warm-up, and keeps the fastest by a 3% margin over the
+// default.
 static int autotune_tiles(srx_ctx *c) {
     const int prec = c->opt.precision;
     hipStream_t st = c->stream;
@@ -911,10 +912,12 @@ static int autotune_tiles(srx_ctx *c) {
     for (int pass = 0; pass < 5 && !rc; ++pass) {
         const FftPlan &p = c->pl[pass == 2 ? 0 : (pass == 1 || pass == 3) ? 1 : 2];
         const int tmax = fft_tile_max(pass, c->n, p, prec);
+        // the default first (the largest tile; the wavefront x pass where it applies); another
+        // candidate must be more than 3% faster to replace it, so near-ties do not flip between runs
         std::vector<int> cand;
         if (pass == 2 && green_wave_ok(c->n)) cand.push_back(-1);   // one wavefront per row
-        for (int t = 1; t < tmax; t *= 2) cand.push_back(t);
         cand.push_back(tmax);
+        for (int t = 1; t < tmax; t *= 2) cand.push_back(t);
         double best = 1e300;
         for (int t : cand) {
             auto run = [&]() {
@@ -938,7 +941,7 @@ static int autotune_tiles(srx_ctx *c) {
                 break;
             }
             const double us = 1000.0 * ms / 3.0;
-            if (us < best) {
+            if (us < (best < 1e300 ? 0.97 * best : best)) {
                 best = us;
                 c->tiles[pass] = t;
                 c->tile_us[pass] = us;
