@@ -146,15 +146,17 @@ typedef struct {
     int    increments;       /* increments completed */
     long   voxels;           /* CellDim product */
     double setup_seconds;    /* srx_open */
-    double run_seconds;      /* all increments and srx_close */
+    double run_seconds;      /* the member's increments */
+    double run_phase_seconds;/* wall time of the whole run phase (every member's increments) */
     srx_step_report last;    /* report of the last completed increment */
 } srx_ensemble_result;
 
 /* Runs every increment of n independent decks (an ensemble of RVEs, each the equivalent of one
-   `trial <deck>` job) on this GPU, `concurrency` of them at a time (0: 4), each on its own stream
-   from its own host thread, largest grids first.  Output files go to workdirs[i] (NULL: each deck's
-   directory).  results[i] is filled for every deck; returns 0 or the first member's error code.
-   No counterpart in the reference, which runs one RVE per mpirun job. */
+   `trial <deck>` job) on this GPU.  Every member is set up first (all stay resident on the GPU),
+   then the increments run `concurrency` members at a time (0: 4), each on its own stream from its
+   own host thread, largest grids first; then all are closed.  Output files go to workdirs[i] (NULL:
+   each deck's directory).  results[i] is filled for every deck; returns 0 or the first member's
+   error code.  No counterpart in the reference, which runs one RVE per mpirun job. */
 int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int n, const srx_options *opt,
                      int concurrency, srx_ensemble_result *results);
 

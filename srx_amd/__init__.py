@@ -82,7 +82,8 @@ class MlpTrainOpts(C.Structure):
 class EnsembleResult(C.Structure):
     """srx_ensemble_result of include/srx.h."""
     _fields_ = [("index", C.c_int), ("rc", C.c_int), ("increments", C.c_int), ("voxels", C.c_long),
-                ("setup_seconds", C.c_double), ("run_seconds", C.c_double), ("last", StepReport)]
+                ("setup_seconds", C.c_double), ("run_seconds", C.c_double), ("run_phase_seconds", C.c_double),
+                ("last", StepReport)]
 
 _lib = None
 
@@ -336,7 +337,8 @@ def mlp_forward(blob: np.ndarray, x: np.ndarray) -> np.ndarray:
 
 def run_ensemble(decks, workdirs=None, concurrency: int = 4, **opt) -> list[dict]:
     """srx_run_ensemble: every increment of each deck, `concurrency` RVEs at a time on this GPU.
-    Returns one dict per deck (rc, increments, voxels, setup_seconds, run_seconds, last report);
+    Returns one dict per deck (rc, increments, voxels, setup_seconds, run_seconds, run_phase_seconds
+    (wall time of all members' increments, setup excluded), last report);
     raises SrxError if any member failed."""
     n = len(decks)
     arr = (C.c_char_p * n)(*[d.encode() for d in decks])
@@ -345,7 +347,8 @@ def run_ensemble(decks, workdirs=None, concurrency: int = 4, **opt) -> list[dict
     o = options(**opt)
     rc = lib().srx_run_ensemble(arr, wds, n, C.byref(o), concurrency, res)
     out = [{"index": r.index, "rc": r.rc, "increments": r.increments, "voxels": r.voxels,
-            "setup_seconds": r.setup_seconds, "run_seconds": r.run_seconds, "last": r.last.as_dict()} for r in res]
+            "setup_seconds": r.setup_seconds, "run_seconds": r.run_seconds,
+            "run_phase_seconds": r.run_phase_seconds, "last": r.last.as_dict()} for r in res]
     if rc:
         raise SrxError(rc, "srx_run_ensemble")
     return out

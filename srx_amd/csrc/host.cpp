@@ -1364,35 +1364,48 @@ int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int 
     srx_options o;
     if (opt) o = *opt; else srx_default_options(&o);
     if (o.nthreads <= 0) o.nthreads = std::max(1, (int)std::thread::hardware_concurrency() / workers);
-    std::atomic<int> next{0};
-    auto work = [&]() {
-        for (;;) {
-            const int k = next.fetch_add(1);
-            if (k >= n) return;
-            const int i = order[k];
-            srx_ensemble_result &r = results[i];
-            std::memset(&r, 0, sizeof(r));
-            r.index = i;
-            r.voxels = size[i];
-            const auto t0 = std::chrono::steady_clock::now();
-            srx_ctx *c = nullptr;
-            int rc = srx_open(decks[i], workdirs ? workdirs[i] : nullptr, &o, &c);
-            const auto t1 = std::chrono::steady_clock::now();
-            while (rc == 0) {
-                rc = srx_step(c, &r.last);
-                if (rc == 0) ++r.increments;
-            }
-            if (rc == SRX_E_DONE || rc == SRX_E_CREEP_STOP) rc = 0;
-            if (c) srx_close(c);
-            const auto t2 = std::chrono::steady_clock::now();
-            r.rc = rc;
-            r.setup_seconds = std::chrono::duration<double>(t1 - t0).count();
-            r.run_seconds = std::chrono::duration<double>(t2 - t1).count();
-        }
+    std::vector<srx_ctx *> ctx(n, nullptr);
+    for (int i = 0; i < n; ++i) {
+        std::memset(&results[i], 0, sizeof(results[i]));
+        results[i].index = i;
+        results[i].voxels = size[i];
+    }
+    // `workers` host threads take members from a shared queue in `order`, running `job` on each
+    auto pool = [&](auto job) {
+        std::atomic<int> next{0};
+        std::vector<std::thread> th;
+        for (int w = 0; w < workers; ++w)
+            th.emplace_back([&]() {
+                for (int k; (k = next.fetch_add(1)) < n;) job(order[k]);
+            });
+        for (auto &t : th) t.join();
     };
-    std::vector<std::thread> pool;
-    for (int w = 0; w < workers; ++w) pool.emplace_back(work);
-    for (auto &t : pool) t.join();
+    // 1. setup of every member (all stay resident: the run phase below is the increment loop alone)
+    pool([&](int i) {
+        const auto t0 = std::chrono::steady_clock::now();
+        results[i].rc = srx_open(decks[i], workdirs ? workdirs[i] : nullptr, &o, &ctx[i]);
+        results[i].setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+    // 2. the increments, `workers` members at a time, each on its own stream
+    const auto r0 = std::chrono::steady_clock::now();
+    pool([&](int i) {
+        srx_ensemble_result &r = results[i];
+        if (r.rc) return;
+        const auto t0 = std::chrono::steady_clock::now();
+        int rc = 0;
+        while (rc == 0) {
+            rc = srx_step(ctx[i], &r.last);
+            if (rc == 0) ++r.increments;
+        }
+        r.rc = (rc == SRX_E_DONE || rc == SRX_E_CREEP_STOP) ? 0 : rc;
+        r.run_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    });
+    const double run_phase = std::chrono::duration<double>(std::chrono::steady_clock::now() - r0).count();
+    // 3. teardown
+    for (int i = 0; i < n; ++i) {
+        results[i].run_phase_seconds = run_phase;
+        if (ctx[i]) srx_close(ctx[i]);
+    }
     for (int i = 0; i < n; ++i)
         if (results[i].rc) return results[i].rc;
     return 0;

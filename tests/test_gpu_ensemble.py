@@ -31,6 +31,11 @@ def make_cases(root):
 def test_ensemble_matches_members_run_alone(tmp_path):
     decks = make_cases(str(tmp_path / "ens"))
     res = srx_amd.run_ensemble(decks, concurrency=3, write_files=1)
+    # every member is set up before the timed run phase, which covers all members' increments
+    phase = res[0]["run_phase_seconds"]
+    assert all(r["run_phase_seconds"] == phase for r in res)
+    assert phase >= max(r["run_seconds"] for r in res) > 0
+    assert all(r["setup_seconds"] > 0 for r in res)
     alone = make_cases(str(tmp_path / "alone"))
     for k, deck in enumerate(alone):
         s = srx_amd.Solver(deck, write_files=1)
