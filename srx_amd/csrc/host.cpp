@@ -87,24 +87,6 @@ void c66_to_c4(const double c[36], double t[81]) {
                 }
 }
 
-// Q6 rotation of B vectors (crystal -> sample for a = TranMat_xt2sa); 6x6 with hydro 1
-void q6_from(const double a[9], double Q[36]) {
-    std::memset(Q, 0, 36 * sizeof(double));
-    for (int be = 0; be < 5; ++be) {
-        double m[9];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) {
-                double s = 0.0;
-                for (int k = 0; k < 3; ++k)
-                    for (int l = 0; l < 3; ++l) s += a[3 * i + k] * BB.b[be][k][l] * a[3 * j + l];
-                m[3 * i + j] = s;
-            }
-        double v[6];
-        t9_to_b6(m, v);
-        for (int al = 0; al < 5; ++al) Q[6 * al + be] = v[al];
-    }
-    Q[35] = 1.0;
-}
 
 // Gauss-Jordan inverse with partial pivoting (n <= 6)
 bool invert(double *A, int n) {
@@ -149,20 +131,6 @@ void voigt_to_full(const double *v, double m[9]) {   // kinematics.c:5-12
     m[6] = v[4]; m[7] = v[3]; m[8] = v[2];
 }
 
-// EulerToTransMatrix (kinematics.c:187-240), opt 2: angles (rad) -> sample->crystal matrix
-void euler_to_matrix(double t1, double ph, double t2, double a[9]) {
-    const double ct1 = std::cos(t1), st1 = std::sin(t1), cph = std::cos(ph), sph = std::sin(ph);
-    const double ct2 = std::cos(t2), st2 = std::sin(t2);
-    a[0] = ct1 * ct2 - st1 * st2 * cph;
-    a[1] = st1 * ct2 + ct1 * st2 * cph;
-    a[2] = st2 * sph;
-    a[3] = -1.0 * ct1 * st2 - st1 * ct2 * cph;
-    a[4] = -1.0 * st1 * st2 + ct1 * ct2 * cph;
-    a[5] = ct2 * sph;
-    a[6] = st1 * sph;
-    a[7] = -1.0 * ct1 * sph;
-    a[8] = cph;
-}
 // opt 1: matrix -> angles in degrees
 void matrix_to_euler(const double a[9], double &t1, double &ph, double &t2) {
     double p = std::acos(a[8]), x1, x2;
@@ -531,8 +499,8 @@ int setup(srx_ctx *c) {
     // microstructure (init.c:233-396)
     const long N = c->nvox;
     const int nthr = c->opt.nthreads > 0 ? c->opt.nthreads : omp_get_max_threads();
-    std::vector<double> tran(9 * N, 0.0), sig(6 * N, 0.0);
     std::vector<int> phase(N, 0);
+    std::vector<double> eul(3 * N);   // Bunge angles (degrees) per voxel, for the device setup
     c->grain.assign(N, 0);
     {
         // The reference reads N records with fgets(buf, 80) and sscanf (init.c:253-301).  When each
@@ -560,7 +528,6 @@ int setup(srx_ctx *c) {
             pos = end + 1;
         }
         if ((long)start.size() < N) return 1139;   // fgets hits EOF before N records
-        std::vector<double> eul(3 * N);
         std::vector<int> ijk(3 * N);
         auto parse = [&](const char *line, long v) {
             double t1 = 0, th = 0, om = 0;
@@ -600,67 +567,8 @@ int setup(srx_ctx *c) {
             if (idx != v || jph < 1 || jph > NPH) return 1139;
             if (jph == 1) ++nph1;
         }
-#pragma omp parallel for schedule(static) num_threads(nthr)
-        for (long v = 0; v < N; ++v) {
-            const int jph = phase[v];
-            if (!c->ph[jph - 1].xtal) continue;
-            const double t1 = eul[3 * v] * (PI_REF / 180.), th = eul[3 * v + 1] * (PI_REF / 180.);
-            const double om = eul[3 * v + 2] * (PI_REF / 180.);
-            double s2x[9];
-            euler_to_matrix(t1, th, om, s2x);
-            for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) tran[9 * v + 3 * a + b] = s2x[3 * b + a];
-        }
         c->wgt_ph1 = (1.0 * (double)nph1) / N;
     }
-    // C066 = <C_gr> over the grid; C_gr = Q diag Q^T in the B basis.  Chunked sums
-    // keep the result independent of the thread count.
-    {
-        const long CH = 4096;
-        const long nch = (N + CH - 1) / CH;
-        std::vector<double> part(36 * nch, 0.0);
-#pragma omp parallel for schedule(static) num_threads(nthr)
-        for (long ch = 0; ch < nch; ++ch) {
-            double acc[36] = {0};
-            for (long v = ch * CH; v < std::min(N, (ch + 1) * CH); ++v) {
-                const int jph = phase[v];
-                if (!c->ph[jph - 1].xtal) continue;
-                double Q[36];
-                q6_from(&tran[9 * v], Q);
-                const double *sinv = c->hc.ph[jph - 1].sinv;
-                for (int i = 0; i < 6; ++i)
-                    for (int j = 0; j < 6; ++j) {
-                        double s = 0.0;
-                        for (int k = 0; k < 6; ++k) s += Q[6 * i + k] * (1.0 / sinv[k]) * Q[6 * j + k];
-                        acc[6 * i + j] += s;
-                    }
-            }
-            std::memcpy(&part[36 * ch], acc, sizeof(acc));
-        }
-        double C[36] = {0};
-        for (long ch = 0; ch < nch; ++ch)
-            for (int k = 0; k < 36; ++k) C[k] += part[36 * ch + k];
-        for (int k = 0; k < 36; ++k) c->C066[k] = C[k] * c->wgt;
-        // exact block structure: the hydrostatic axis of a cubic crystal is isotropic
-        for (int k = 0; k < 5; ++k) c->C066[6 * k + 5] = c->C066[30 + k] = 0.0;
-        std::memcpy(c->S066, c->C066, sizeof(c->C066));
-        if (!invert(c->S066, 6)) return 35;
-        for (int i = 0; i < 6; ++i)
-            for (int j = 0; j < 6; ++j) c->hc.S066[i][j] = c->S066[6 * i + j];
-        c66_to_c4(c->C066, c->hc.C0);
-        c66_to_c4(c->S066, c->hc.S0);
-        // G6[(ik)][(jl)] = C0_ijkl (+ C0_ilkj for j != l): A_ik = sum_jl C0_ijkl n_j n_l as a 6x6 product
-        static const int PI6[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
-        for (int a = 0; a < 6; ++a)
-            for (int b = 0; b < 6; ++b) {
-                const int i = PI6[a][0], k = PI6[a][1], j = PI6[b][0], l = PI6[b][1];
-                double v = c->hc.C0[((i * 3 + j) * 3 + k) * 3 + l];
-                if (j != l) v += c->hc.C0[((i * 3 + l) * 3 + k) * 3 + j];
-                c->hc.G6[6 * a + b] = v;
-                c->hc.G6f[6 * a + b] = (float)v;
-            }
-    }
-
     // boundary conditions (init.c:865-901)
     {
         if (d.VelGrad_BC_Flag[0] + d.VelGrad_BC_Flag[1] + d.VelGrad_BC_Flag[2] == 2) return 116;
@@ -692,34 +600,10 @@ int setup(srx_ctx *c) {
         c->DisGradAvg[k] = c->Udot[k] * d.TimeStep;
         c->dDisGradAvg[k] = c->acum[k] = c->DisGradAvg_t[k] = c->uniform[k] = 0.0;
     }
-    {
-        double e9[9], e6[6], sc6[6];
-        for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) e9[3 * i + j] = (c->DisGradAvg[3 * i + j] + c->DisGradAvg[3 * j + i]) / 2.0;
-        t9_to_b6(e9, e6);
-        t9_to_b6(c->Scauchy, sc6);
-#pragma omp parallel for schedule(static) num_threads(nthr)
-        for (long v = 0; v < N; ++v) {
-            const int jph = phase[v];
-            if (!c->ph[jph - 1].xtal) continue;
-            double Q[36], t[6], s6[6];
-            q6_from(&tran[9 * v], Q);
-            const double *sinv = c->hc.ph[jph - 1].sinv;
-            for (int k = 0; k < 6; ++k) {
-                double a = 0.0;
-                for (int i = 0; i < 6; ++i) a += Q[6 * i + k] * e6[i];
-                t[k] = a / sinv[k];
-            }
-            for (int i = 0; i < 6; ++i) {
-                double a = 0.0;
-                for (int k = 0; k < 6; ++k) a += Q[6 * i + k] * t[k];
-                s6[i] = a + (c->creep == 1 ? sc6[i] : 0.0);
-            }
-            for (int i = 0; i < 6; ++i) sig[i * N + v] = s6[i];
-        }
-        for (long v = 0; v < N; ++v)
-            for (int i = 0; i < 6; ++i) c->sig_sum6[i] += sig[i * N + v];
-    }
+    // per voxel on the device after allocation (launch_setup): Sig = C_gr : sym(Udot dt) (+ Scauchy)
+    double e9[9];
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) e9[3 * i + j] = (c->DisGradAvg[3 * i + j] + c->DisGradAvg[3 * j + i]) / 2.0;
     c->e_vm = c->d_vm * d.TimeStep;
     c->err = c->opt.err_override > 0 ? c->opt.err_override : d.Err;
     c->itermax = c->opt.itermax_override > 0 ? c->opt.itermax_override : d.IterMax;
@@ -755,17 +639,54 @@ int setup(srx_ctx *c) {
     HIPCK(hipMalloc(&F.phase, sizeof(int) * N));
     HIPCK(hipMalloc(&F.nr_it, sizeof(int) * N));
     if ((rc = ctx_zero(st, F.nr_it, sizeof(int) * N)) ||
-        (rc = ctx_copy(st, F.phase, phase.data(), sizeof(int) * N, hipMemcpyHostToDevice)) ||
-        (rc = ctx_copy(st, F.sig, sig.data(), sizeof(double) * 6 * N, hipMemcpyHostToDevice)))
+        (rc = ctx_copy(st, F.phase, phase.data(), sizeof(int) * N, hipMemcpyHostToDevice)))
         return rc;
+    c->nr_blocks = nr_block_count((int)N);
+    c->end_blocks = end_block_count((int)N);
+    const int pb = std::max(std::max(c->nr_blocks * NR_NPART, c->end_blocks * END_NPART), setup_block_count((int)N) * 42);
+    HIPCK(hipMalloc(&c->partials, sizeof(double) * pb));
+    HIPCK(hipMalloc(&c->red, sizeof(double) * 64));
+    HIPCK(hipHostMalloc((void **)&c->hred, sizeof(double) * 64, hipHostMallocDefault));
+    // microstructure and InitFields on the device: orientations, initial stress, and the grid sums
+    // of C_gr and Sig
+    double sums[42];
     {
-        std::vector<double> soa(9 * N);
-        for (long v = 0; v < N; ++v)
-            for (int k = 0; k < 9; ++k) soa[k * N + v] = tran[9 * v + k];
-        if ((rc = ctx_copy(st, F.tran, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice)) ||
-            (rc = ctx_copy(st, F.tran0, soa.data(), sizeof(double) * 9 * N, hipMemcpyHostToDevice)))
-            return rc;
+        double *deul = nullptr;
+        HIPCK(hipMalloc(&deul, sizeof(double) * 3 * N));
+        rc = ctx_copy(st, deul, eul.data(), sizeof(double) * 3 * N, hipMemcpyHostToDevice);
+        if (!rc) {
+            launch_setup(st, c->dc, F, (int)N, deul, PI_REF / 180., e9, c->Scauchy, c->creep, c->partials, c->red);
+            rc = ctx_copy(st, sums, c->red, sizeof(sums), hipMemcpyDeviceToHost);
+        }
+        (void)hipFree(deul);
+        if (rc) return rc;
     }
+    for (int i = 0; i < 6; ++i) c->sig_sum6[i] = sums[36 + i];
+    // C066 = <C_gr> over the grid (the device sums, in a fixed order), its inverse and tables
+    {
+        for (int k = 0; k < 36; ++k) c->C066[k] = sums[k] * c->wgt;
+        // exact block structure: the hydrostatic axis of a cubic crystal is isotropic
+        for (int k = 0; k < 5; ++k) c->C066[6 * k + 5] = c->C066[30 + k] = 0.0;
+        std::memcpy(c->S066, c->C066, sizeof(c->C066));
+        if (!invert(c->S066, 6)) return 35;
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 6; ++j) c->hc.S066[i][j] = c->S066[6 * i + j];
+        c66_to_c4(c->C066, c->hc.C0);
+        c66_to_c4(c->S066, c->hc.S0);
+        // G6[(ik)][(jl)] = C0_ijkl (+ C0_ilkj for j != l): A_ik = sum_jl C0_ijkl n_j n_l as a 6x6 product
+        static const int PI6[6][2] = {{0, 0}, {1, 1}, {2, 2}, {1, 2}, {0, 2}, {0, 1}};
+        for (int a = 0; a < 6; ++a)
+            for (int b = 0; b < 6; ++b) {
+                const int i = PI6[a][0], k = PI6[a][1], j = PI6[b][0], l = PI6[b][1];
+                double v = c->hc.C0[((i * 3 + j) * 3 + k) * 3 + l];
+                if (j != l) v += c->hc.C0[((i * 3 + l) * 3 + k) * 3 + j];
+                c->hc.G6[6 * a + b] = v;
+                c->hc.G6f[6 * a + b] = (float)v;
+            }
+    }
+
+    if (int e = ctx_copy(st, c->dc, &c->hc, sizeof(DevConst), hipMemcpyHostToDevice)) return e;
+
     const size_t H = (size_t)(c->n[2] / 2 + 1);
     const size_t spec_elems = 9 * H * c->n[0] * c->n[1];
     const size_t csz = c->opt.precision == SRX_PREC_MIXED ? 2 * sizeof(float) : 2 * sizeof(double);
@@ -776,12 +697,6 @@ int setup(srx_ctx *c) {
                                                    : upload_twiddles<double>(c->n[ax], &c->tw[ax], st);
         if (r) return r;
     }
-    c->nr_blocks = nr_block_count((int)N);
-    c->end_blocks = end_block_count((int)N);
-    const int pb = std::max(c->nr_blocks * NR_NPART, c->end_blocks * END_NPART);
-    HIPCK(hipMalloc(&c->partials, sizeof(double) * pb));
-    HIPCK(hipMalloc(&c->red, sizeof(double) * 64));
-    HIPCK(hipHostMalloc((void **)&c->hred, sizeof(double) * 64, hipHostMallocDefault));
     HIPCK(hipMalloc(&c->dctl, sizeof(IterCtl)));
     HIPCK(hipHostMalloc((void **)&c->hctl, sizeof(IterCtl), hipHostMallocDefault));
     return 0;
@@ -817,7 +732,7 @@ struct Timer {
         if (c->evused + 2 > c->evpool.size()) {
             for (int k = 0; k < 256; ++k) {
                 hipEvent_t e;
-                hipEventCreate(&e);
+                (void)hipEventCreate(&e);
                 c->evpool.push_back(e);
             }
         }
@@ -836,11 +751,11 @@ const char *KT_NAMES[8] = {"fwd_z", "fwd_y", "green_x", "inv_y", "inv_z", "nr", 
 
 void collect_timing(srx_ctx *c) {
     if (!c->timing) return;
-    hipStreamSynchronize(c->stream);
+    (void)hipStreamSynchronize(c->stream);
     for (size_t i = 0; i < c->evslot.size(); ++i) {
         if (c->evit[i] > c->iters_last) continue;
         float ms = 0;
-        hipEventElapsedTime(&ms, c->evpool[2 * i], c->evpool[2 * i + 1]);
+        (void)hipEventElapsedTime(&ms, c->evpool[2 * i], c->evpool[2 * i + 1]);
         c->kt[c->evslot[i]].ms += ms;
         c->kt[c->evslot[i]].calls += 1;
     }
@@ -1560,7 +1475,7 @@ static void drop_graphs(srx_ctx *c) {
 
 void srx_close(srx_ctx *c) {
     if (!c) return;
-    if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->fvm) std::fclose(c->fvm);
     if (c->fdd) std::fclose(c->fdd);
     if (c->fdrx) std::fclose(c->fdrx);
@@ -1570,25 +1485,25 @@ void srx_close(srx_ctx *c) {
                       c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->f.rho_tot, c->f.fl0,
                       c->partials, c->red};
     for (double *p : ptrs)
-        if (p) hipFree(p);
-    if (c->f.dsig) hipFree(c->f.dsig);
-    if (c->mlp) hipFree(c->mlp);
-    if (c->mlp_blob) hipFree(c->mlp_blob);
+        if (p) (void)hipFree(p);
+    if (c->f.dsig) (void)hipFree(c->f.dsig);
+    if (c->mlp) (void)hipFree(c->mlp);
+    if (c->mlp_blob) (void)hipFree(c->mlp_blob);
     for (void *p : {(void *)c->rec_x, (void *)c->rec_t, (void *)c->rec_nd, (void *)c->rec_count})
-        if (p) hipFree(p);
-    if (c->f.phase) hipFree(c->f.phase);
-    if (c->f.nr_it) hipFree(c->f.nr_it);
-    if (c->specB) hipFree(c->specB);
-    if (c->specC) hipFree(c->specC);
+        if (p) (void)hipFree(p);
+    if (c->f.phase) (void)hipFree(c->f.phase);
+    if (c->f.nr_it) (void)hipFree(c->f.nr_it);
+    if (c->specB) (void)hipFree(c->specB);
+    if (c->specC) (void)hipFree(c->specC);
     for (int i = 0; i < 3; ++i)
-        if (c->tw[i]) hipFree(c->tw[i]);
-    if (c->dc) hipFree(c->dc);
-    if (c->hred) hipHostFree(c->hred);
-    if (c->dctl) hipFree(c->dctl);
-    if (c->hctl) hipHostFree(c->hctl);
-    for (hipEvent_t e : c->evpool) hipEventDestroy(e);
+        if (c->tw[i]) (void)hipFree(c->tw[i]);
+    if (c->dc) (void)hipFree(c->dc);
+    if (c->hred) (void)hipHostFree(c->hred);
+    if (c->dctl) (void)hipFree(c->dctl);
+    if (c->hctl) (void)hipHostFree(c->hctl);
+    for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
     drop_graphs(c);
-    if (c->stream) hipStreamDestroy(c->stream);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -1612,7 +1527,7 @@ int srx_mlp_set(srx_ctx *c, const float *blob) {
 
 static void free_recorder(srx_ctx *c) {
     for (void *p : {(void *)c->rec_x, (void *)c->rec_t, (void *)c->rec_nd, (void *)c->rec_count})
-        if (p) hipFree(p);
+        if (p) (void)hipFree(p);
     c->rec_x = c->rec_t = c->rec_nd = nullptr;
     c->rec_count = nullptr;
     c->rec_stride = c->rec_cap = 0;
@@ -1722,11 +1637,11 @@ int srx_fft_forward(const double *in, int ncomp, int n0, int n1, int n2, int pre
                         }
                     }
     }
-    if (din) hipFree(din);
-    if (sB) hipFree(sB);
-    if (sC) hipFree(sC);
+    if (din) (void)hipFree(din);
+    if (sB) (void)hipFree(sB);
+    if (sC) (void)hipFree(sC);
     for (int i = 0; i < 3; ++i)
-        if (tw[i]) hipFree(tw[i]);
+        if (tw[i]) (void)hipFree(tw[i]);
     return rc;
 }
 
@@ -1764,11 +1679,11 @@ int srx_fft_inverse(const double *in, int ncomp, int n0, int n1, int n2, int pre
         HIPCK(hipDeviceSynchronize());
         HIPCK(hipMemcpy(out, dout, sizeof(double) * ncomp * N, hipMemcpyDeviceToHost));
     }
-    if (dout) hipFree(dout);
-    if (sB) hipFree(sB);
-    if (sC) hipFree(sC);
+    if (dout) (void)hipFree(dout);
+    if (sB) (void)hipFree(sB);
+    if (sC) (void)hipFree(sC);
     for (int i = 0; i < 3; ++i)
-        if (tw[i]) hipFree(tw[i]);
+        if (tw[i]) (void)hipFree(tw[i]);
     return rc;
 }
 

@@ -1991,6 +1991,116 @@ __global__ void k_mlp_rec_adv(int *__restrict__ count, int cap, int nsamp, const
 }
 
 // ---------------------------------------------------------------------------
+// Setup on the device: per voxel, TranMat_xt2sa from the Bunge angles (EulerToTransMatrix,
+// kinematics.c:187-240, transposed as init.c:273-282 stores it), the initial stress
+// sig = C_gr : (Udot dt) (+ the creep stress; InitFields, init.c:444-516) with C_gr = Q diag Q^T in
+// the B basis, and per-block partial sums of C_gr (36) and sig (6) for the grid averages.
+// ---------------------------------------------------------------------------
+constexpr int SETUP_BLOCK = 256;
+constexpr int SETUP_NPART = 42;
+
+__global__ void __launch_bounds__(SETUP_BLOCK) k_setup(const DevConst *__restrict__ cst, Fields f, int nvox,
+                                                       const double *__restrict__ eul, double deg, Vec9 e9,
+                                                       Vec9 sc9, int creep, double *__restrict__ partials) {
+    const int v = blockIdx.x * SETUP_BLOCK + threadIdx.x;
+    const size_t N = (size_t)nvox;
+    double acc[SETUP_NPART];
+#pragma unroll
+    for (int k = 0; k < SETUP_NPART; ++k) acc[k] = 0.0;
+    if (v < nvox) {
+        const int ph = f.phase[v];
+        const bool xt = ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal;
+        double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        double s6[6] = {0, 0, 0, 0, 0, 0};
+        if (xt) {
+            const double t1 = eul[3 * (size_t)v] * deg, th = eul[3 * (size_t)v + 1] * deg;
+            const double om = eul[3 * (size_t)v + 2] * deg;
+            const double ct1 = cos(t1), st1 = sin(t1), cph = cos(th), sph = sin(th), ct2 = cos(om), st2 = sin(om);
+            // sample -> crystal rows, stored transposed (crystal -> sample)
+            const double s2x[9] = {ct1 * ct2 - st1 * st2 * cph, st1 * ct2 + ct1 * st2 * cph, st2 * sph,
+                                   -1.0 * ct1 * st2 - st1 * ct2 * cph, -1.0 * st1 * st2 + ct1 * ct2 * cph, ct2 * sph,
+                                   st1 * sph, -1.0 * ct1 * sph, cph};
+#pragma unroll
+            for (int i = 0; i < 3; ++i)
+#pragma unroll
+                for (int j = 0; j < 3; ++j) a[3 * i + j] = s2x[3 * j + i];
+            double Q[5][5];
+            q5_from(a, Q);
+            const double *sinv = cst->ph[ph - 1].sinv;
+            double c[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) c[k] = 1.0 / sinv[k];
+            // C_gr deviatoric block; the hydrostatic entry is 1/sinv[5], the off-blocks zero
+#pragma unroll
+            for (int i = 0; i < 5; ++i)
+#pragma unroll
+                for (int j = 0; j < 5; ++j) {
+                    double t = 0.0;
+#pragma unroll
+                    for (int k = 0; k < 5; ++k) t += Q[i][k] * c[k] * Q[j][k];
+                    acc[6 * i + j] = t;
+                }
+            acc[35] = 1.0 / sinv[5];
+            // B-basis strain e6 of sym(Udot dt), then sig = Q (c o (Q^T e6))
+            double e6[6];
+            e6[0] = SRX_R2 * (e9.v[4] - e9.v[0]);
+            e6[1] = SRX_R6 * (2.0 * e9.v[8] - e9.v[0] - e9.v[4]);
+            e6[2] = 2.0 * SRX_R2 * e9.v[5];
+            e6[3] = 2.0 * SRX_R2 * e9.v[2];
+            e6[4] = 2.0 * SRX_R2 * e9.v[1];
+            e6[5] = SRX_R3 * (e9.v[0] + e9.v[4] + e9.v[8]);
+            double t[5];
+#pragma unroll
+            for (int k = 0; k < 5; ++k) {
+                double x = 0.0;
+#pragma unroll
+                for (int i = 0; i < 5; ++i) x += Q[i][k] * e6[i];
+                t[k] = x * c[k];
+            }
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                double x = 0.0;
+#pragma unroll
+                for (int k = 0; k < 5; ++k) x += Q[i][k] * t[k];
+                s6[i] = x;
+            }
+            s6[5] = e6[5] / sinv[5];
+            if (creep == 1) {
+                const double sc[6] = {SRX_R2 * (sc9.v[4] - sc9.v[0]), SRX_R6 * (2.0 * sc9.v[8] - sc9.v[0] - sc9.v[4]),
+                                      2.0 * SRX_R2 * sc9.v[5], 2.0 * SRX_R2 * sc9.v[2], 2.0 * SRX_R2 * sc9.v[1],
+                                      SRX_R3 * (sc9.v[0] + sc9.v[4] + sc9.v[8])};
+#pragma unroll
+                for (int i = 0; i < 6; ++i) s6[i] += sc[i];
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) {
+            f.tran[k * N + v] = a[k];
+            f.tran0[k * N + v] = a[k];
+        }
+#pragma unroll
+        for (int i = 0; i < 6; ++i) {
+            f.sig[i * N + v] = s6[i];
+            acc[36 + i] = s6[i];
+        }
+    }
+    // block sums in a fixed order: wavefronts, then the four wavefront sums in LDS
+    __shared__ double red[SETUP_BLOCK / 64][SETUP_NPART];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < SETUP_NPART; ++k) {
+        const double x = wave_sum(acc[k]);
+        if (lane == 0) red[w][k] = x;
+    }
+    __syncthreads();
+    for (int k = threadIdx.x; k < SETUP_NPART; k += SETUP_BLOCK) {
+        double x = 0.0;
+        for (int j = 0; j < SETUP_BLOCK / 64; ++j) x += red[j][k];
+        partials[(size_t)k * gridDim.x + blockIdx.x] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // host launch wrappers
 // ---------------------------------------------------------------------------
 static int min_radix(const FftPlan &p) {
@@ -2195,6 +2305,21 @@ void launch_mlp_record(hipStream_t s, const DevConst *cst, const Fields &f, int 
         hipLaunchKernelGGL(k_mlp_rec_adv, dim3(1), dim3(64), 0, s, count, cap, nsamp, ctl);
     }
     check_launch("mlp_record");
+}
+
+int setup_block_count(int nvox) { return (nvox + SETUP_BLOCK - 1) / SETUP_BLOCK; }
+
+void launch_setup(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, const double *eul, double deg,
+                  const double *e9, const double *sc9, int creep, double *partials, double *out) {
+    Vec9 e, sc;
+    for (int k = 0; k < 9; ++k) {
+        e.v[k] = e9[k];
+        sc.v[k] = sc9[k];
+    }
+    const int nb = setup_block_count(nvox);
+    hipLaunchKernelGGL(k_setup, dim3(nb), dim3(SETUP_BLOCK), 0, s, cst, f, nvox, eul, deg, e, sc, creep, partials);
+    check_launch("setup");
+    launch_reduce(s, partials, nb, SETUP_NPART, out, nullptr);
 }
 
 int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }

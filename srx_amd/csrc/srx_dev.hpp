@@ -228,6 +228,13 @@ void mlp_pack(hipStream_t s, const float *dblob, MlpPacked *M);
 size_t mlp_packed_bytes();
 void launch_mlp_record(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int stride, int after,
                        float *X, float *T, float *ND, int *count, int cap, const IterCtl *ctl);
+// Device setup: TranMat (and its initial copy) from Bunge angles in degrees (times deg = pi/180),
+// the initial stress from e9 = Udot dt (+ sc9 for creep runs), and out[0..36) = sum of C_gr
+// (B basis, row-major 6x6), out[36..42) = sum of the initial stress.  partials: 42 x
+// setup_block_count(nvox) doubles.
+int setup_block_count(int nvox);
+void launch_setup(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, const double *eul, double deg,
+                  const double *e9, const double *sc9, int creep, double *partials, double *out);
 int nr_block_count(int nvox);
 int end_block_count(int nvox);
 // Field conversion helpers used by srx_get_field (device -> AoS tensor on device scratch)

@@ -212,3 +212,20 @@ def test_wavefront_green_pass_matches_oracle(tmp_path, oracle_lib, n):
         assert rg["nr_iters_total"] == ro["nr_iters_total"]
         compare_reports(rg, ro, 1e-10)
         compare_fields(g, o, 1e-9)
+
+
+@pytest.mark.parametrize("creep", [False, True])
+def test_device_setup_matches_oracle(tmp_path, oracle_lib, creep):
+    """Orientations, the initial stress C_gr : (Udot dt) (plus the creep stress) and the averaged
+    stiffness, computed on the device at srx_open, against the oracle's setup (init.c:233-301,
+    444-516) before any increment."""
+    kw = dict(extra={"VelGrad_BC_Flag": "0 0 0 0 0 0", "VelGrad_BC": "0 0 0 0 0 0",
+                     "Stress_BC_Flag": "1 1 1 1 1 1", "Stress_BC": "0. 0. 50. 0. 0. 0."}) if creep else {}
+    deck = make(tmp_path, n=(12, 10, 8), grains=5, seed=16, nsteps=1, **kw)
+    o = oracle_lib.Oracle(deck)
+    g = srx_amd.Solver(deck)
+    for name in ("tranmat", "sig"):
+        a, b = g.field(name), o.field(name)
+        assert np.abs(a - b).max() <= 1e-13 * max(np.abs(b).max(), 1.0), name
+    g.close()
+    o.close()
