@@ -1030,25 +1030,27 @@ __device__ __forceinline__ void sys_const(const DevPhase &P, double rP, double r
 // below 3e-18), e^x / 2 = 2^(k-1) (C + S), e^-x / 2 = 2^(-k-1) (C - S).  For k = 0 the series are
 // the result, so small arguments lose nothing to cancellation.  Accurate to a few ulp; the two
 // short series replace one long exp polynomial and a reciprocal.
+// Series coefficients of cosh (1/12! .. 1/2!, 1) and sinh (1/13! .. 1/3!, 1) read from memory: the
+// uniform loads land in scalar registers, which the three-operand FMA takes as its addend directly
+// (a literal addend is first copied into a vector register at every use).
+__constant__ double k_sinh_coef[14] = {2.08767569878680989792e-09, 2.75573192239858906526e-07,
+                                       2.48015873015873015873e-05, 1.38888888888888888889e-03,
+                                       4.16666666666666666667e-02, 0.5, 1.0,
+                                       1.60590438368216145994e-10, 2.50521083854417187751e-08,
+                                       2.75573192239858906526e-06, 1.98412698412698412698e-04,
+                                       8.33333333333333333333e-03, 1.66666666666666666667e-01, 1.0};
 __device__ __forceinline__ void sinh_cosh(double x, double &sh, double &ch) {
     const double kf = rint(x * 1.4426950408889634);
     double r = fma(-kf, 6.93147180369123816490e-01, x);
     r = fma(-kf, 1.90821492927058770002e-10, r);
     const double r2 = r * r;
-    double c = 2.08767569878680989792e-09;            // 1/12!
-    c = fma(c, r2, 2.75573192239858906526e-07);       // 1/10!
-    c = fma(c, r2, 2.48015873015873015873e-05);       // 1/8!
-    c = fma(c, r2, 1.38888888888888888889e-03);       // 1/6!
-    c = fma(c, r2, 4.16666666666666666667e-02);       // 1/4!
-    c = fma(c, r2, 0.5);
-    c = fma(c, r2, 1.0);
-    double o = 1.60590438368216145994e-10;            // 1/13!
-    o = fma(o, r2, 2.50521083854417187751e-08);       // 1/11!
-    o = fma(o, r2, 2.75573192239858906526e-06);       // 1/9!
-    o = fma(o, r2, 1.98412698412698412698e-04);       // 1/7!
-    o = fma(o, r2, 8.33333333333333333333e-03);       // 1/5!
-    o = fma(o, r2, 1.66666666666666666667e-01);       // 1/3!
-    o = fma(o, r2, 1.0);
+    const double *K = k_sinh_coef;
+    double c = K[0];
+#pragma unroll
+    for (int i = 1; i < 7; ++i) c = fma(c, r2, K[i]);
+    double o = K[7];
+#pragma unroll
+    for (int i = 8; i < 14; ++i) o = fma(o, r2, K[i]);
     o *= r;
     const int k = (int)kf;
     const double ep = ldexp(c + o, k - 1), em = ldexp(c - o, -k - 1);   // e^x / 2, e^-x / 2
