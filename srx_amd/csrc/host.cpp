@@ -330,7 +330,7 @@ struct srx_ctx {
     void *specB = nullptr, *specC = nullptr;
     void *tw[3] = {nullptr, nullptr, nullptr};
     FftPlan pl[3];
-    double *partials = nullptr, *red = nullptr, *hred = nullptr;
+    double *partials = nullptr, *red = nullptr, *hred = nullptr, *redseg = nullptr;
     IterCtl *dctl = nullptr, *hctl = nullptr;   // outer-iteration control block (device, pinned host copy)
     int iters_last = 0;                        // outer iterations of the last increment
     int nr_blocks = 0, end_blocks = 0;
@@ -646,6 +646,7 @@ int setup(srx_ctx *c) {
     const int pb = std::max(std::max(c->nr_blocks * NR_NPART, c->end_blocks * END_NPART), setup_block_count((int)N) * 42);
     HIPCK(hipMalloc(&c->partials, sizeof(double) * pb));
     HIPCK(hipMalloc(&c->red, sizeof(double) * 64));
+    HIPCK(hipMalloc(&c->redseg, sizeof(double) * NR_RED_SEG_DOUBLES));
     HIPCK(hipHostMalloc((void **)&c->hred, sizeof(double) * 64, hipHostMallocDefault));
     // microstructure and InitFields on the device: orientations, initial stress, and the grid sums
     // of C_gr and Sig
@@ -1134,8 +1135,8 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
             if (rec)
                 launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 1, c->rec_x, c->rec_t, c->rec_nd,
                                   c->rec_count, c->rec_cap, g);
-            { Timer t(c, 7, launched); launch_reduce(c->stream, c->partials, c->nr_blocks, NR_NPART, c->red, g); }
-            launch_control(c->stream, c->dc, c->red, g);
+            { Timer t(c, 7, launched); launch_reduce_nr(c->stream, c->partials, c->nr_blocks, c->redseg, g); }
+            launch_control(c->stream, c->dc, c->redseg, g);
         }
         if (gx) {
             hipGraph_t graph = nullptr;
@@ -1483,7 +1484,7 @@ void srx_close(srx_ctx *c) {
     double *ptrs[] = {c->f.sig, c->f.sig_pre, c->f.eps, c->f.edot, c->f.dg, c->f.dg0, c->f.tran, c->f.tran0,
                       c->f.rho_s, c->f.rho_m, c->f.rho_m_old, c->f.rho_F, c->f.rho_P, c->f.gamdot, c->f.gam_acum,
                       c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->f.rho_tot, c->f.fl0,
-                      c->partials, c->red};
+                      c->partials, c->red, c->redseg};
     for (double *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->f.dsig) (void)hipFree(c->f.dsig);

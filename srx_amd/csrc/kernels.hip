@@ -1649,14 +1649,59 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__
     (void)nvals;
 }
 
+// First stage of the per-iteration reduction: block (k, sg) reduces value k over the sg-th of
+// RED_SEGS contiguous ranges of block partials, in a fixed order (strided accumulators, then the
+// wavefronts, then the four wavefront results), into seg[k * RED_SEGS + sg].  k_control finishes.
+constexpr int RED_SEGS = 16, RED_SEG_BLOCK = 256;
+static_assert(NR_RED_SEG_DOUBLES == NR_NPART * RED_SEGS, "segment buffer size");
+__global__ void __launch_bounds__(RED_SEG_BLOCK) k_reduce_seg(const double *__restrict__ part, int nblocks,
+                                                              unsigned long long maxmask, double *__restrict__ seg,
+                                                              const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    const int k = blockIdx.x, sg = blockIdx.y;
+    const bool ismax = (maxmask >> k) & 1ull;
+    const int chunk = (nblocks + RED_SEGS - 1) / RED_SEGS;
+    const int b1 = min(nblocks, (sg + 1) * chunk);
+    const double *row = part + (size_t)k * nblocks;
+    const double init = ismax ? -1e308 : 0.0;
+    double acc[4] = {init, init, init, init};
+    for (int b0 = sg * chunk + threadIdx.x; b0 < b1; b0 += 4 * RED_SEG_BLOCK) {
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int b = b0 + u * RED_SEG_BLOCK;
+            if (b < b1) acc[u] = ismax ? fmax(acc[u], row[b]) : acc[u] + row[b];
+        }
+    }
+    double a = ismax ? fmax(fmax(acc[0], acc[1]), fmax(acc[2], acc[3])) : (acc[0] + acc[1]) + (acc[2] + acc[3]);
+    a = ismax ? wave_max(a) : wave_sum(a);
+    __shared__ double sh[RED_SEG_BLOCK / 64];
+    if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = a;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double r = sh[0];
+        for (int w = 1; w < RED_SEG_BLOCK / 64; ++w) r = ismax ? fmax(r, sh[w]) : r + sh[w];
+        seg[k * RED_SEGS + sg] = r;
+    }
+}
+
 // One outer iteration's bookkeeping on the device (the host loop of evolution.c:1150-1256):
 // the dDisGradAvg this iteration's inverse-z pass applied joins the uniform part, the reduced
 // sums give the errors and get_smacro's averages and correction, and `done` decides whether the
 // next iteration runs.  One thread; the iteration's other kernels have finished (stream order).
-__global__ void k_control(const DevConst *__restrict__ cst, const double *__restrict__ red, IterCtl *ctl) {
+__global__ void k_control(const DevConst *__restrict__ cst, const double *__restrict__ seg, IterCtl *ctl) {
+    if (ctl->done) return;
+    // second stage of the reduction: the RED_SEGS partial results of each value, in order
+    __shared__ double red[NR_NPART];
+    if (threadIdx.x < NR_NPART) {
+        const int k = threadIdx.x;
+        const bool ismax = k == 15;
+        double r = seg[k * RED_SEGS];
+        for (int sg = 1; sg < RED_SEGS; ++sg) r = ismax ? fmax(r, seg[k * RED_SEGS + sg]) : r + seg[k * RED_SEGS + sg];
+        red[k] = r;
+    }
+    __syncthreads();
     if (threadIdx.x != 0) return;
     IterCtl &c = *ctl;
-    if (c.done) return;
     for (int k = 0; k < 9; ++k) c.uniform[k] += c.ddg[k];
     c.iter += 1;
     if (red[16] > 0) {   // a non-finite stress (the reference asserts, constitutive.c:683)
@@ -2355,8 +2400,14 @@ void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals
     check_launch("reduce");
 }
 
-void launch_control(hipStream_t s, const DevConst *cst, const double *red, IterCtl *ctl) {
-    SRX_LAUNCH(k_control, dim3(1), dim3(64), 0, s, cst, red, ctl);
+void launch_reduce_nr(hipStream_t s, const double *partials, int nblocks, double *seg, const IterCtl *ctl) {
+    SRX_LAUNCH(k_reduce_seg, dim3(NR_NPART, RED_SEGS), dim3(RED_SEG_BLOCK), 0, s, partials, nblocks, 1ull << 15, seg,
+               ctl);
+    check_launch("reduce");
+}
+
+void launch_control(hipStream_t s, const DevConst *cst, const double *seg, IterCtl *ctl) {
+    SRX_LAUNCH(k_control, dim3(1), dim3(64), 0, s, cst, seg, ctl);
     check_launch("control");
 }
 

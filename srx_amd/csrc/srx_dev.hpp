@@ -216,7 +216,12 @@ void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, in
                double *partials, const IterCtl *ctl = nullptr);
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out,
                    const IterCtl *ctl = nullptr);
-void launch_control(hipStream_t s, const DevConst *cst, const double *red, IterCtl *ctl);
+// The per-iteration reduction of the Newton kernel's block partials in two fixed-order stages:
+// launch_reduce_nr writes NR_NPART x 16 segment results to seg, launch_control sums them and does
+// the iteration's bookkeeping (get_smacro and the convergence test).
+constexpr int NR_RED_SEG_DOUBLES = NR_NPART * 16;
+void launch_reduce_nr(hipStream_t s, const double *partials, int nblocks, double *seg, const IterCtl *ctl);
+void launch_control(hipStream_t s, const DevConst *cst, const double *seg, IterCtl *ctl);
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9,
                      const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,
