@@ -1025,11 +1025,13 @@ __device__ __forceinline__ void sys_const(const DevPhase &P, double rP, double r
     g0 = P.k_g0 * rmn * rsF;
 }
 
-// sinh(x) and cosh(x) for x >= 0 from one reduction x = k ln2 + r, |r| <= ln2/2 (Cody-Waite
-// split of ln2): C = cosh r and S = sinh r by their Taylor series (to r^12 and r^13, truncation
-// below 3e-18), e^x / 2 = 2^(k-1) (C + S), e^-x / 2 = 2^(-k-1) (C - S).  For k = 0 the series are
-// the result, so small arguments lose nothing to cancellation.  Accurate to a few ulp; the two
-// short series replace one long exp polynomial and a reciprocal.
+// sinh(x) and cosh(x) from one reduction x = k ln2 + r, |r| <= ln2/2 (Cody-Waite split of ln2):
+// C = cosh r and S = sinh r by their Taylor series (to r^12 and r^13, truncation below 3e-18), and
+// with e^x / 2 = 2^(k-1) (C + S), e^-x / 2 = 2^(-k-1) (C - S):
+//   sinh x = a S + b C,  cosh x = a C + b S,  a = 2^(k-1) + 2^(-k-1),  b = 2^(k-1) - 2^(-k-1).
+// For k = 0, a = 1 and b = 0 exactly, so small arguments are the series themselves (no
+// cancellation); for |k| >= 1 the two terms of sinh have the sign of x.  Accurate to a few ulp for
+// either sign of x; overflows to inf where e^|x| / 2 does.
 // Series coefficients of cosh (1/12! .. 1/2!, 1) and sinh (1/13! .. 1/3!, 1) read from memory: the
 // uniform loads land in scalar registers, which the three-operand FMA takes as its addend directly
 // (a literal addend is first copied into a vector register at every use).
@@ -1053,22 +1055,22 @@ __device__ __forceinline__ void sinh_cosh(double x, double &sh, double &ch) {
     for (int i = 8; i < 14; ++i) o = fma(o, r2, K[i]);
     o *= r;
     const int k = (int)kf;
-    const double ep = ldexp(c + o, k - 1), em = ldexp(c - o, -k - 1);   // e^x / 2, e^-x / 2
-    sh = k == 0 ? o : ep - em;
-    ch = k == 0 ? c : ep + em;
+    const double hp = ldexp(1.0, k - 1), hm = ldexp(1.0, -k - 1);
+    const double a = hp + hm, b = hp - hm;
+    sh = fma(a, o, b * c);
+    ch = fma(a, c, b * o);
 }
 
 // slip rate and its derivative, FORWARD_PLUS_BACKWARD (constitutive.c:634-684):
-//   g  = g0 sinh((|rss|-tp)/(tau0+tau_cut)) sign(rss),  g2 = g0 cosh(..)/(tau0+tau_cut), for |rss| > tp
-// Branch free: inactive systems evaluate at 0 and are masked.
-__device__ __forceinline__ void slip_rate(double rss, double tp, double inv, double g0, double &g, double &g2) {
-    const double ar = fabs(rss);
-    const bool act = ar > tp;
+//   g  = G sinh((|rss|-tp)/(tau0+tau_cut)) sign(rss),  g2 = G cosh(..)/(tau0+tau_cut), for |rss| > tp
+// with inv = 1/(tau0+tau_cut) and e = |rss| inv - tp inv (one FMA: its sign is the activity test and
+// it is the argument).  The caller folds any factor common to g and g2 into G (the Newton solve
+// folds in the time step).  Branch free: inactive systems evaluate at 0 (sinh 0 = 0) and g2 is masked.
+__device__ __forceinline__ void slip_rate(double rss, double e, double inv, double G, double &g, double &g2) {
     double sh, ch;
-    sinh_cosh(act ? (ar - tp) * inv : 0.0, sh, ch);
-    const double gg = act ? g0 * sh : 0.0;
-    g = (rss > 0) ? gg : -gg;
-    g2 = act ? g0 * ch * inv : 0.0;
+    sinh_cosh(copysign(fmax(e, 0.0), rss), sh, ch);
+    g = G * sh;
+    g2 = (e > 0.0 ? G * inv : 0.0) * ch;
 }
 
 // ---------------------------------------------------------------------------
@@ -1220,8 +1222,9 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     const double rAh = 1.0 / Ah;
     const double r0h = qh - Shh * lamh;
 
-    // per-system constants, kept in LDS (lane-major, conflict free); the first call of an
-    // increment uses the lagged rho_m in tau_pass
+    // per-system constants, kept in LDS (lane-major, conflict free): tau_pass * inv, inv and
+    // gamma_0 * dt (the slip rates enter the residual and the Jacobian times dt); the first call of
+    // an increment uses the lagged rho_m in tau_pass
     double *tp = lds + 0 * NSYS * NR_BLOCK + threadIdx.x, *inv = lds + 1 * NSYS * NR_BLOCK + threadIdx.x;
     double *g0 = lds + 2 * NSYS * NR_BLOCK + threadIdx.x, *tpo = lds + 3 * NSYS * NR_BLOCK + threadIdx.x;
     {
@@ -1237,12 +1240,12 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
                 const double rmn = rho_mobile(P, rP, rF);
                 double t, iv, g;
                 sys_const(P, rP, rF, rmn, rmn, t, iv, g);
-                tp[s * NR_BLOCK] = t;
+                tp[s * NR_BLOCK] = t * iv;
                 inv[s * NR_BLOCK] = iv;
-                g0[s * NR_BLOCK] = g;
+                g0[s * NR_BLOCK] = g * dt;
                 if (FIRST) {
                     const double rmo = f.rho_m[s * N + v];
-                    tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo));
+                    tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo)) * iv;
                     f.rho_m_old[s * N + v] = rmo;
                     f.rho_m[s * N + v] = rmn;
                 }
@@ -1264,7 +1267,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         for (int i = 0; i < 5; ++i) ws[i] = (double)wsp[i];
         wsh = (double)wsp[5];
     }
-    double sg[5], sh = lamh + wsh, pre[5], preh = sh, ed[5];
+    double sg[5], sh = lamh + wsh, pre[5], preh = sh, ed[5];   // ed: edot dt of the iterate
 #pragma unroll
     for (int i = 0; i < 5; ++i) { sg[i] = lam[i] + ws[i]; pre[i] = sg[i]; ed[i] = 0.0; }
     int it = 0, itot = 0;
@@ -1295,11 +1298,11 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         for (int s = 0; s < NSYS; ++s) {
             const double *p = P.p[s];
             const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
-            const double tps = tpu[s * NR_BLOCK];
-            if (__any(fabs(rss) > tps)) {
-                double g, g2;
-                slip_rate(rss, tps, inv[s * NR_BLOCK], g0[s * NR_BLOCK], g, g2);
-                const double g2dt = g2 * dt;
+            const double iv = inv[s * NR_BLOCK];
+            const double e = fma(fabs(rss), iv, -tpu[s * NR_BLOCK]);
+            if (__any(e > 0.0)) {
+                double g, g2dt;
+                slip_rate(rss, e, iv, g0[s * NR_BLOCK], g, g2dt);
 #pragma unroll
                 for (int i = 0; i < 5; ++i) {
                     ed[i] += p[i] * g;
@@ -1312,7 +1315,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         double R[5];
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            double acc = r0[i] + ed[i] * dt;
+            double acc = r0[i] + ed[i];
 #pragma unroll
             for (int j = 0; j < 5; ++j) acc += A[ps(i, j)] * sg[j];
             R[i] = acc;
@@ -1413,7 +1416,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         double ce2 = 0.0;
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            const double acc = q[i] + ed[i] * dt + spp[i];
+            const double acc = q[i] + ed[i] + spp[i];
             ce2 += acc * acc;
         }
         const double eh = qh + P.sinv[5] * preh;
@@ -1813,7 +1816,7 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             const double *p = P.p[s];
             const double rss = p[0] * pre[0] + p[1] * pre[1] + p[2] * pre[2] + p[3] * pre[3] + p[4] * pre[4];
             double g, g2;
-            slip_rate(rss, tp, inv, g0, g, g2);
+            slip_rate(rss, fma(fabs(rss), inv, -(tp * inv)), inv, g0, g, g2);
             bad |= !isfinite(g);
 #pragma unroll
             for (int i = 0; i < 5; ++i) edc[i] += p[i] * g;
