@@ -66,7 +66,15 @@ typedef struct {
                                 results differ from the reference within that tolerance).
                                 2: the start comes from the learned network installed by srx_mlp_set
                                 (same convergence test; no counterpart in the reference) */
-    int    reserved[8];      /* [0]: kernel timing (srx_kernel_times); [1] = 1: enqueue the outer
+    int    gnd;              /* 1: geometrically necessary dislocations, as a build of the reference with
+                                DD_GND (evpFlags.h, which then also selects DD_GND_FIRST_ORDER): after each
+                                increment the grain-boundary-aware gradient of the slip rates
+                                (Gradient_ShearRate, constitutive.c:882-1115) updates the screw and edge-I
+                                GND densities (constitutive.c:1428-1519, deck key L0), which enter rho_F,
+                                rho_P (constitutive.c:565-576) and rho_tot (evolution.c:1040-1046);
+                                read them with SRX_F_RHO_G1 / SRX_F_RHO_G2. Needs warm_start = 0 (else
+                                srx_open returns SRX_E_ARG). 0 (default): the shipped build */
+    int    reserved[7];      /* [0]: kernel timing (srx_kernel_times); [1] = 1: enqueue the outer
                                 iterations launch by launch instead of replaying each batch of 8
                                 from a HIP graph captured on first use (the default); [2] = 1: keep
                                 the default FFT tiles instead of timing the candidates at srx_open */
@@ -111,7 +119,9 @@ enum {
     SRX_F_FLUCT = 16,     /* [9] accumulated strain fluctuation (evolution.c:1198) */
     SRX_F_TRIAL_GAM_ACUM = 17, /* [12] */
     SRX_F_RHO_TOT = 18,   /* [1] rho_tot used by the last hardening update */
-    SRX_F_PHASE = 19, SRX_F_GRAIN = 20   /* [1] as doubles */
+    SRX_F_PHASE = 19, SRX_F_GRAIN = 20,  /* [1] as doubles */
+    SRX_F_RHO_G1 = 21, SRX_F_RHO_G2 = 22 /* [12] GND densities rho_g1 (screw), rho_g2 (edge-I) (evp.h:349);
+                                            gnd option only, read-only */
 };
 
 /* Replaces GetNameList + SetupJob (io.c:96-164, init.c:728-1216) and

@@ -21,7 +21,7 @@ class Options(C.Structure):
         ("itermax_override", C.c_int), ("nsteps_override", C.c_int), ("tau0_mode", C.c_int),
         ("tau0_value", C.c_double), ("svm_mode", C.c_int), ("svm_stray", C.c_double),
         ("write_files", C.c_int), ("write_fields", C.c_int), ("nthreads", C.c_int),
-        ("verbose", C.c_int), ("warm_start", C.c_int), ("reserved", C.c_int * 8),
+        ("verbose", C.c_int), ("warm_start", C.c_int), ("gnd", C.c_int), ("reserved", C.c_int * 7),
     ]
 
 
@@ -52,7 +52,7 @@ FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velg
           "tranmat": (5, 9), "rho_s": (6, 12), "rho_m": (7, 12), "rho_F": (8, 12), "rho_P": (9, 12),
           "gamdot": (10, 12), "gam_acum": (11, 12), "rho_dipole": (12, 12), "rho_forest": (13, 12),
           "rho_inplane": (14, 12), "nr_iters": (15, 1), "fluct": (16, 9), "trial_gam_acum": (17, 12),
-          "rho_tot": (18, 1), "phase": (19, 1), "grain": (20, 1)}
+          "rho_tot": (18, 1), "phase": (19, 1), "grain": (20, 1), "rho_g1": (21, 12), "rho_g2": (22, 12)}
 
 _lib = None
 

@@ -368,6 +368,9 @@ typedef struct {
     double bburg, tau0xf;
     double c1, c2, c3, c4, c5, c6, c7, c8, q_slip, q_bulk;
     double chi[NSYS][NSYS], chi1[NSYS][NSYS];
+    /* projections of the GND densities (init.c:1118-1141) and bp = bb x bn (init.c:200-202) */
+    double cos_g1[NSYS][NSYS], sin_g1[NSYS][NSYS], cos_g2[NSYS][NSYS], sin_g2[NSYS][NSYS];
+    double cos_g3[NSYS][NSYS], sin_g3[NSYS][NSYS], bp[NSYS][3];
 } orc_phase;
 
 static void resolve(char *out, const char *dir, const char *name)
@@ -479,6 +482,9 @@ struct orc_state {
     double *rho_s, *rho_m, *rho_F, *rho_P, *trial_rho_s, *gamdot, *gam_acum, *trial_gam_acum;
     double *rho_dipole, *rho_forest, *rho_inplane;          /* 12 each */
     double *rho_tot;
+    double *rho_g1, *rho_g2, *rho_g3;   /* GND densities, 12 each (gnd option; rho_g3 stays 0 in first order) */
+    double *gadot_grad;                 /* 36: gradient of gamdot per system (constitutive.c:882-1115) */
+    unsigned char *gbc;                 /* GB_checkX/Y/Z[0..1] as bits 0..5 (constitutive.c:731-880) */
     float *dsig;      /* warm_start 1: last NR call's stress change, B basis (float, as the GPU keeps it) */
     int ws_use;       /* warm_start 1: this sweep starts from the extrapolated stress */
     double *nrit;
@@ -544,6 +550,66 @@ static void update_schmid(orc_state *s)   /* kinematics.c:430-463 */
     }
 }
 
+/* Gradient_ExchangeGrainID (constitutive.c:731-880), one rank: periodic neighbours.  Bits 0..5 are
+   GB_checkX[0], GB_checkX[1], GB_checkY[0], GB_checkY[1], GB_checkZ[0], GB_checkZ[1] (1: the +/-
+   neighbour is in the same grain).  GB_checkZ[1] compares with the y- neighbour, as the reference
+   does (823, 842, 860). */
+static void gradient_exchange_grain_id(orc_state *s)
+{
+    const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
+    long p;
+    for (p = 0; p < s->nxyz; p++) {
+        int x = (int)(p / ((long)n1 * n2)), y = (int)((p / n2) % n1), z = (int)(p % n2);
+        int xp = x + 1 >= n0 ? 0 : x + 1, xm = x - 1 < 0 ? n0 - 1 : x - 1;
+        int yp = y + 1 >= n1 ? 0 : y + 1, ym = y - 1 < 0 ? n1 - 1 : y - 1;
+        int zp = z + 1 >= n2 ? 0 : z + 1;
+        int g = s->grain[p];
+        unsigned char b = 0;
+        b |= (g == s->grain[((long)xp * n1 + y) * n2 + z]) << 0;
+        b |= (g == s->grain[((long)xm * n1 + y) * n2 + z]) << 1;
+        b |= (g == s->grain[((long)x * n1 + yp) * n2 + z]) << 2;
+        b |= (g == s->grain[((long)x * n1 + ym) * n2 + z]) << 3;
+        b |= (g == s->grain[((long)x * n1 + y) * n2 + zp]) << 4;
+        b |= (g == s->grain[((long)x * n1 + ym) * n2 + z]) << 5;
+        s->gbc[p] = b;
+    }
+}
+
+/* one-sided or central difference inside the grain (constitutive.c:970-1010): c0, c1 = same grain on
+   the + / - side */
+static double gb_diff(int c0, int c1, double v, double vp, double vm)
+{
+    if (!c0 && c1) return v - vm;
+    if (c0 && !c1) return vp - v;
+    if (c0 && c1) return (vp - vm) / 2.0;
+    return 0.0;
+}
+
+/* Gradient_ShearRate (constitutive.c:882-1115), one rank */
+static void gradient_shear_rate(orc_state *s)
+{
+    const int n0 = s->n[0], n1 = s->n[1], n2 = s->n[2];
+    long p;
+    #pragma omp parallel for schedule(static)
+    for (p = 0; p < s->nxyz; p++) {
+        int x = (int)(p / ((long)n1 * n2)), y = (int)((p / n2) % n1), z = (int)(p % n2), i;
+        int xp = x + 1 >= n0 ? 0 : x + 1, xm = x - 1 < 0 ? n0 - 1 : x - 1;
+        int yp = y + 1 >= n1 ? 0 : y + 1, ym = y - 1 < 0 ? n1 - 1 : y - 1;
+        int zp = z + 1 >= n2 ? 0 : z + 1, zm = z - 1 < 0 ? n2 - 1 : z - 1;
+        long pxp = ((long)xp * n1 + y) * n2 + z, pxm = ((long)xm * n1 + y) * n2 + z;
+        long pyp = ((long)x * n1 + yp) * n2 + z, pym = ((long)x * n1 + ym) * n2 + z;
+        long pzp = ((long)x * n1 + y) * n2 + zp, pzm = ((long)x * n1 + y) * n2 + zm;
+        const unsigned char b = s->gbc[p];
+        for (i = 0; i < NSYS; i++) {
+            const double *gd = s->gamdot;
+            double *g = s->gadot_grad + 36 * p + 3 * i;
+            g[0] = gb_diff(b & 1, (b >> 1) & 1, gd[12 * p + i], gd[12 * pxp + i], gd[12 * pxm + i]);
+            g[1] = gb_diff((b >> 2) & 1, (b >> 3) & 1, gd[12 * p + i], gd[12 * pyp + i], gd[12 * pym + i]);
+            g[2] = gb_diff((b >> 4) & 1, (b >> 5) & 1, gd[12 * p + i], gd[12 * pzp + i], gd[12 * pzm + i]);
+        }
+    }
+}
+
 static int setup(orc_state *s)
 {
     orc_deck *d = &s->dk;
@@ -603,6 +669,28 @@ static int setup(orc_state *s)
         interaction_matrix(0.0, 0.0, d->CrossSlip0[ph], d->GlissileJunction0[ph], d->HirthLock0[ph],
                            d->LomerCottrellLock0[ph], s->ph[ph].chi1);
     }
+    for (ph = 0; ph < NPH; ph++) {   /* init.c:1118-1141 (sin and cos of the GND projections), 200-202 (bp) */
+        orc_phase *P = &s->ph[ph];
+        int i, j;
+        for (j = 0; j < P->nsys; j++) {
+            P->bp[j][0] = P->bb[j][1] * P->bn[j][2] - P->bb[j][2] * P->bn[j][1];
+            P->bp[j][1] = P->bb[j][2] * P->bn[j][0] - P->bb[j][0] * P->bn[j][2];
+            P->bp[j][2] = P->bb[j][0] * P->bn[j][1] - P->bb[j][1] * P->bn[j][0];
+        }
+        for (i = 0; i < P->nsys; i++) for (j = 0; j < P->nsys; j++) {
+            double sv[3], cs;
+            sv[0] = P->bn[j][1] * P->bb[j][2] - P->bn[j][2] * P->bb[j][1];
+            sv[1] = P->bn[j][2] * P->bb[j][0] - P->bn[j][0] * P->bb[j][2];
+            sv[2] = P->bn[j][0] * P->bb[j][1] - P->bn[j][1] * P->bb[j][0];
+            cs = fabs(P->bn[i][0] * sv[0] + P->bn[i][1] * sv[1] + P->bn[i][2] * sv[2]);
+            P->cos_g1[i][j] = fabs(P->bn[i][0] * P->bb[j][0] + P->bn[i][1] * P->bb[j][1] + P->bn[i][2] * P->bb[j][2]);
+            P->sin_g1[i][j] = fabs(sqrt(fabs(1.0 - P->cos_g1[i][j] * P->cos_g1[i][j])));
+            P->cos_g2[i][j] = cs;
+            P->sin_g2[i][j] = fabs(sqrt(fabs(1.0 - cs * cs)));
+            P->cos_g3[i][j] = fabs(P->bn[i][0] * P->bn[j][0] + P->bn[i][1] * P->bn[j][1] + P->bn[i][2] * P->bn[j][2]);
+            P->sin_g3[i][j] = fabs(sqrt(fabs(1.0 - P->cos_g3[i][j] * P->cos_g3[i][j])));
+        }
+    }
 
     /* allocation */
 #define AL(x, n) s->x = (double *)calloc((size_t)(n) * s->nxyz, sizeof(double))
@@ -615,6 +703,11 @@ static int setup(orc_state *s)
         if (!s->dsig) return 2005;
     }
     AL(conv_s, 1); AL(conv_e, 1);
+    if (s->opt.gnd) {
+        AL(rho_g1, 12); AL(rho_g2, 12); AL(rho_g3, 12); AL(gadot_grad, 36);
+        s->gbc = (unsigned char *)calloc(s->nxyz, 1);
+        if (!s->rho_g1 || !s->rho_g2 || !s->rho_g3 || !s->gadot_grad || !s->gbc) return 2005;
+    }
 #undef AL
     s->phase = (int *)calloc(s->nxyz, sizeof(int));
     s->grain = (int *)calloc(s->nxyz, sizeof(int));
@@ -736,13 +829,14 @@ static int setup(orc_state *s)
     s->itermax = s->opt.itermax_override > 0 ? s->opt.itermax_override : d->IterMax;
     s->nsteps = s->opt.nsteps_override > 0 ? s->opt.nsteps_override : d->N_steps;
     if (s->opt.tau0_mode == 1) for (ph = 0; ph < NPH; ph++) s->ph[ph].tau0xf = s->opt.tau0_value;
+    if (s->opt.gnd) gradient_exchange_grain_id(s);
     return 0;
 }
 
 /* ------------------------------------------------------------------ */
 /* StrainRate_Orowan, FORWARD_PLUS_BACKWARD (constitutive.c:507-728)    */
-/* GND densities are identically zero (DD_GND off), so the GND terms of */
-/* 565-576 contribute exactly +0.                                       */
+/* Without the gnd option the GND densities are identically zero (DD_GND */
+/* off), so the GND terms of 565-576 contribute exactly +0.             */
 /* ------------------------------------------------------------------ */
 static void strain_rate(orc_state *s, long idx, int jph, const double *stress, double *edot, double d_edot[6][6])
 {
@@ -757,9 +851,19 @@ static void strain_rate(orc_state *s, long idx, int jph, const double *stress, d
     int i, j, k;
     for (i = 0; i < ns; i++) {
         double aF = 0.0, aP = 0.0;
-        for (j = 0; j < ns; j++) {
-            aF += P->chi1[i][j] * (rs[j] + 0.0 + 0.0 + 0.0);
-            aP += P->chi[i][j] * (rs[j] + 0.0 + 0.0 + 0.0);
+        if (s->opt.gnd) {
+            const double *g1 = s->rho_g1 + 12 * idx, *g2 = s->rho_g2 + 12 * idx, *g3 = s->rho_g3 + 12 * idx;
+            for (j = 0; j < ns; j++) {
+                aF += P->chi1[i][j] * (rs[j] + fabs(g1[j] * P->cos_g1[i][j]) + fabs(g2[j] * P->cos_g2[i][j]) +
+                                       fabs(g3[j] * P->cos_g3[i][j]));
+                aP += P->chi[i][j] * (rs[j] + fabs(g1[j] * P->sin_g1[i][j]) + fabs(g2[j] * P->sin_g2[i][j]) +
+                                      fabs(g3[j] * P->sin_g3[i][j]));
+            }
+        } else {
+            for (j = 0; j < ns; j++) {
+                aF += P->chi1[i][j] * (rs[j] + 0.0 + 0.0 + 0.0);
+                aP += P->chi[i][j] * (rs[j] + 0.0 + 0.0 + 0.0);
+            }
         }
         rF[i] = aF; rho_f[i] = aF; rP[i] = aP;
     }
@@ -1085,6 +1189,21 @@ static void dislocation_evolution(orc_state *s, int istep)
             }
             s->trial_rho_s[12 * p + i] = *rs;
             s->gam_acum[12 * p + i] = s->trial_gam_acum[12 * p + i];
+            if (s->opt.gnd) {
+                /* DD_GND with DD_GND_FIRST_ORDER (evpFlags.h): screw, edge-I, edge-II (1428-1519) */
+                const double *gg = s->gadot_grad + 36 * p + 3 * i;
+                const double L0 = s->dk.L0;
+                double gnd1, dot_GND;
+                gnd1 = gg[0] * P->bp[i][0] + gg[1] * P->bp[i][1] + gg[2] * P->bp[i][2];
+                dot_GND = gnd1 / bb / L0;
+                s->rho_g1[12 * p + i] += dot_GND * dt;
+                gnd1 = gg[0] * P->bb[i][0] + gg[1] * P->bb[i][1] + gg[2] * P->bb[i][2];
+                dot_GND = -1.0 * gnd1 / bb / L0;
+                s->rho_g2[12 * p + i] += dot_GND * dt;
+                gnd1 = 0.0;
+                dot_GND = gnd1;
+                s->rho_g3[12 * p + i] += dot_GND * dt;
+            }
         }
         (void)dot_SSD;
     }
@@ -1161,7 +1280,11 @@ int orc_step(orc_state *s, orc_report *rep)
         int jph = s->phase[p];
         double acc = 0.0;
         if (is_xtal(s, jph))
-            for (i = 0; i < s->ph[jph - 1].nsys; i++) acc += s->rho_s[12 * p + i] + s->rho_m[12 * p + i] + sqrt(pow(0.0, 2.0) + pow(0.0, 2.0) + pow(0.0, 2.0));
+            for (i = 0; i < s->ph[jph - 1].nsys; i++) {
+                const double g1 = s->opt.gnd ? s->rho_g1[12 * p + i] : 0.0, g2 = s->opt.gnd ? s->rho_g2[12 * p + i] : 0.0;
+                const double g3 = s->opt.gnd ? s->rho_g3[12 * p + i] : 0.0;
+                acc += s->rho_s[12 * p + i] + s->rho_m[12 * p + i] + sqrt(pow(g1, 2.0) + pow(g2, 2.0) + pow(g3, 2.0));
+            }
         s->rho_tot[p] = acc;
     }
     /* evolution.c:1115-1128 */
@@ -1265,12 +1388,25 @@ int orc_step(orc_state *s, orc_report *rep)
         memcpy(r.edot_avg, da, sizeof(da));
     }
     if (s->dk.Update_Flag == 1 && s->istep > 1) update_orient(s);
+    if (s->opt.gnd && s->dk.Hard_Flag == 1 && s->istep > 1) gradient_shear_rate(s);   /* evolution.c:1346-1353 */
     dislocation_evolution(s, s->istep);
     r.rho_F_avg = avg12(s, s->rho_F);
     r.rho_P_avg = avg12(s, s->rho_P);
     r.rho_s_avg = avg12(s, s->rho_s);
     r.rho_m_avg = avg12(s, s->rho_m);
     r.rho_g_avg = 0.0;
+    if (s->opt.gnd) {   /* AvgDislocation, GNDs (constitutive.c:1745-1757) */
+        double acc = 0.0;
+        for (p = 0; p < N; p++) {
+            int jph = s->phase[p];
+            if (!is_xtal(s, jph)) continue;
+            for (i = 0; i < s->ph[jph - 1].nsys; i++) {
+                const long q = 12 * p + i;
+                acc += sqrt(s->rho_g1[q] * s->rho_g1[q] + s->rho_g2[q] * s->rho_g2[q] + s->rho_g3[q] * s->rho_g3[q]);
+            }
+        }
+        r.rho_g_avg = acc * (1.0 / N);
+    }
     r.rho_dipole_avg = avg12(s, s->rho_dipole);
     r.rho_forest_avg = avg12(s, s->rho_forest);
     r.rho_inplane_avg = avg12(s, s->rho_inplane);
@@ -1338,6 +1474,8 @@ static double *field_ptr(orc_state *s, int f, int *nc)
     case ORC_F_FLUCT: *nc = 9; return s->fluct;
     case ORC_F_TRIAL_GAM_ACUM: *nc = 12; return s->trial_gam_acum;
     case ORC_F_RHO_TOT: *nc = 1; return s->rho_tot;
+    case ORC_F_RHO_G1: *nc = 12; return s->rho_g1;
+    case ORC_F_RHO_G2: *nc = 12; return s->rho_g2;
     default: return 0;
     }
 }
@@ -1407,5 +1545,6 @@ void orc_close(orc_state *s)
     free(s->rho_forest); free(s->rho_inplane); free(s->rho_tot); free(s->nrit); free(s->work); free(s->ksig);
     free(s->dsig);
     free(s->nvec); free(s->conv_s); free(s->conv_e); free(s->phase); free(s->grain);
+    free(s->rho_g1); free(s->rho_g2); free(s->rho_g3); free(s->gadot_grad); free(s->gbc);
     free(s);
 }

@@ -39,7 +39,8 @@ typedef struct {
     int    nthreads;       /* OpenMP threads (0 = default) */
     int    verbose;
     int    warm_start;     /* 1: extrapolated Newton start (no counterpart in the reference) */
-    int    reserved[8];
+    int    gnd;            /* 1: geometrically necessary dislocations as a DD_GND build (first order) */
+    int    reserved[7];
 } orc_options;
 
 /* Same layout as srx_step_report in include/srx.h. */
@@ -67,7 +68,7 @@ enum {
     ORC_F_GAMDOT = 10, ORC_F_GAM_ACUM = 11, ORC_F_RHO_DIPOLE = 12,
     ORC_F_RHO_FOREST = 13, ORC_F_RHO_INPLANE = 14, ORC_F_NR_ITERS = 15,
     ORC_F_FLUCT = 16, ORC_F_TRIAL_GAM_ACUM = 17, ORC_F_RHO_TOT = 18,
-    ORC_F_PHASE = 19, ORC_F_GRAIN = 20
+    ORC_F_PHASE = 19, ORC_F_GRAIN = 20, ORC_F_RHO_G1 = 21, ORC_F_RHO_G2 = 22
 };
 
 void orc_default_options(orc_options *o);

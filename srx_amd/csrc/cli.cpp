@@ -10,7 +10,7 @@
 
 int main(int argc, char **argv) {
     if (argc < 2) {
-        std::fprintf(stderr, "usage: %s <deck> [--mixed] [--no-fields] [--verbose]\n", argv[0]);
+        std::fprintf(stderr, "usage: %s <deck> [--mixed] [--no-fields] [--verbose] [--gnd]\n", argv[0]);
         return 1;
     }
     srx_options o;
@@ -21,6 +21,7 @@ int main(int argc, char **argv) {
         if (!std::strcmp(argv[i], "--mixed")) o.precision = SRX_PREC_MIXED;
         if (!std::strcmp(argv[i], "--no-fields")) o.write_fields = 0;
         if (!std::strcmp(argv[i], "--verbose")) o.verbose = 1;
+        if (!std::strcmp(argv[i], "--gnd")) o.gnd = 1;   // the GND path of a DD_GND build
     }
     srx_ctx *ctx = nullptr;
     int rc = srx_open(argv[1], ".", &o, &ctx);

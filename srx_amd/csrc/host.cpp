@@ -438,6 +438,8 @@ int setup(srx_ctx *c) {
     if (d.Update_Flag != 1) return SRX_E_UPDATE_FLAG;
     // istep % PrintControl[1] at evolution.c:1456 divides by zero for a period <= 0
     if (c->opt.write_fields && d.PrintControl[0] == 1 && d.PrintControl[1] <= 0) return SRX_E_ARG;
+    // the GND option runs with the reference's Newton start only
+    if (c->opt.gnd && c->opt.warm_start != 0) return SRX_E_ARG;
 
     // phases (init.c:815-856)
     std::memset(&c->hc, 0, sizeof(c->hc));
@@ -488,12 +490,36 @@ int setup(srx_ctx *c) {
         D.c1Gb3 = c1 * P.G * P.b * 1.0E-3;
         D.k_tcut = (kT / MY_J_IN_EV) / c2 / c3 / P.b / P.b * 0.1;
         D.k_g0 = 1E-3 * 1.0E10 * c2 * P.b * D.expmq;
+        // GND option: bp = bb x bn (init.c:200-202) and the projections of init.c:1118-1141
+        for (int j = 0; j < NSYS; ++j) {
+            const bool in = j < P.nsys;
+            D.bp[j][0] = in ? P.bb[j][1] * P.bn[j][2] - P.bb[j][2] * P.bn[j][1] : 0.0;
+            D.bp[j][1] = in ? P.bb[j][2] * P.bn[j][0] - P.bb[j][0] * P.bn[j][2] : 0.0;
+            D.bp[j][2] = in ? P.bb[j][0] * P.bn[j][1] - P.bb[j][1] * P.bn[j][0] : 0.0;
+        }
+        for (int i = 0; i < NSYS; ++i)
+            for (int j = 0; j < NSYS; ++j) {
+                double c1g = 0.0, c2g = 0.0;
+                if (i < P.nsys && j < P.nsys) {
+                    const double sv[3] = {P.bn[j][1] * P.bb[j][2] - P.bn[j][2] * P.bb[j][1],
+                                          P.bn[j][2] * P.bb[j][0] - P.bn[j][0] * P.bb[j][2],
+                                          P.bn[j][0] * P.bb[j][1] - P.bn[j][1] * P.bb[j][0]};
+                    c2g = std::fabs(P.bn[i][0] * sv[0] + P.bn[i][1] * sv[1] + P.bn[i][2] * sv[2]);
+                    c1g = std::fabs(P.bn[i][0] * P.bb[j][0] + P.bn[i][1] * P.bb[j][1] + P.bn[i][2] * P.bb[j][2]);
+                }
+                D.cos_g1[i][j] = c1g;
+                D.sin_g1[i][j] = std::fabs(std::sqrt(std::fabs(1.0 - c1g * c1g)));
+                D.cos_g2[i][j] = c2g;
+                D.sin_g2[i][j] = std::fabs(std::sqrt(std::fabs(1.0 - c2g * c2g)));
+            }
     }
     c->hc.dt = d.TimeStep;
     c->hc.nu = d.ElastConst_PhaseI[3];
     c->hc.T = d.T__K;
     c->hc.D0 = d.D_0;
     c->hc.elast_bc = d.ElastBC;
+    c->hc.gnd = c->opt.gnd ? 1 : 0;
+    c->hc.L0 = d.L0;
     for (int i = 0; i < 3; ++i) c->hc.n[i] = c->n[i];
 
     // microstructure (init.c:233-396)
@@ -632,6 +658,28 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.rho_inplane, 12, 0.0)) || (rc = dalloc(&F.rho_tot, 1, 0.0)) ||
         (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
+    if (c->opt.gnd) {
+        // GND densities start at zero (init.c:1077-1087); same-grain flags of Gradient_ExchangeGrainID
+        // (constitutive.c:731-880): +x, -x, +y, -y, +z and, as the reference compares, the y- neighbour
+        // for the z- side; periodic
+        if ((rc = dalloc(&c->hc.rho_g1, 12, 0.0)) || (rc = dalloc(&c->hc.rho_g2, 12, 0.0)) ||
+            (rc = dalloc(&c->hc.gnd_fp, 24, 0.0)))
+            return rc;
+        const int n0 = c->n[0], n1 = c->n[1], n2 = c->n[2];
+        std::vector<unsigned char> gb(N);
+        const std::vector<int> &gr = c->grain;
+        for (long v = 0; v < N; ++v) {
+            const int x = (int)(v / ((long)n1 * n2)), y = (int)((v / n2) % n1), z = (int)(v % n2);
+            const int xp = x + 1 >= n0 ? 0 : x + 1, xm = x - 1 < 0 ? n0 - 1 : x - 1;
+            const int yp = y + 1 >= n1 ? 0 : y + 1, ym = y - 1 < 0 ? n1 - 1 : y - 1, zp = z + 1 >= n2 ? 0 : z + 1;
+            auto at = [&](int i, int j, int k) { return gr[((long)i * n1 + j) * n2 + k]; };
+            const int g = gr[v];
+            gb[v] = (unsigned char)((g == at(xp, y, z)) | (g == at(xm, y, z)) << 1 | (g == at(x, yp, z)) << 2 |
+                                    (g == at(x, ym, z)) << 3 | (g == at(x, y, zp)) << 4 | (g == at(x, ym, z)) << 5);
+        }
+        HIPCK(hipMalloc(&c->hc.gbc, N));
+        if ((rc = ctx_copy(st, c->hc.gbc, gb.data(), N, hipMemcpyHostToDevice))) return rc;
+    }
     if (c->opt.warm_start == 1 || c->opt.warm_start == 2) {
         HIPCK(hipMalloc(&F.dsig, sizeof(float) * 6 * N));
         if ((rc = ctx_zero(st, F.dsig, sizeof(float) * 6 * N))) return rc;
@@ -1131,7 +1179,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
             if (rec)
                 launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 0, c->rec_x, c->rec_t, c->rec_nd,
                                   c->rec_count, c->rec_cap, g);
-            { Timer t(c, 5, launched); launch_nr(c->stream, c->dc, c->f, N, launched == 1, rotated, c->partials, g); }
+            { Timer t(c, 5, launched); launch_nr(c->stream, c->dc, c->f, N, launched == 1, rotated, c->partials, g, c->opt.gnd); }
             if (rec)
                 launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 1, c->rec_x, c->rec_t, c->rec_nd,
                                   c->rec_count, c->rec_cap, g);
@@ -1187,7 +1235,8 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     std::memcpy(r.velgrad_avg, vga, sizeof(vga));
     // Edot/Eps, update_orient (istep>1), DislocationEvolution, averages
     const int do_orient = (d.Update_Flag == 1 && c->istep > 1) ? 1 : 0;
-    { Timer t(c, 6); launch_end(c->stream, c->dc, c->f, N, iter == 1 ? 1 : 0, rotated, do_orient, 1, c->istep, c->partials); }
+    { Timer t(c, 6); launch_end(c->stream, c->dc, c->f, N, iter == 1 ? 1 : 0, rotated, do_orient, 1, c->istep, c->partials,
+                                      c->opt.gnd); }
     { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->end_blocks, END_NPART, c->red); }
     int rc = sync_read(c, END_NPART);
     if (rc) return rc;
@@ -1213,10 +1262,18 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     r.rho_P_avg = h[13] * inv;
     r.rho_s_avg = h[14] * inv;
     r.rho_m_avg = h[15] * inv;
-    r.rho_g_avg = 0.0;
+    r.rho_g_avg = 0.0;   // set below with the GND option
     r.rho_dipole_avg = h[16] * inv;
     r.rho_forest_avg = h[17] * inv;
     r.rho_inplane_avg = h[20] * inv;
+    if (c->opt.gnd) {
+        // GND densities: gradient of this increment's slip rates when Hard_Flag = 1 and istep > 1
+        // (evolution.c:1346-1353), then the GND branch of DislocationEvolution and its mean
+        launch_gnd(c->stream, c->dc, c->hc, c->f, (int)N, (d.Hard_Flag == 1 && c->istep > 1) ? 1 : 0, c->partials);
+        launch_reduce(c->stream, c->partials, gnd_block_count((int)N), 1, c->red);
+        if ((rc = sync_read(c, 1))) return rc;
+        r.rho_g_avg = c->hred[0] * inv;
+    }
     if (c->fdd) {
         std::fprintf(c->fdd, "%e\t%e\t%e\t%e\t%e\t%e\t%e\n", c->TimeTot, c->e_vm, r.rho_s_avg, r.rho_m_avg,
                      r.rho_dipole_avg, r.rho_inplane_avg, r.rho_forest_avg);
@@ -1345,6 +1402,7 @@ static int field_info(int id, int &nc) {
     case SRX_F_FLUCT: nc = 9; return 0;
     case SRX_F_RHO_S: case SRX_F_RHO_M: case SRX_F_RHO_F: case SRX_F_RHO_P: case SRX_F_GAMDOT: case SRX_F_GAM_ACUM:
     case SRX_F_RHO_DIPOLE: case SRX_F_RHO_FOREST: case SRX_F_RHO_INPLANE: case SRX_F_TRIAL_GAM_ACUM: nc = 12; return 0;
+    case SRX_F_RHO_G1: case SRX_F_RHO_G2: nc = 12; return 0;
     case SRX_F_NR_ITERS: case SRX_F_PHASE: case SRX_F_GRAIN: case SRX_F_RHO_TOT: nc = 1; return 0;
     default: return SRX_E_FIELD;
     }
@@ -1396,7 +1454,9 @@ int srx_get_field(srx_ctx *c, int id, double *out, size_t n) {
         const double *src = id == SRX_F_RHO_S ? c->f.rho_s : id == SRX_F_RHO_M ? c->f.rho_m : id == SRX_F_RHO_F ? c->f.rho_F
                           : id == SRX_F_RHO_P ? c->f.rho_P : id == SRX_F_GAMDOT ? c->f.gamdot
                           : id == SRX_F_RHO_DIPOLE ? c->f.rho_dipole : id == SRX_F_RHO_FOREST ? c->f.rho_forest
-                          : id == SRX_F_RHO_INPLANE ? c->f.rho_inplane : c->f.gam_acum;
+                          : id == SRX_F_RHO_INPLANE ? c->f.rho_inplane : id == SRX_F_RHO_G1 ? c->hc.rho_g1
+                          : id == SRX_F_RHO_G2 ? c->hc.rho_g2 : c->f.gam_acum;
+        if (!src) return SRX_E_FIELD;   // GND densities without the GND option
         tmp.resize((size_t)12 * N);
         if ((rc = fetch(src, sizeof(double) * 12 * N, tmp.data()))) return rc;
         for (long v = 0; v < N; ++v)
@@ -1484,10 +1544,12 @@ void srx_close(srx_ctx *c) {
     double *ptrs[] = {c->f.sig, c->f.sig_pre, c->f.eps, c->f.edot, c->f.dg, c->f.dg0, c->f.tran, c->f.tran0,
                       c->f.rho_s, c->f.rho_m, c->f.rho_m_old, c->f.rho_F, c->f.rho_P, c->f.gamdot, c->f.gam_acum,
                       c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->f.rho_tot, c->f.fl0,
+                      c->hc.rho_g1, c->hc.rho_g2, c->hc.gnd_fp,
                       c->partials, c->red, c->redseg};
     for (double *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->f.dsig) (void)hipFree(c->f.dsig);
+    if (c->hc.gbc) (void)hipFree(c->hc.gbc);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->mlp_blob) (void)hipFree(c->mlp_blob);
     for (void *p : {(void *)c->rec_x, (void *)c->rec_t, (void *)c->rec_nd, (void *)c->rec_count})

@@ -988,6 +988,16 @@ __device__ __forceinline__ void rho_pf(const DevPhase &P, const double rs[NSYS],
     }
 }
 
+// GND parts of rho_F and rho_P (constitutive.c:565-576 with DD_GND): sum_j chi1_sj (|rho_g1,j
+// cos_g1,sj| + |rho_g2,j cos_g2,sj|) and the chi / sin counterpart, kept per voxel in `gnd_fp` and
+// refreshed whenever the GND densities change (nothing added without the GND option).
+__device__ __forceinline__ void gnd_add(const DevConst *__restrict__ cst, int s, size_t N, int v, double &rP,
+                                        double &rF) {
+    const double *fp = cst->gnd_fp;
+    rF += fp[s * N + v];
+    rP += fp[(NSYS + s) * N + v];
+}
+
 // mobile density (constitutive.c:632)
 // sqrt(x) and 1/sqrt(x) for normal x > 0: hardware reciprocal square root refined by two
 // coupled Newton steps and a final residual correction of the root (within an ulp); the
@@ -1144,7 +1154,7 @@ __device__ __forceinline__ double nr_in(const double *stg, int slot, const doubl
 
 // WS: Newton start 0 = lam (the reference), 1 = lam + the previous call's change (extrapolated),
 // 2 = lam + the learned prediction `wsp` (crystal frame)
-template <bool FIRST, bool ROT, int PH, int WS>
+template <bool FIRST, bool ROT, int PH, int WS, bool GND>
 __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
                                          double *lds, const double *stg, const float *wsp, NrResult &res) {
     constexpr bool STAGED = !FIRST;
@@ -1237,6 +1247,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             if (s < nsys) {
                 double rP, rF;
                 rho_pf(P, rs, s, rP, rF);
+                if (GND) gnd_add(cst, s, N, v, rP, rF);
                 const double rmn = rho_mobile(P, rP, rF);
                 double t, iv, g;
                 sys_const(P, rP, rF, rmn, rmn, t, iv, g);
@@ -1538,7 +1549,7 @@ __device__ __forceinline__ void nr_learned_start(const DevConst *__restrict__ cs
     for (int k = 0; k < 6; ++k) wsp[k] = (float)w6[k];
 }
 
-template <bool FIRST, bool ROT, int WS>
+template <bool FIRST, bool ROT, int WS, bool GND = false>
 __global__ void __launch_bounds__(NR_BLOCK)
 __attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8)))
 k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ partials,
@@ -1575,9 +1586,9 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS>(cst, f, v, N, lds, stg, wsp, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS, GND>(cst, f, v, N, lds, stg, wsp, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS>(cst, f, v, N, lds, stg, wsp, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS, GND>(cst, f, v, N, lds, stg, wsp, r);
             }
         }
         todo &= ~__ballot(mine);
@@ -1761,7 +1772,7 @@ constexpr int END_BLOCK = 128;
 #define SRX_END_UNROLL 1
 #endif
 
-template <int PH>
+template <int PH, bool GND>
 __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
                                           int lagged_possible, int do_orient, int istep, double *vals) {
     const DevPhase &P = cst->ph[PH];
@@ -1802,13 +1813,21 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
     // rho_tot of evolution.c:1040-1046, from the densities at the start of the increment
 #pragma unroll
     for (int s = 0; s < NSYS; ++s)
-        if (s < nsys) rho_tot += rs[s] + f.rho_m_old[s * N + v] + sqrt(0.0);
+        if (s < nsys) {
+            double g1 = 0.0, g2 = 0.0;
+            if (GND) {
+                g1 = cst->rho_g1[s * N + v];
+                g2 = cst->rho_g2[s * N + v];
+            }
+            rho_tot += rs[s] + f.rho_m_old[s * N + v] + sqrt(g1 * g1 + g2 * g2 + 0.0 * 0.0);
+        }
     f.rho_tot[v] = rho_tot;
 #pragma unroll SRX_END_UNROLL
     for (int s = 0; s < NSYS; ++s) {
         if (s < nsys) {
             double rP, rF;
             rho_pf(P, rs, s, rP, rF);
+            if (GND) gnd_add(cst, s, N, v, rP, rF);
             const double rmn = f.rho_m[s * N + v];
             const double rmo = f.rho_m_old[s * N + v];
             double tp, inv, g0;
@@ -1937,6 +1956,7 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
 #ifndef SRX_END_WAVES_PER_EU
 #define SRX_END_WAVES_PER_EU 2
 #endif
+template <bool GND>
 __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(SRX_END_WAVES_PER_EU, 8))) k_end(const DevConst *__restrict__ cst, Fields f, int nvox,
                                                    int lagged_possible, int do_orient, int istep,
                                                    double *__restrict__ partials) {
@@ -1954,9 +1974,9 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) end_voxel<0>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
+                if (cst->ph[0].xtal) end_voxel<0, GND>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
             } else {
-                if (cst->ph[1].xtal) end_voxel<1>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
+                if (cst->ph[1].xtal) end_voxel<1, GND>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
             }
         }
         todo &= ~__ballot(mine);
@@ -1983,6 +2003,89 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
         double acc = red[0][k];
         for (int w = 1; w < END_BLOCK / 64; ++w) acc += red[w][k];
         partials[(size_t)k * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// GND option (a DD_GND build, first order): after the hardening update of an increment,
+//   Gradient_ShearRate (constitutive.c:882-1115): per system, the one-sided or central difference
+//   of gamdot along x, y, z inside the grain (the same-grain flags of Gradient_ExchangeGrainID,
+//   constitutive.c:731-880, periodic neighbours, with its z- flag taken from the y- neighbour);
+//   computed when Hard_Flag = 1 and istep > 1, zero before (init.c:1083);
+//   DislocationEvolution's GND branch (constitutive.c:1428-1519 with DD_GND_FIRST_ORDER):
+//   rho_g1 += (grad . bp) / b / L0 dt (screw), rho_g2 += -(grad . bb) / b / L0 dt (edge-I),
+//   edge-II += 0;
+// then the GND parts of rho_F and rho_P of the new densities, and per block the sum of
+// sqrt(rho_g1^2 + rho_g2^2) for AvgDislocation's GND mean (constitutive.c:1745-1757).
+// One thread per voxel; the neighbours' gamdot rows are read from HBM (L2 serves the reuse).
+// ---------------------------------------------------------------------------
+constexpr int GND_BLOCK = 256;
+
+__device__ __forceinline__ double gb_diff(bool c0, bool c1, double v, double vp, double vm) {
+    return (!c0 && c1) ? v - vm : (c0 && !c1) ? vp - v : (c0 && c1) ? (vp - vm) / 2.0 : 0.0;
+}
+
+__global__ void __launch_bounds__(GND_BLOCK) k_gnd(const DevConst *__restrict__ cst, const int *__restrict__ phase,
+                                                   const double *__restrict__ gamdot, const unsigned char *__restrict__ gbc,
+                                                   double *__restrict__ rho_g1, double *__restrict__ rho_g2,
+                                                   double *__restrict__ gnd_fp, int nvox, int grad,
+                                                   double *__restrict__ partials) {
+    const int v = blockIdx.x * GND_BLOCK + threadIdx.x;
+    const size_t N = (size_t)nvox;
+    double acc = 0.0;
+    const int ph = v < nvox ? phase[v] : 0;
+    if (v < nvox && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal) {
+        const DevPhase &P = cst->ph[ph - 1];
+        const int n0 = cst->n[0], n1 = cst->n[1], n2 = cst->n[2];
+        const int x = v / (n1 * n2), y = (v / n2) % n1, z = v % n2;
+        const int xp = x + 1 >= n0 ? 0 : x + 1, xm = x - 1 < 0 ? n0 - 1 : x - 1;
+        const int yp = y + 1 >= n1 ? 0 : y + 1, ym = y - 1 < 0 ? n1 - 1 : y - 1;
+        const int zp = z + 1 >= n2 ? 0 : z + 1, zm = z - 1 < 0 ? n2 - 1 : z - 1;
+        const size_t pxp = ((size_t)xp * n1 + y) * n2 + z, pxm = ((size_t)xm * n1 + y) * n2 + z;
+        const size_t pyp = ((size_t)x * n1 + yp) * n2 + z, pym = ((size_t)x * n1 + ym) * n2 + z;
+        const size_t pzp = ((size_t)x * n1 + y) * n2 + zp, pzm = ((size_t)x * n1 + y) * n2 + zm;
+        const unsigned b = gbc[v];
+        const double dt = cst->dt, L0 = cst->L0;
+        const int nsys = P.nsys;
+        if (grad) {
+            for (int i = 0; i < nsys; ++i) {
+                const double *gd = gamdot + (size_t)i * N;
+                const double g0 = gd[v];
+                const double gx = gb_diff(b & 1, b & 2, g0, gd[pxp], gd[pxm]);
+                const double gy = gb_diff(b & 4, b & 8, g0, gd[pyp], gd[pym]);
+                const double gz = gb_diff(b & 16, b & 32, g0, gd[pzp], gd[pzm]);
+                const double s1 = gx * P.bp[i][0] + gy * P.bp[i][1] + gz * P.bp[i][2];
+                rho_g1[(size_t)i * N + v] += s1 / P.b / L0 * dt;
+                const double s2 = gx * P.bb[i][0] + gy * P.bb[i][1] + gz * P.bb[i][2];
+                rho_g2[(size_t)i * N + v] += -1.0 * s2 / P.b / L0 * dt;
+            }
+        }
+        double g1[NSYS], g2[NSYS];
+        for (int j = 0; j < NSYS; ++j) {
+            g1[j] = j < nsys ? rho_g1[(size_t)j * N + v] : 0.0;
+            g2[j] = j < nsys ? rho_g2[(size_t)j * N + v] : 0.0;
+        }
+        for (int i = 0; i < NSYS; ++i) {
+            double dF = 0.0, dP = 0.0;
+            if (i < nsys) {
+                for (int j = 0; j < nsys; ++j) {
+                    dF += P.chi1[i][j] * (fabs(g1[j] * P.cos_g1[i][j]) + fabs(g2[j] * P.cos_g2[i][j]));
+                    dP += P.chi[i][j] * (fabs(g1[j] * P.sin_g1[i][j]) + fabs(g2[j] * P.sin_g2[i][j]));
+                }
+                acc += sqrt(g1[i] * g1[i] + g2[i] * g2[i] + 0.0 * 0.0);
+            }
+            gnd_fp[(size_t)i * N + v] = dF;
+            gnd_fp[(size_t)(NSYS + i) * N + v] = dP;
+        }
+    }
+    __shared__ double red[GND_BLOCK / 64];
+    const double w = wave_sum(acc);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = w;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int k = 1; k < GND_BLOCK / 64; ++k) t += red[k];
+        partials[blockIdx.x] = t;
     }
 }
 
@@ -2376,9 +2479,14 @@ int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
 int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, const IterCtl *ctl) {
+               double *partials, const IterCtl *ctl, int gnd) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
-    if (f.dsig && f.mlp) {   // learned Newton start (the first call of an increment starts at lam)
+    if (gnd) {   // GND option (plain start only)
+        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (first_call) SRX_LAUNCH((k_nr<true, false, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else if (rotated) SRX_LAUNCH((k_nr<false, true, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+        else SRX_LAUNCH((k_nr<false, false, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
+    } else if (f.dsig && f.mlp) {   // learned Newton start (the first call of an increment starts at lam)
         if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
         else if (first_call) SRX_LAUNCH((k_nr<true, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
         else if (rotated) SRX_LAUNCH((k_nr<false, true, 2>), g, b, 0, s, cst, f, nvox, partials, ctl);
@@ -2428,10 +2536,23 @@ void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int cre
 }
 
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int,
-                int do_orient, int, int istep, double *partials) {
-    SRX_LAUNCH(k_end, dim3(end_block_count(nvox)), dim3(END_BLOCK), 0, s, cst, f, nvox, lagged_possible,
-                       do_orient, istep, partials);
+                int do_orient, int, int istep, double *partials, int gnd) {
+    if (gnd)
+        SRX_LAUNCH(k_end<true>, dim3(end_block_count(nvox)), dim3(END_BLOCK), 0, s, cst, f, nvox, lagged_possible,
+                   do_orient, istep, partials);
+    else
+        SRX_LAUNCH(k_end<false>, dim3(end_block_count(nvox)), dim3(END_BLOCK), 0, s, cst, f, nvox, lagged_possible,
+                   do_orient, istep, partials);
     check_launch("end");
+}
+
+int gnd_block_count(int nvox) { return (nvox + GND_BLOCK - 1) / GND_BLOCK; }
+
+void launch_gnd(hipStream_t s, const DevConst *cst, const DevConst &hc, const Fields &f, int nvox, int grad,
+                double *partials) {
+    SRX_LAUNCH(k_gnd, dim3(gnd_block_count(nvox)), dim3(GND_BLOCK), 0, s, cst, f.phase, f.gamdot, hc.gbc, hc.rho_g1,
+               hc.rho_g2, hc.gnd_fp, nvox, grad, partials);
+    check_launch("gnd");
 }
 
 // test path: forward FFT of ncomp real comps to layout C (x pass done), no Green

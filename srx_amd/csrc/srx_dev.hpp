@@ -34,6 +34,9 @@ struct DevPhase {
     double c1Gb3;         // c_1*G*b*1e-3: tau_pass = c1Gb3*sqrt(|rho_P + rho_m|)
     double k_tcut;        // kT/MY_J_IN_EV/c2/c3/b^2*0.1: tau_cut = k_tcut*sqrt(rho_f)
     double k_g0;          // 1e-3*TRIAL_FREQ*c2*b*exp(-q_slip): gamma_0 = k_g0*rho_m/sqrt(rho_f)
+    // GND option: bp = bb x bn (init.c:200-202) and the projections of init.c:1118-1141
+    double bp[NSYS][3];
+    double cos_g1[NSYS][NSYS], cos_g2[NSYS][NSYS], sin_g1[NSYS][NSYS], sin_g2[NSYS][NSYS];
 };
 
 struct DevConst {
@@ -49,7 +52,14 @@ struct DevConst {
     double Udot_a[9];     // applied spin (antisymmetric part of Udot)
     int n[3];
     int elast_bc;
-    int pad;
+    int gnd;              // 1: GND densities evolve and enter rho_F, rho_P, rho_tot (a DD_GND build)
+    double L0;            // deck L0: length that scales the GND rates (constitutive.c:1437, 1470)
+    // GND option (null otherwise).  Kept here rather than among the kernel arguments so that kernels
+    // load them only inside their GND branch.
+    double *rho_g1, *rho_g2;   // [12][N] GND densities, screw and edge-I (edge-II stays zero in the
+                               //   first-order model)
+    double *gnd_fp;       // [24][N] GND parts of rho_F (12) and rho_P (12) of the current densities
+    unsigned char *gbc;   // [N] same-grain flags of the +x,-x,+y,-y,+z and 'z-' neighbours (bits 0..5)
 };
 
 // Outer fixed-point control kept on the device.  The mixed boundary-condition algebra
@@ -213,7 +223,7 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3],
                              const FftPlan &pz, const void *twz, const IterCtl *ctl, double scale, int tile = 0);
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, const IterCtl *ctl = nullptr);
+               double *partials, const IterCtl *ctl = nullptr, int gnd = 0);
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out,
                    const IterCtl *ctl = nullptr);
 // The per-iteration reduction of the Newton kernel's block partials in two fixed-order stages:
@@ -225,7 +235,12 @@ void launch_control(hipStream_t s, const DevConst *cst, const double *seg, IterC
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9,
                      const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,
-                int do_orient, int do_hardening, int istep, double *partials);
+                int do_orient, int do_hardening, int istep, double *partials, int gnd = 0);
+// GND option: gradient of gamdot (grad = 1), GND density update, GND parts of rho_F / rho_P and one
+// partial per block (gnd_block_count) of sum sqrt(rho_g1^2 + rho_g2^2)
+int gnd_block_count(int nvox);
+void launch_gnd(hipStream_t s, const DevConst *cst, const DevConst &hc, const Fields &f, int nvox, int grad,
+                double *partials);
 // Learned Newton start: operand packing (mlp.hip) and the recording of training samples around a
 // Newton call (kernels.hip): `after` = 0 writes the features of the sampled voxels before the
 // call, 1 their targets after it and advances the sample count.
