@@ -975,6 +975,12 @@ __device__ __forceinline__ double rcp(double x) {
     r = fma(r, fma(-x, r, 1.0), r);
     return r;
 }
+// one refinement step only (relative error about the square of the hardware estimate's): the
+// pivots of the Newton solve's LDL^T, where it only perturbs the step direction at the 1e-14 level
+__device__ __forceinline__ double rcp_step(double x) {
+    const double r = __builtin_amdgcn_rcp(x);
+    return fma(r, fma(-x, r, 1.0), r);
+}
 
 // forest (chi1) and parallel (chi) densities of system s (constitutive.c:562-577); the
 // tables are zero beyond nsys, so the fixed-length sums equal the reference's.
@@ -1161,7 +1167,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     // compile-time phase index: every material constant is a scalar load at a fixed offset
     const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
-    double lam[5], q[5], lamh, qh, rsig;
+    double lam[5], q[5], lamh, qh, rsig2;
     double A[15], r0[5];
     double ws[5] = {0, 0, 0, 0, 0}, wsh = 0.0;   // warm start: crystal-frame offset of the first iterate
     {
@@ -1192,7 +1198,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         double sn = 0.0;
 #pragma unroll
         for (int k = 0; k < 6; ++k) sn += lam6[k] * lam6[k];
-        rsig = 1.0 / sqrt(sn);
+        rsig2 = 1.0 / sn;
         // crystal frame: lam' = Q^T lam, q' = Q^T (eps_p - eps_c)
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
@@ -1282,21 +1288,22 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
     for (int i = 0; i < 5; ++i) { sg[i] = lam[i] + ws[i]; pre[i] = sg[i]; ed[i] = 0.0; }
     int it = 0, itot = 0;
-    double erral = 10 * 1E-10;
+    // stopping rule ||d sig|| / ||lam|| <= 1e-10, compared squared (no square root per iteration)
+    double err2 = 1E-18;
     // With the learned start (WS 2), a solve that does not converge (non-finite iterate, or 500
     // iterations) is repeated from lam, the reference's start; `itot` counts both attempts.  (The
     // extrapolated start has not needed it; the second attempt costs it registers.)
     for (int att = 0; att < (WS == 2 ? 2 : 1); ++att) {
     if (att == 1) {
-        if (erral <= 1E-10) break;
+        if (err2 <= 1E-20) break;
 #pragma unroll
         for (int i = 0; i < 5; ++i) { sg[i] = lam[i]; pre[i] = sg[i]; }
         sh = lamh;
         preh = sh;
         it = 0;
-        erral = 10 * 1E-10;
+        err2 = 1E-18;
     }
-    while (it < 500 && erral > 1E-10) {
+    while (it < 500 && err2 > 1E-20) {
         ++it;
         ++itot;
         double K[15];
@@ -1338,7 +1345,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
             for (int k = 0; k < j; ++k) dj -= K[ps(j, k)] * K[ps(j, k)] * K[ps(k, k)];
             // K[ps(k,k)] holds D_k until it is inverted below
-            const double rj = rcp(dj);
+            const double rj = rcp_step(dj);
 #pragma unroll
             for (int i = j + 1; i < 5; ++i) {
                 double acc = K[ps(i, j)];
@@ -1357,7 +1364,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             y[i] = acc;
         }
 #pragma unroll
-        for (int i = 0; i < 5; ++i) y[i] *= rcp(K[ps(i, i)]);
+        for (int i = 0; i < 5; ++i) y[i] *= rcp_step(K[ps(i, i)]);
 #pragma unroll
         for (int i = 4; i >= 0; --i) {
 #pragma unroll
@@ -1373,7 +1380,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         }
         preh = sh;
         sh -= dh;
-        erral = sqrt(cnr) * rsig;
+        err2 = cnr * rsig2;
     }
     }
     if (WS) {
@@ -1388,36 +1395,39 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     for (int i = 0; i < 5; ++i) cs2 += (sg[i] - lam[i]) * (sg[i] - lam[i]);
     res.cs = sqrt(cs2);
     {
-        // the staging slots now hold the per-system constants: orientations from global memory
+        // the staging slots now hold the per-system constants: the orientation from global memory
         double a[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
-        // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre, with
-        // S' pre = Q2 (sinv o (Q2^T pre)) (S' = Q2 diag(sinv) Q2^T, as sprime builds it)
+        double Q[5][5];
+        q5_from(a, Q);
+        // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre.  With the texture
+        // rotating, S' pre = A pre - Sc pre (A = Sc + S' is still in registers, Sc = Q^T S066 Q), so
+        // the initial orientation S' was built from is not read again.
         double spp[5];
         if (ROT) {
-            double a0[9], R[9];
+            double u[5], w[5];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) R[3 * i + j] = a[i] * a0[j] + a[3 + i] * a0[3 + j] + a[6 + i] * a0[6 + j];
-            double Q2[5][5];
-            q5_from(R, Q2);
-            double t[5];
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
+            for (int i = 0; i < 5; ++i) {
                 double acc = 0.0;
 #pragma unroll
-                for (int j = 0; j < 5; ++j) acc += Q2[j][k] * pre[j];
-                t[k] = P.sinv[k] * acc;
+                for (int k = 0; k < 5; ++k) acc += Q[i][k] * pre[k];
+                u[i] = acc;
             }
 #pragma unroll
             for (int i = 0; i < 5; ++i) {
                 double acc = 0.0;
 #pragma unroll
-                for (int k = 0; k < 5; ++k) acc += Q2[i][k] * t[k];
+                for (int k = 0; k < 5; ++k) acc += cst->S066[i][k] * u[k];
+                w[i] = acc;
+            }
+#pragma unroll
+            for (int i = 0; i < 5; ++i) {
+                double acc = 0.0;
+#pragma unroll
+                for (int j = 0; j < 5; ++j) acc += A[ps(i, j)] * pre[j];
+#pragma unroll
+                for (int k = 0; k < 5; ++k) acc -= Q[k][i] * w[k];
                 spp[i] = acc;
             }
         } else {
@@ -1434,8 +1444,6 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         ce2 += eh * eh;
         res.ce = sqrt(ce2);
         // stress back to the sample frame
-        double Q[5][5];
-        q5_from(a, Q);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             double acc = 0.0;
