@@ -1100,9 +1100,9 @@ __device__ __forceinline__ void slip_rate(double rss, double e, double inv, doub
 // used for Edot and the errors are the reference's.
 // ---------------------------------------------------------------------------
 constexpr int NR_BLOCK = 64;
-// slip systems evaluated per trip of the Newton kernel's system loop
-#ifndef SRX_NR_UNROLL
-#define SRX_NR_UNROLL 2
+// slip systems evaluated together per trip of the Newton kernel's system loop (divides 12)
+#ifndef SRX_NR_GROUP
+#define SRX_NR_GROUP 2
 #endif
 
 __device__ __forceinline__ double wave_sum(double v) {
@@ -1312,21 +1312,37 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
         const double *tpu = (FIRST && it == 1) ? tpo : tp;
-#pragma unroll SRX_NR_UNROLL
-        for (int s = 0; s < NSYS; ++s) {
-            const double *p = P.p[s];
-            const double rss = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
-            const double iv = inv[s * NR_BLOCK];
-            const double e = fma(fabs(rss), iv, -tpu[s * NR_BLOCK]);
-            if (__any(e > 0.0)) {
-                double g, g2dt;
-                slip_rate(rss, e, iv, g0[s * NR_BLOCK], g, g2dt);
+        // SRX_NR_GROUP systems per trip, under one wavefront-uniform branch: their slip-rate
+        // evaluations are independent dependency chains the scheduler interleaves (an inactive
+        // system of the group evaluates at 0 and adds exact zeros)
+        constexpr int NG = SRX_NR_GROUP;
+        static_assert(NSYS % NG == 0, "systems are taken in groups of SRX_NR_GROUP");
+#pragma unroll 1
+        for (int s0 = 0; s0 < NSYS; s0 += NG) {
+            double rss[NG], ivs[NG], e[NG];
+            bool act = false;
 #pragma unroll
-                for (int i = 0; i < 5; ++i) {
-                    ed[i] += p[i] * g;
-                    const double gp = g2dt * p[i];
+            for (int u = 0; u < NG; ++u) {
+                const double *p = P.p[s0 + u];
+                rss[u] = p[0] * sg[0] + p[1] * sg[1] + p[2] * sg[2] + p[3] * sg[3] + p[4] * sg[4];
+                ivs[u] = inv[(s0 + u) * NR_BLOCK];
+                e[u] = fma(fabs(rss[u]), ivs[u], -tpu[(s0 + u) * NR_BLOCK]);
+                act |= e[u] > 0.0;
+            }
+            if (__any(act)) {
+                double g[NG], g2dt[NG];
 #pragma unroll
-                    for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
+                for (int u = 0; u < NG; ++u) slip_rate(rss[u], e[u], ivs[u], g0[(s0 + u) * NR_BLOCK], g[u], g2dt[u]);
+#pragma unroll
+                for (int u = 0; u < NG; ++u) {
+                    const double *p = P.p[s0 + u];
+#pragma unroll
+                    for (int i = 0; i < 5; ++i) {
+                        ed[i] += p[i] * g[u];
+                        const double gp = g2dt[u] * p[i];
+#pragma unroll
+                        for (int j = i; j < 5; ++j) K[ps(i, j)] += gp * p[j];
+                    }
                 }
             }
         }
