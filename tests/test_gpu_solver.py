@@ -71,6 +71,35 @@ def test_converged_increments_match_oracle(tmp_path, oracle_lib):
         compare_fields(g, o, 1e-6)
 
 
+def test_drop_in_check_64cubed_125_grains(tmp_path, oracle_lib):
+    """The drop-in correctness configuration: 64^3 voxels, 125 Voronoi grains, random texture,
+    three increments (the texture rotates from the third) with fixed outer iterations, every field
+    and report value against the oracle on identical inputs."""
+    deck = make(tmp_path, n=(64, 64, 64), grains=125, seed=11, nsteps=3)
+    o = oracle_lib.Oracle(deck, fixed_iters=6)
+    g = srx_amd.Solver(deck, fixed_iters=6)
+    for step in range(3):
+        ro, rg = o.step(), g.step()
+        assert rg["nr_iters_total"] == ro["nr_iters_total"]
+        assert rg["nr_iters_max"] == ro["nr_iters_max"]
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9)
+    g.close()
+
+
+def test_twenty_converged_increments_16cubed(tmp_path, oracle_lib):
+    """The CPU-runnable configuration: 16^3, 8 grains, 20 increments to the deck's Err; every
+    increment's report within a few times Err of the oracle's, at most one outer iteration apart."""
+    deck = make(tmp_path, n=(16, 16, 16), grains=8, seed=1, nsteps=20)
+    o = oracle_lib.Oracle(deck)
+    with srx_amd.Solver(deck) as g:
+        for step in range(20):
+            ro, rg = o.step(), g.step()
+            assert rg["converged"] and ro["converged"]
+            assert abs(rg["iters"] - ro["iters"]) <= 1, (step, rg["iters"], ro["iters"])
+            compare_reports(rg, ro, 1e-4)
+
+
 def test_reference_deck_two_crystalline_phases_2d(tmp_path, oracle_lib):
     """The reference's own deck: 256x256x1 (a one-voxel z extent), both phases crystalline with
     different slip-mode files, all six velocity components prescribed."""
