@@ -1726,8 +1726,17 @@ __global__ void __launch_bounds__(RED_SEG_BLOCK) k_reduce_seg(const double *__re
 // the dDisGradAvg this iteration's inverse-z pass applied joins the uniform part, the reduced
 // sums give the errors and get_smacro's averages and correction, and `done` decides whether the
 // next iteration runs.  One thread; the iteration's other kernels have finished (stream order).
+__device__ __forceinline__ void control_step(const DevConst *__restrict__ cst, IterCtl &c, const double *red);
+
 __global__ void k_control(const DevConst *__restrict__ cst, const double *__restrict__ seg, IterCtl *ctl) {
     if (ctl->done) return;
+    // The control block is copied to LDS by the whole block, worked on there by one thread (no
+    // dependent HBM round trips) and copied back.
+    static_assert(sizeof(IterCtl) % 8 == 0, "IterCtl is copied as 64-bit words");
+    constexpr int NW = sizeof(IterCtl) / 8;
+    __shared__ IterCtl lc;
+    for (int i = threadIdx.x; i < NW; i += blockDim.x)
+        reinterpret_cast<unsigned long long *>(&lc)[i] = reinterpret_cast<const unsigned long long *>(ctl)[i];
     // second stage of the reduction: the RED_SEGS partial results of each value, in order
     __shared__ double red[NR_NPART];
     if (threadIdx.x < NR_NPART) {
@@ -1738,8 +1747,14 @@ __global__ void k_control(const DevConst *__restrict__ cst, const double *__rest
         red[k] = r;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    IterCtl &c = *ctl;
+    if (threadIdx.x == 0) control_step(cst, lc, red);
+    __syncthreads();
+    for (int i = threadIdx.x; i < NW; i += blockDim.x)
+        reinterpret_cast<unsigned long long *>(ctl)[i] = reinterpret_cast<const unsigned long long *>(&lc)[i];
+}
+
+// one outer iteration's bookkeeping on the reduced values red[] (thread 0 of k_control)
+__device__ __forceinline__ void control_step(const DevConst *__restrict__ cst, IterCtl &c, const double *red) {
     for (int k = 0; k < 9; ++k) c.uniform[k] += c.ddg[k];
     c.iter += 1;
     if (red[16] > 0) {   // a non-finite stress (the reference asserts, constitutive.c:683)
