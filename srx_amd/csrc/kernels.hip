@@ -1909,11 +1909,11 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
         f.edot[i * N + v] = acc;
         const double e = f.eps[i * N + v] + acc * dt;
         f.eps[i * N + v] = e;
-        vals[i] = e;
-        vals[6 + i] = acc;
+        vals[(i) * END_BLOCK] = e;
+        vals[(6 + i) * END_BLOCK] = acc;
     }
-    vals[5] = 0.0;
-    vals[11] = 0.0;
+    vals[(5) * END_BLOCK] = 0.0;
+    vals[(11) * END_BLOCK] = 0.0;
     (void)eds;
     // reorientation (update_orient), with the pre-update TranMat for b and n
     if (do_orient) {
@@ -1926,8 +1926,8 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
         double rot[3][3], rsl[3][3];
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) rsl[i][j] = (lp[i][j] - lp[j][i]) / 2.;
-        vals[18] = sqrt(rsl[2][1] * rsl[2][1] + rsl[0][2] * rsl[0][2] + rsl[1][0] * rsl[1][0]);
-        vals[19] = sqrt(rl[2][1] * rl[2][1] + rl[0][2] * rl[0][2] + rl[1][0] * rl[1][0]);
+        vals[(18) * END_BLOCK] = sqrt(rsl[2][1] * rsl[2][1] + rsl[0][2] * rsl[0][2] + rsl[1][0] * rsl[1][0]);
+        vals[(19) * END_BLOCK] = sqrt(rl[2][1] * rl[2][1] + rl[0][2] * rl[0][2] + rl[1][0] * rl[1][0]);
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) rot[i][j] = (cst->Udot_a[3 * i + j] + rl[i][j] - rsl[i][j]) * dt;
         // Orientation (constitutive.c:142-194)
@@ -1982,14 +1982,14 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
         sFo += rfor;
         sI += rinp;
     }
-    vals[12] = sumF;
-    vals[13] = sumP;
-    vals[14] = sR;
-    vals[15] = sumM;
-    vals[16] = sD;
-    vals[17] = sFo;
-    vals[20] = sI;
-    vals[21] = bad ? 1.0 : 0.0;
+    vals[(12) * END_BLOCK] = sumF;
+    vals[(13) * END_BLOCK] = sumP;
+    vals[(14) * END_BLOCK] = sR;
+    vals[(15) * END_BLOCK] = sumM;
+    vals[(16) * END_BLOCK] = sD;
+    vals[(17) * END_BLOCK] = sFo;
+    vals[(20) * END_BLOCK] = sI;
+    vals[(21) * END_BLOCK] = bad ? 1.0 : 0.0;
 }
 
 #ifndef SRX_END_WAVES_PER_EU
@@ -2003,9 +2003,11 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
-    double vals[END_NPART];
+    // this lane's partial values, in an LDS column (stride END_BLOCK) rather than registers
+    __shared__ double lv[END_NPART * END_BLOCK];
+    double *vals = lv + threadIdx.x;
 #pragma unroll
-    for (int k = 0; k < END_NPART; ++k) vals[k] = 0.0;
+    for (int k = 0; k < END_NPART; ++k) vals[k * END_BLOCK] = 0.0;
     unsigned long long todo = __ballot(valid && ph >= 1 && ph <= NPH);
     while (todo) {
         const int first = __ffsll((unsigned long long)todo) - 1;
@@ -2026,14 +2028,14 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             f.edot[i * N + v] = 0.0;
-            vals[i] = f.eps[i * N + v];
+            vals[i * END_BLOCK] = f.eps[i * N + v];
         }
     }
     __shared__ double red[END_BLOCK / 64][END_NPART];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < END_NPART; ++k) {
-        const double s = wave_sum(vals[k]);
+        const double s = wave_sum(vals[k * END_BLOCK]);
         if (lane == 0) red[wid][k] = s;
     }
     __syncthreads();
