@@ -237,7 +237,7 @@ def main() -> int:
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--ensemble", default=None,
                     help="run an RVE ensemble instead, e.g. 16x32,16x64,12x96,4x128 (count x edge)")
-    ap.add_argument("--concurrency", type=int, default=4, help="ensemble members run at once")
+    ap.add_argument("--concurrency", type=int, default=8, help="ensemble members run at once")
     ap.add_argument("--learned", action="store_true",
                     help="train the learned Newton start and compare plain, extrapolated and learned starts")
     ap.add_argument("--train-increments", type=int, default=4, help="increments recorded for training")
