@@ -162,6 +162,32 @@ def test_mixed_precision_error_vs_fp64(tmp_path):
     assert rel(b.field("rho_s"), a.field("rho_s")) < 1e-6
 
 
+def von_mises(s):
+    s = s.reshape(-1, 3, 3)
+    d = s - np.trace(s, axis1=1, axis2=2)[:, None, None] * np.eye(3) / 3
+    return np.sqrt(1.5 * (d * d).sum(axis=(1, 2)))
+
+
+def test_mixed_precision_against_the_oracle(tmp_path, oracle_lib):
+    """The mixed path against the FP64 oracle, the errors the mixed-precision configuration asks
+    for: the stress-strain curve (macroscopic stress and e_vm of every converged increment) within
+    1e-4 relative, per-voxel von Mises stress within 1e-3 of its maximum, the slip-system
+    dislocation densities within 1e-5 relative."""
+    deck = make(tmp_path, n=(32, 32, 32), grains=20, seed=6, nsteps=5)
+    o = oracle_lib.Oracle(deck)
+    with srx_amd.Solver(deck, precision=srx_amd.PREC_MIXED) as g:
+        for step in range(5):
+            ro, rg = o.step(), g.step()
+            assert rg["converged"] and ro["converged"]
+            assert abs(rg["iters"] - ro["iters"]) <= 1, (step, rg["iters"], ro["iters"])
+            sa, sb = np.array(ro["sig_avg"]), np.array(rg["sig_avg"])
+            assert np.linalg.norm(sb - sa) <= 1e-4 * np.linalg.norm(sa), step
+            assert abs(rg["e_vm"] - ro["e_vm"]) <= 1e-4 * abs(ro["e_vm"]), step
+        vo, vg = von_mises(o.field("sig")), von_mises(g.field("sig"))
+        assert np.abs(vg - vo).max() <= 1e-3 * vo.max()
+        assert rel(g.field("rho_s"), o.field("rho_s")) < 1e-5
+
+
 def test_output_files_match_oracle(tmp_path, oracle_lib):
     """vm.out / dd.out / err.out written by the GPU path carry the oracle's numbers (%e)."""
     deck = make(tmp_path, n=(8, 8, 8), grains=3, seed=7, nsteps=3)
