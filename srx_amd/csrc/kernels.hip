@@ -2089,7 +2089,9 @@ __global__ void __launch_bounds__(GND_BLOCK) k_gnd(const DevConst *__restrict__ 
         const double dt = cst->dt, L0 = cst->L0;
         const int nsys = P.nsys;
         if (grad) {
-            for (int i = 0; i < nsys; ++i) {
+#pragma unroll
+            for (int i = 0; i < NSYS; ++i) {
+                if (i >= nsys) continue;
                 const double *gd = gamdot + (size_t)i * N;
                 const double g0 = gd[v];
                 const double gx = gb_diff(b & 1, b & 2, g0, gd[pxp], gd[pxm]);
@@ -2101,20 +2103,23 @@ __global__ void __launch_bounds__(GND_BLOCK) k_gnd(const DevConst *__restrict__ 
                 rho_g2[(size_t)i * N + v] += -1.0 * s2 / P.b / L0 * dt;
             }
         }
+        // |rho_g| of the phase's systems (zero beyond nsys, where the tables are zero too); the
+        // projection tables are non-negative, so |rho_g cos| = |rho_g| cos exactly
         double g1[NSYS], g2[NSYS];
+#pragma unroll
         for (int j = 0; j < NSYS; ++j) {
-            g1[j] = j < nsys ? rho_g1[(size_t)j * N + v] : 0.0;
-            g2[j] = j < nsys ? rho_g2[(size_t)j * N + v] : 0.0;
+            g1[j] = j < nsys ? fabs(rho_g1[(size_t)j * N + v]) : 0.0;
+            g2[j] = j < nsys ? fabs(rho_g2[(size_t)j * N + v]) : 0.0;
         }
+#pragma unroll
         for (int i = 0; i < NSYS; ++i) {
             double dF = 0.0, dP = 0.0;
-            if (i < nsys) {
-                for (int j = 0; j < nsys; ++j) {
-                    dF += P.chi1[i][j] * (fabs(g1[j] * P.cos_g1[i][j]) + fabs(g2[j] * P.cos_g2[i][j]));
-                    dP += P.chi[i][j] * (fabs(g1[j] * P.sin_g1[i][j]) + fabs(g2[j] * P.sin_g2[i][j]));
-                }
-                acc += sqrt(g1[i] * g1[i] + g2[i] * g2[i] + 0.0 * 0.0);
+#pragma unroll
+            for (int j = 0; j < NSYS; ++j) {
+                dF += P.chi1[i][j] * (g1[j] * P.cos_g1[i][j] + g2[j] * P.cos_g2[i][j]);
+                dP += P.chi[i][j] * (g1[j] * P.sin_g1[i][j] + g2[j] * P.sin_g2[i][j]);
             }
+            acc += sqrt(g1[i] * g1[i] + g2[i] * g2[i] + 0.0 * 0.0);
             gnd_fp[(size_t)i * N + v] = dF;
             gnd_fp[(size_t)(NSYS + i) * N + v] = dP;
         }
