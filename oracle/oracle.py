@@ -87,6 +87,8 @@ def lib():
         L.orc_nr_voxel.argtypes = [C.c_void_p, C.c_int, P(C.c_double), P(C.c_double)]
         L.orc_update_schmid.argtypes = [C.c_void_p]
         L.orc_get_schmid.argtypes = [C.c_void_p, C.c_int, P(C.c_double)]
+        L.orc_orientation.argtypes = [P(C.c_double), P(C.c_double)]
+        L.orc_update_orient.argtypes = [C.c_void_p]
         _lib = L
     return _lib
 

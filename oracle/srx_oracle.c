@@ -1150,6 +1150,13 @@ static void update_orient(orc_state *s)
     }
 }
 
+void orc_orientation(double *tran9, double *rot9)
+{
+    orientation((double (*)[3])tran9, (double (*)[3])rot9);
+}
+
+void orc_update_orient(orc_state *s) { update_orient(s); }
+
 /* DislocationEvolution (constitutive.c:1353-1529), istep <= 2e6 branch */
 static void dislocation_evolution(orc_state *s, int istep)
 {

@@ -101,6 +101,10 @@ int  orc_strain_rate(orc_state *s, int idx, const double *sig6, double *edot6, d
 int  orc_nr_voxel(orc_state *s, int idx, double *err_s, double *err_e);
 /* update_schmid (kinematics.c:430-463) and the sample-frame Schmid tensors of a voxel */
 void orc_update_schmid(orc_state *s);
+/* Orientation (constitutive.c:142-194) on one 3x3 TranMat and spin increment (rot is overwritten
+   with the rotation applied); update_orient (constitutive.c:196-273) over the current fields. */
+void orc_orientation(double *tran9, double *rot9);
+void orc_update_orient(orc_state *s);
 int  orc_get_schmid(orc_state *s, int idx, double *out60);
 
 #ifdef __cplusplus

@@ -245,3 +245,23 @@ def test_error_codes(tmp_path):
     inputs.write_deck(bad, n=(4, 4, 4), mesh="mesh_swapped.txt")
     assert open_status(bad) == 1139
     assert open_status(str(tmp_path / "missing.in")) == 2001
+
+
+@pytest.mark.parametrize("timestep,diverges", [(1.0, True), (0.5, True), (0.25, False), (0.1, False)])
+def test_large_increment_overflows_the_plain_newton_solve(tmp_path, timestep, diverges):
+    """At 1e-3/s, 0.1% strain per increment (TimeStep 1.0) or 0.05% (0.5) puts the slip-rate
+    argument of the elastic predictor near 150 and the reference's plain Newton iteration
+    (evolution.c:217-391, NR_MODIFIED off) overflows sinh/cosh to inf/NaN in the first increment,
+    where the reference stops at assert(isnan(tmp_gam)==0) (constitutive.c:683); the oracle
+    returns SRX_E_NAN (3001) instead.  0.025% and 0.01% per increment converge.  This is why
+    the benchmark and the parity tests run TimeStep 0.1, the reference deck's own (input:61)."""
+    deck = inputs.make_case(str(tmp_path), n=(8, 8, 8), grains=8, seed=1, nsteps=3, timestep=timestep)
+    o = Oracle(deck, nthreads=8)
+    if diverges:
+        with pytest.raises(RuntimeError, match="code 3001"):
+            o.step()
+    else:
+        for _ in range(3):
+            r = o.step()
+            assert r["converged"] == 1 and np.isfinite(r["s_vm"])
+    o.close()
