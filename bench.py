@@ -54,9 +54,11 @@ def fft_green_bytes_per_voxel(real_bytes: int, n2: int, c_out: int = 9) -> float
 
 def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: int, keep_fields: bool = False,
                   warm_start: int = 0, blob=None) -> dict:
-    """One solver run: `warmup` untimed increments, then `steps` timed ones.  timing=2 records
-    HIP events around the Newton kernel only (the headline run: the other launches carry no
-    event overhead); timing=1 around every kernel (the per-kernel breakdown run)."""
+    """One solver run: `warmup` untimed increments, then `steps` timed ones (wall clock).
+    timing=0: no events, every batch of outer iterations replayed from its HIP graph (the
+    headline run); timing=2: HIP events around each Newton-kernel launch on the solver's stream
+    (the roofline run); timing=1: events around every kernel (the per-kernel breakdown run).
+    Batches with events are enqueued launch by launch."""
     import srx_amd
     s = srx_amd.Solver(deck, precision=precision, reserved=[timing], warm_start=warm_start)
     if blob is not None:
@@ -83,30 +85,41 @@ def run_precision(deck: str, precision: int, steps: int, warmup: int, timing: in
             "nr_wave": sum(r["nr_wave_iters"] for r in reps)}
 
 
-def cpu_baseline(deck: str, nvox: int, iters_per_increment: float, outer_iters: int,
-                 gpu_nr_per_iter: float) -> dict | None:
-    """The FP64 CPU oracle on a bounded sample: the first `outer_iters` outer iterations of
-    increment 0 of the same 128^3 deck, scaled to an increment by the GPU's iterations/increment."""
+def cpu_baseline(n: int, grains: int, seed: int, increments: int) -> dict | None:
+    """The FP64 CPU oracle on a bounded sample: the first `increments` increments of a
+    `n`^3, `grains`-grain Voronoi RVE with the same material, loading and Err, each iterated to
+    convergence and all of them timed; the rate is voxels x increments / seconds."""
     try:
         from oracle import oracle
+        from srx_amd import inputs
         oracle.build()
         threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
-        o = oracle.Oracle(deck, fixed_iters=outer_iters, nthreads=threads)
-        t0 = time.perf_counter()
-        r = o.step()
-        t1 = time.perf_counter()
+        work = tempfile.mkdtemp(prefix="srx_cpu_")
+        deck = inputs.make_case(work, n=(n, n, n), grains=grains, seed=seed, nsteps=increments, timestep=0.1)
+        o = oracle.Oracle(deck, nthreads=threads)
+        nvox = n ** 3
+        secs, iters, nr = [], 0, 0
+        for k in range(increments):
+            t0 = time.perf_counter()
+            r = o.step()
+            secs.append(time.perf_counter() - t0)
+            if not r["converged"]:
+                raise RuntimeError("oracle increment %d did not converge" % k)
+            iters += r["iters"]
+            nr += r["nr_iters_total"]
+            print("bench: cpu oracle increment %d: %d outer iterations, %.1f s" % (k, r["iters"], secs[-1]),
+                  file=sys.stderr, flush=True)
         o.close()
     except Exception as exc:  # the baseline is reported, never required
         return {"error": str(exc)}
-    t_iter = (t1 - t0) / outer_iters
-    value = nvox / (t_iter * iters_per_increment)
-    return {"value": value, "unit": "voxel-increments/s", "cores": threads, "kind": "port",
-            "sample": "CPU oracle (OpenMP, FP64), first %d outer iterations of increment 0 at %d^3 "
-                      "(%.2f s/iteration, %.2f Newton iterations/voxel/iteration against %.2f in the GPU's "
-                      "timed increments, so an upper bound for the CPU); scaled by the GPU run's "
-                      "%.1f outer iterations per increment" % (outer_iters, round(nvox ** (1 / 3)), t_iter,
-                                                               r["nr_iters_total"] / nvox / outer_iters,
-                                                               gpu_nr_per_iter, iters_per_increment)}
+    total = sum(secs)
+    return {"value": nvox * increments / total, "unit": "voxel-increments/s", "cores": threads, "kind": "port",
+            "sample": "CPU oracle (C, OpenMP, FP64) run from the start on a %d^3, %d-grain Voronoi RVE of the same "
+                      "material, loading and Err: increments 0-%d, each iterated to convergence and all timed "
+                      "(%.1f s, %d outer iterations, %.2f Newton iterations per voxel per outer iteration); "
+                      "rate = voxels x increments / seconds" % (n, grains, increments - 1, total, iters,
+                                                               nr / nvox / max(iters, 1)),
+            "seconds": total, "outer_iters": iters}
 
 
 def ensemble_bench(args) -> int:
@@ -232,7 +245,11 @@ def main() -> int:
     ap.add_argument("--grains", type=int, default=1000)
     ap.add_argument("--seed", type=int, default=20261202)
     ap.add_argument("--precisions", default="fp64,mixed")
-    ap.add_argument("--cpu-iters", type=int, default=2)
+    ap.add_argument("--cpu-n", type=int, default=64, help="edge of the CPU-baseline RVE")
+    ap.add_argument("--cpu-grains", type=int, default=125)
+    ap.add_argument("--cpu-increments", type=int, default=4, help="converged increments the CPU baseline times")
+    ap.add_argument("--from-start", type=int, default=50,
+                    help="also run this many increments from istep 0 in each precision (0: skip)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--workdir", default=None)
     ap.add_argument("--ensemble", default=None,
@@ -258,24 +275,31 @@ def main() -> int:
     n = (args.n, args.n, args.n)
     nvox = n[0] * n[1] * n[2]
     deck = os.path.join(work, "deck.in")
+    nsteps_deck = max(args.warmup + args.steps, args.from_start) + 1
     if not os.path.exists(deck):
-        inputs.make_case(work, n=n, grains=args.grains, seed=args.seed, nsteps=args.warmup + args.steps + 1,
-                         timestep=0.1)
+        inputs.make_case(work, n=n, grains=args.grains, seed=args.seed, nsteps=nsteps_deck, timestep=0.1)
     print("bench: inputs in %s" % work, file=sys.stderr, flush=True)
 
-    res, brk = {}, {}
+    res, nre, brk, fs = {}, {}, {}, {}
     for name in args.precisions.split(","):
         prec = srx_amd.PREC_MIXED if name == "mixed" else srx_amd.PREC_FP64
-        print("bench: %s run (%d warmup + %d timed increments)" % (name, args.warmup, args.steps),
+        print("bench: %s run (%d warmup + %d timed increments, graph replay)" % (name, args.warmup, args.steps),
               file=sys.stderr, flush=True)
-        res[name] = run_precision(deck, prec, args.steps, args.warmup, timing=2, keep_fields=True)
+        res[name] = run_precision(deck, prec, args.steps, args.warmup, timing=0, keep_fields=True)
         r = res[name]
-        print("bench: %s %.3f s, %d outer iterations, Newton kernel %.1f ms" % (
-            name, r["seconds"], r["iters"], r["kernels"]["nr"][0]), file=sys.stderr, flush=True)
+        print("bench: %s %.3f s, %d outer iterations" % (name, r["seconds"], r["iters"]), file=sys.stderr, flush=True)
+        nre[name] = run_precision(deck, prec, args.steps, args.warmup, timing=2)
+        e = nre[name]
+        print("bench: %s Newton-event run %.3f s, %d outer iterations, Newton kernel %.1f ms in %d launches" % (
+            name, e["seconds"], e["iters"], e["kernels"]["nr"][0], e["kernels"]["nr"][1]), file=sys.stderr, flush=True)
         brk[name] = run_precision(deck, prec, args.steps, args.warmup, timing=1)
         b = brk[name]
         print("bench: %s breakdown run %.3f s, kernels %s" % (
             name, b["seconds"], {k: round(v[0], 1) for k, v in b["kernels"].items()}), file=sys.stderr, flush=True)
+        if args.from_start > 0:
+            fs[name] = run_precision(deck, prec, args.from_start, 0, timing=0, keep_fields=True)
+            print("bench: %s %d increments from the start: %.3f s, %d outer iterations" % (
+                name, args.from_start, fs[name]["seconds"], fs[name]["iters"]), file=sys.stderr, flush=True)
 
     head = "mixed" if "mixed" in res else "fp64"
     H = res[head]
@@ -285,55 +309,72 @@ def main() -> int:
            "scaling": "weak", "vs_baseline": None,
            "dtype": "f64+f32" if head == "mixed" else "f64", "data": "synthetic",
            "config": {"workload": "EVP-FFT increment loop, %d^3 periodic Voronoi, %d grains, FCC Cu, "
-                                  "x3 tension 1e-3/s, TimeStep 0.1 s, Err 2e-5" % (args.n, args.grains),
-                      "n": args.n, "grains": args.grains, "precision": head, "timestep": 0.1, "err": 2e-5}}
+                                  "x3 tension 1e-3/s, TimeStep 0.1 s, Err 2e-5, increments %d-%d" % (
+                                      args.n, args.grains, args.warmup, args.warmup + args.steps - 1),
+                      "n": args.n, "grains": args.grains, "precision": head, "timestep": 0.1, "err": 2e-5,
+                      "execution": "outer-iteration batches replayed from HIP graphs, no events"}}
+
+    def stress_errors(a, b):
+        fa, fb = a["fields"], b["fields"]
+        return {"rel_macro_stress_error": float(np.linalg.norm(a["sig_avg"] - b["sig_avg"]) / np.linalg.norm(b["sig_avg"])),
+                "max_voxel_von_mises_error_rel": float(np.abs(fa["vm"] - fb["vm"]).max() / fb["vm"].max()),
+                "max_slip_system_rho_s_error_rel": float(np.abs(fa["rho_s"] - fb["rho_s"]).max()
+                                                         / np.abs(fb["rho_s"]).max())}
+
     per = {}
     for name, r in res.items():
-        iters = max(r["iters"], 1)
         kt = brk[name]["kernels"]
+        e = nre[name]
         fg_names = ("fwd_z", "fwd_y", "green_x", "inv_y", "inv_z")
         fg_ms = sum(kt[k][0] for k in fg_names if k in kt)
-        nr_ms = r["kernels"].get("nr", (0, 0))[0]   # the headline run's own Newton events
+        b_iters = max(brk[name]["iters"], 1)
+        nr_ms = e["kernels"].get("nr", (0, 0))[0]
         rb = 4 if name == "mixed" else 8
         b_fg = fft_green_bytes_per_voxel(rb, n[2]) * nvox
         per[name] = {
             "value": args.steps * nvox / r["seconds"], "seconds": r["seconds"],
-            "breakdown_run_seconds": brk[name]["seconds"],
+            "newton_event_run_seconds": e["seconds"], "breakdown_run_seconds": brk[name]["seconds"],
             "outer_iters_per_increment": r["iters"] / args.steps,
-            "nr_iters_per_voxel_per_iter": r["nr_total"] / nvox / iters,
+            "nr_iters_per_voxel_per_iter": r["nr_total"] / nvox / max(r["iters"], 1),
             "nr_wavefront_divergence": 64.0 * r["nr_wave"] / max(r["nr_total"], 1),
             "kernel_ms": {k: v[0] for k, v in kt.items()},
-            "fft_green_gbs": b_fg * iters / (fg_ms * 1e-3) / 1e9 if fg_ms else None,
-            "fft_green_frac_hbm": (b_fg * iters / (fg_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if fg_ms else None,
-            "nr_tflops": F_NR * r["nr_total"] / (nr_ms * 1e-3) / 1e12 if nr_ms else None,
+            "fft_green_gbs": b_fg * b_iters / (fg_ms * 1e-3) / 1e9 if fg_ms else None,
+            "fft_green_frac_hbm": (b_fg * b_iters / (fg_ms * 1e-3) / 1e9) / HBM_PEAK_GBS if fg_ms else None,
+            "nr_tflops": F_NR * e["nr_total"] / (nr_ms * 1e-3) / 1e12 if nr_ms else None,
             "fft_green_bytes_per_voxel": fft_green_bytes_per_voxel(rb, n[2]),
         }
+        if name in fs:
+            f = fs[name]
+            per[name]["from_start"] = {"increments": args.from_start, "seconds": f["seconds"],
+                                       "value": args.from_start * nvox / f["seconds"],
+                                       "outer_iters_per_increment": f["iters"] / args.from_start}
     out["fp64"] = per.get("fp64")
     out["mixed"] = per.get("mixed")
     if "fp64" in res and "mixed" in res:
-        a, b = res["mixed"]["sig_avg"], res["fp64"]["sig_avg"]
-        fm, ff = res["mixed"]["fields"], res["fp64"]["fields"]
         out["mixed_vs_fp64"] = {"speedup": per["mixed"]["value"] / per["fp64"]["value"],
-                                "rel_macro_stress_error": float(np.linalg.norm(a - b) / np.linalg.norm(b)),
-                                "max_voxel_von_mises_error_rel": float(np.abs(fm["vm"] - ff["vm"]).max() / ff["vm"].max()),
-                                "max_slip_system_rho_s_error_rel": float(np.abs(fm["rho_s"] - ff["rho_s"]).max()
-                                                                         / np.abs(ff["rho_s"]).max())}
+                                **stress_errors(res["mixed"], res["fp64"])}
+        if fs:
+            out["mixed_vs_fp64_from_start"] = {
+                "increments": args.from_start,
+                "speedup": per["mixed"]["from_start"]["value"] / per["fp64"]["from_start"]["value"],
+                **stress_errors(fs["mixed"], fs["fp64"])}
     # roofline of the dominant kernel (largest share in the breakdown run); the Newton kernel is
-    # timed with HIP events in the headline run's own timed region
+    # timed with HIP events on the solver's stream in the Newton-event run over the same increments
     bk = brk[head]["kernels"]
     dom = max(bk, key=lambda k: bk[k][0])
-    kt = H["kernels"] if dom == "nr" else bk
     if dom == "nr":
-        ach = F_NR * H["nr_total"] / (kt["nr"][0] * 1e-3) / 1e12
-        out["roofline"] = {"kernel": "nr", "bound": "mfma", "peak_kind": "FP64 (vector = matrix rate on MI355X)",
+        E = nre[head]
+        kt = E["kernels"]
+        ach = F_NR * E["nr_total"] / (kt["nr"][0] * 1e-3) / 1e12
+        out["roofline"] = {"kernel": "nr", "bound": "fp64_valu", "peak_kind": "FP64 vector ALU (no MFMA in this kernel)",
                            "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
                            "frac": ach / FP64_PEAK_TFLOPS, "traffic": NR_TRAFFIC_BYTES_PER_VOXEL * nvox,
-                           "traffic_unit": "bytes per launch (rocprofv3 PMC)",
-                           "algorithmic": "%.0f flop per voxel per Newton iteration" % F_NR}
+                           "traffic_unit": "HBM bytes per launch (rocprofv3 PMC, profiles/)",
+                           "algorithmic": "%.0f flop per voxel per Newton iteration x %.3f Newton iterations per voxel "
+                                          "per launch" % (F_NR, E["nr_total"] / nvox / max(E["iters"], 1))}
     else:
+        kt = bk
         rb = 4 if head == "mixed" else 8
-        per_launch = {"green_x": None}
-        # bytes of the single pass: read 6 comps + write 9 comps of the half spectrum
         h = (n[2] // 2 + 1) / n[2]
         pass_bytes = {"fwd_z": (48 + 6 * 2 * rb * h), "fwd_y": 6 * 2 * 2 * rb * h, "green_x": 15 * 2 * rb * h,
                       "inv_y": 9 * 2 * 2 * rb * h, "inv_z": 9 * 2 * rb * h + 144}
@@ -342,13 +383,19 @@ def main() -> int:
         out["roofline"] = {"kernel": dom, "bound": "hbm", "achieved": ach, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                            "frac": ach / HBM_PEAK_GBS, "traffic": None,
                            "algorithmic": "%.1f B per voxel per launch" % bpv}
-        del per_launch
+    try:   # the achievable HBM rate of a plain streaming copy, beside the FFT passes' rate
+        hs = srx_amd.hbm_stream(1 << 30, 5)
+        out["hbm_copy_ceiling"] = hs
+        for name in per:
+            if per[name].get("fft_green_gbs"):
+                per[name]["fft_green_frac_of_copy_ceiling"] = per[name]["fft_green_gbs"] / hs["best_gbs"]
+    except Exception as exc:
+        out["hbm_copy_ceiling"] = {"error": str(exc)}
     out["roofline"]["avg_launch_ms"] = kt[dom][0] / max(kt[dom][1], 1)
+    out["roofline"]["launches"] = kt[dom][1]
     if not args.no_cpu_baseline:
-        ipi = (res.get("fp64") or H)["iters"] / args.steps
         print("bench: cpu baseline sample", file=sys.stderr, flush=True)
-        ref = res.get("fp64") or H
-        out["cpu_baseline"] = cpu_baseline(deck, nvox, ipi, args.cpu_iters, ref["nr_total"] / nvox / max(ref["iters"], 1))
+        out["cpu_baseline"] = cpu_baseline(args.cpu_n, args.cpu_grains, args.seed, args.cpu_increments)
     print(json.dumps(out))
     return 0
 

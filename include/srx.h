@@ -33,7 +33,11 @@ enum { SRX_PREC_FP64 = 0, SRX_PREC_MIXED = 1 };
 enum {
     SRX_OK = 0,
     SRX_E_FILE = 2001,          /* deck, slip-mode or mesh file cannot be opened */
-    SRX_E_GRID = 2002,          /* CellDim not supported (each extent must be 2^a 3^b 5^c, <= 1024) */
+    SRX_E_GRID = 2002,          /* CellDim not supported: each extent must be 2^a 3^b 5^c, <= 1024; the x
+                                   extent CellDim[0] also within the x pass's LDS row (10 n0 complex values
+                                   in 48 KB, 9 n0 / smallest radix <= 1024): up to 300 in FP64 (not 250,
+                                   270) and up to 512 in mixed precision (not 250, 270, 360, 375, 384, 405,
+                                   432, 450, 480, 486, 500) */
     SRX_E_HIP = 2003,           /* HIP runtime error */
     SRX_E_SYSTEMS = 2004,       /* more than 12 slip systems in a mode */
     SRX_E_NOMEM = 2005,         /* allocation failed */
@@ -230,6 +234,15 @@ int srx_mlp_samples(srx_ctx *ctx, float *x, float *t, size_t max, size_t *n);
    n0 = 64, 128 and 256, where it is also the default) and their timed launch durations in
    microseconds. */
 int srx_fft_tiles(srx_ctx *ctx, int *tiles, double *us);
+
+/* Measurement helper (no counterpart in the reference): the HBM ceiling of a plain streaming
+   copy on the current GPU, against which the HBM-bound passes are judged.  Copies `bytes` (at
+   least 1 MiB; two buffers of that size are allocated and freed) with 16-byte accesses for each of
+   12 launch shapes (1, 2 or 4 accesses in flight per lane; 4, 8, 16 or 32 blocks of 256 threads
+   per compute unit), `reps` timed launches each after a warm-up, and returns the best launch's
+   read + write GB/s, the mean over that shape's launches, and the shape. */
+int srx_hbm_stream(size_t bytes, int reps, double *best_gbs, double *mean_gbs, int *best_unroll,
+                   int *best_blocks_per_cu);
 
 /* Text for an error code. */
 const char *srx_strerror(int code);

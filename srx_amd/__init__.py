@@ -67,7 +67,7 @@ FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velg
 EXPORTS = ["srx_open", "srx_step", "srx_run", "srx_get_field", "srx_set_field", "srx_dims", "srx_close",
            "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse",
            "srx_run_ensemble", "srx_mlp_default_train_options", "srx_mlp_train", "srx_mlp_grad",
-           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples", "srx_fft_tiles"]
+           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples", "srx_fft_tiles", "srx_hbm_stream"]
 
 MLP_BLOB = 6210     # SRX_MLP_BLOB
 MLP_NPARAM = 6150   # SRX_MLP_NPARAM
@@ -123,6 +123,7 @@ def lib():
         L.srx_mlp_record.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
         L.srx_mlp_samples.argtypes = [C.c_void_p, F, F, C.c_size_t, P(C.c_size_t)]
         L.srx_fft_tiles.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double)]
+        L.srx_hbm_stream.argtypes = [C.c_size_t, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int), P(C.c_int)]
         _lib = L
     return _lib
 
@@ -260,6 +261,17 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+
+def hbm_stream(nbytes: int = 1 << 30, reps: int = 5) -> dict:
+    """srx_hbm_stream: read + write GB/s of the best plain streaming-copy launch shape on this GPU."""
+    best, mean = C.c_double(), C.c_double()
+    u, b = C.c_int(), C.c_int()
+    rc = lib().srx_hbm_stream(nbytes, reps, C.byref(best), C.byref(mean), C.byref(u), C.byref(b))
+    if rc:
+        raise SrxError(rc, "srx_hbm_stream")
+    return {"best_gbs": best.value, "mean_gbs": mean.value, "unroll": u.value, "blocks_per_cu": b.value,
+            "bytes_each_way": nbytes}
 
 
 def fft_forward(x: np.ndarray, precision: int = PREC_FP64) -> np.ndarray:

@@ -374,7 +374,7 @@ namespace {
         hipError_t e_ = (x);                                                                             \
         if (e_ != hipSuccess) {                                                                          \
             std::fprintf(stderr, "srx: %s failed: %s (%s:%d)\n", #x, hipGetErrorString(e_), __FILE__, __LINE__); \
-            return SRX_E_HIP;                                                                            \
+            return e_ == hipErrorOutOfMemory ? SRX_E_NOMEM : SRX_E_HIP;                                  \
         }                                                                                                \
     } while (0)
 
@@ -383,8 +383,12 @@ std::string resolve(const std::string &dir, const char *name) {
     return dir + "/" + name;
 }
 
-// largest LDS tile check: every pass must fit its butterflies and 64 KB of LDS
-bool grid_supported(const FftPlan pl[3], const int n[3]) {
+// The passes' limits, as the launchers size them: a y or z pass holds one pencil (padded pitch
+// n + 1) and the twiddle table in 48 KB of LDS with at most 2 butterflies per thread; the Green x
+// pass (LDS kernel, one row per block) holds 9 component rows and the twiddle table, 10 rows of n0
+// complex values (16 B FP64, 8 B mixed), with at most 4 butterflies per thread.
+bool grid_supported(const FftPlan pl[3], const int n[3], int prec) {
+    const long csz = prec == SRX_PREC_MIXED ? 8 : 16;
     for (int ax = 0; ax < 3; ++ax) {
         const FftPlan &p = pl[ax];
         int mr = 1 << 30;
@@ -393,7 +397,8 @@ bool grid_supported(const FftPlan pl[3], const int n[3]) {
         const int slots = ax == 0 ? 9 : 1;
         const int maxb = ax == 0 ? 4 : 2;   // butterflies per thread of the x pass and of the y/z passes
         if ((long)slots * p.n / std::max(mr, 1) > (long)maxb * 256) return false;
-        if ((long)slots * p.n * 16 > 48 * 1024) return false;   // one tile row within the LDS budget
+        const long lds = ax == 0 ? 10L * p.n * csz : (2L * p.n + 1) * csz;
+        if (lds > 48 * 1024) return false;
     }
     return n[0] >= 1 && n[1] >= 1 && n[2] >= 1;
 }
@@ -431,7 +436,7 @@ int setup(srx_ctx *c) {
     for (int i = 0; i < 3; ++i) c->n[i] = d.CellDim[i];
     for (int i = 0; i < 3; ++i)
         if (!make_plan(c->n[i], c->pl[i])) return SRX_E_GRID;
-    if (!grid_supported(c->pl, c->n)) return SRX_E_GRID;
+    if (!grid_supported(c->pl, c->n, c->opt.precision)) return SRX_E_GRID;
     c->nvox = (long)c->n[0] * c->n[1] * c->n[2];
     c->wgt = 1. / c->nvox;
     const double kT = (d.T__K * BOLZ * J_IN_EV);
@@ -656,6 +661,7 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.gamdot, 12, 0.0)) || (rc = dalloc(&F.gam_acum, 12, 0.0)) ||
         (rc = dalloc(&F.rho_dipole, 12, 0.0)) || (rc = dalloc(&F.rho_forest, 12, 0.0)) ||
         (rc = dalloc(&F.rho_inplane, 12, 0.0)) || (rc = dalloc(&F.rho_tot, 1, 0.0)) ||
+        (rc = dalloc(&F.acr, 15, 0.0)) || (rc = dalloc(&F.nrc, 36, 0.0)) ||
         (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
     if (c->opt.gnd) {
@@ -876,12 +882,17 @@ static int autotune_tiles(srx_ctx *c) {
     for (int pass = 0; pass < 5 && !rc; ++pass) {
         const FftPlan &p = c->pl[pass == 2 ? 0 : (pass == 1 || pass == 3) ? 1 : 2];
         const int tmax = fft_tile_max(pass, c->n, p, prec);
-        // the default first (the largest tile; the wavefront x pass where it applies); another
-        // candidate must be more than 3% faster to replace it, so near-ties do not flip between runs
+        // the default first (the largest tile); another candidate must be more than 3% faster to
+        // replace it, so near-ties do not flip between runs.  Candidates differ only in pencils per
+        // block, which leaves every result bit-identical; the wavefront x pass (a different
+        // operation order) is not timed against the LDS kernel but always used where it applies, so
+        // results never depend on the timings (or on what else the GPU runs meanwhile).
         std::vector<int> cand;
         if (pass == 2 && green_wave_ok(c->n)) cand.push_back(-1);   // one wavefront per row
-        cand.push_back(tmax);
-        for (int t = 1; t < tmax; t *= 2) cand.push_back(t);
+        else {
+            cand.push_back(tmax);
+            for (int t = 1; t < tmax; t *= 2) cand.push_back(t);
+        }
         double best = 1e300;
         for (int t : cand) {
             auto run = [&]() {
@@ -1384,6 +1395,56 @@ int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int 
     return 0;
 }
 
+int srx_hbm_stream(size_t bytes, int reps, double *best_gbs, double *mean_gbs, int *best_unroll, int *best_blocks) {
+    if (!best_gbs || bytes < (1u << 20) || reps < 1) return SRX_E_ARG;
+    const size_t n16 = bytes / 16;
+    void *a = nullptr, *b = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return SRX_E_HIP;
+    if (hipMalloc(&a, n16 * 16) != hipSuccess || hipMalloc(&b, n16 * 16) != hipSuccess) rc = SRX_E_NOMEM;
+    if (!rc && (hipMemset(a, 0, n16 * 16) != hipSuccess || hipMemset(b, 0, n16 * 16) != hipSuccess ||
+                hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess))
+        rc = SRX_E_HIP;
+    double best = 0.0, mean = 0.0;
+    int bu = 0, bb = 0;
+    const int unrolls[3] = {1, 2, 4}, per_cu[4] = {4, 8, 16, 32};
+    for (int iu = 0; iu < 3 && !rc; ++iu)
+        for (int ib = 0; ib < 4 && !rc; ++ib) {
+            const int blocks = cus * per_cu[ib];
+            launch_stream_copy(nullptr, a, b, n16, unrolls[iu], blocks);
+            double sum = 0.0, top = 0.0;
+            for (int r = 0; r < reps && !rc; ++r) {
+                float ms = 0.0f;
+                if (hipEventRecord(e0, nullptr) != hipSuccess) { rc = SRX_E_HIP; break; }
+                launch_stream_copy(nullptr, a, b, n16, unrolls[iu], blocks);
+                if (hipEventRecord(e1, nullptr) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                    hipEventElapsedTime(&ms, e0, e1) != hipSuccess) { rc = SRX_E_HIP; break; }
+                const double gbs = 2.0 * (double)n16 * 16.0 / (ms * 1e-3) / 1e9;
+                sum += gbs;
+                top = std::max(top, gbs);
+            }
+            if (top > best) {
+                best = top;
+                mean = sum / reps;
+                bu = unrolls[iu];
+                bb = per_cu[ib];
+            }
+        }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (a) (void)hipFree(a);
+    if (b) (void)hipFree(b);
+    if (rc) return rc;
+    *best_gbs = best;
+    if (mean_gbs) *mean_gbs = mean;
+    if (best_unroll) *best_unroll = bu;
+    if (best_blocks) *best_blocks = bb;
+    return 0;
+}
+
 int srx_kernel_times(srx_ctx *c, const char **names, double *ms, int *calls, int *count) {
     if (!c || !count) return SRX_E_ARG;
     const int n = std::min(*count, c->nkt);
@@ -1543,7 +1604,7 @@ void srx_close(srx_ctx *c) {
     if (c->ferr) std::fclose(c->ferr);
     double *ptrs[] = {c->f.sig, c->f.sig_pre, c->f.eps, c->f.edot, c->f.dg, c->f.dg0, c->f.tran, c->f.tran0,
                       c->f.rho_s, c->f.rho_m, c->f.rho_m_old, c->f.rho_F, c->f.rho_P, c->f.gamdot, c->f.gam_acum,
-                      c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->f.rho_tot, c->f.fl0,
+                      c->f.rho_dipole, c->f.rho_forest, c->f.rho_inplane, c->f.rho_tot, c->f.fl0, c->f.acr, c->f.nrc,
                       c->hc.rho_g1, c->hc.rho_g2, c->hc.gnd_fp,
                       c->partials, c->red, c->redseg};
     for (double *p : ptrs)
@@ -1652,7 +1713,7 @@ static int fft_test_common(int ncomp, int n0, int n1, int n2, int prec, FftPlan 
     const int n[3] = {n0, n1, n2};
     for (int i = 0; i < 3; ++i)
         if (!make_plan(n[i], pl[i])) return SRX_E_GRID;
-    if (!grid_supported(pl, n) || ncomp < 1 || ncomp > 9) return SRX_E_GRID;
+    if (!grid_supported(pl, n, prec) || ncomp < 1 || ncomp > 9) return SRX_E_GRID;
     for (int ax = 0; ax < 3; ++ax) {
         int r = prec == SRX_PREC_MIXED ? upload_twiddles<float>(n[ax], &tw[ax]) : upload_twiddles<double>(n[ax], &tw[ax]);
         if (r) return r;

@@ -1148,8 +1148,20 @@ __device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const do
 }
 
 // Staging slots of the Newton kernel's inputs in LDS (one 64-voxel slice per component).
-constexpr int ST_SIG = 0, ST_EPS = 6, ST_DG = 11, ST_TRAN = 20, ST_TRAN0 = 29;
-constexpr int ST_COMPS_ROT = 38, ST_COMPS = 29;
+constexpr int ST_SIG = 0, ST_EPS = 6, ST_DG = 11, ST_TRAN = 20;
+constexpr int ST_COMPS = 29;
+// LDS slots of the per-system constants (tau_pass * inv, inv, gamma_0 dt of 12 systems), also the
+// component order of Fields::nrc
+constexpr int NRC_TP = 0, NRC_INV = NSYS, NRC_G0 = 2 * NSYS, NRC_COMPS = 3 * NSYS;
+
+// Per-voxel values every Newton call of an increment starts from, in the crystal frame of the
+// current orientation: lam' = Q^T lam, q' = Q^T (eps_p - eps_c), the matrix A = Sc + S' (packed),
+// r0 = q' - Sc lam' (= Q^T (eps_p - eps_c - S066 lam), Q being orthogonal), the hydrostatic parts
+// and 1/||lam||^2.
+struct NrPro {
+    double lam[5], q[5], r0[5], A[15];
+    double lamh, qh, rsig2;
+};
 
 // Input component k of a field: from the LDS staging slice (lane-major, stg points at this
 // lane's element of slot 0) when STAGED, else from global memory.
@@ -1160,79 +1172,110 @@ __device__ __forceinline__ double nr_in(const double *stg, int slot, const doubl
 
 // WS: Newton start 0 = lam (the reference), 1 = lam + the previous call's change (extrapolated),
 // 2 = lam + the learned prediction `wsp` (crystal frame)
-template <bool FIRST, bool ROT, int PH, int WS, bool GND>
-__device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                                         double *lds, const double *stg, const float *wsp, NrResult &res) {
+// Prologue of a Newton call.  FIRST (the increment's first call): inputs from global memory, A
+// assembled from the orientations (Sc = Q^T S066 Q and the rotated grain compliance S') and stored
+// in f.acr for the increment's later calls.  Otherwise: inputs from the LDS staging slots (stg
+// points at this lane's element of slot 0) and A from f.acr.
+template <bool FIRST, bool ROT, int PH>
+__device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
+                                            const double *stg, NrPro &pro) {
     constexpr bool STAGED = !FIRST;
-    // compile-time phase index: every material constant is a scalar load at a fixed offset
     const DevPhase &P = cst->ph[PH];
-    const double dt = cst->dt;
-    double lam[5], q[5], lamh, qh, rsig2;
-    double A[15], r0[5];
-    double ws[5] = {0, 0, 0, 0, 0}, wsh = 0.0;   // warm start: crystal-frame offset of the first iterate
-    {
-        double a[9];
+    if (!FIRST) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = nr_in<STAGED>(stg, ST_TRAN, f.tran, k, N, v);
-        double Q[5][5];
-        q5_from(a, Q);
-        double lam6[6], ec[6], ep[5];
+        for (int k = 0; k < 15; ++k) pro.A[k] = f.acr[k * N + v];
+    }
+    double a[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a[k] = nr_in<STAGED>(stg, ST_TRAN, f.tran, k, N, v);
+    double Q[5][5];
+    q5_from(a, Q);
+    double lam6[6], w[5];
+    {
+        double ec[6], ep[5];
 #pragma unroll
         for (int k = 0; k < 6; ++k) lam6[k] = nr_in<STAGED>(stg, ST_SIG, f.sig, k, N, v);
 #pragma unroll
         for (int k = 0; k < 5; ++k) ep[k] = nr_in<STAGED>(stg, ST_EPS, f.eps, k, N, v);
-        {
-            double d[9];
+        double d[9];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) d[k] = nr_in<STAGED>(stg, ST_DG, f.dg, k, N, v);
-            // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
-            const double s00 = d[0], s11 = d[4], s22 = d[8];
-            const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
-            ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
-            ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
-            ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
-            ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
-            ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
-            ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
+        for (int k = 0; k < 9; ++k) d[k] = nr_in<STAGED>(stg, ST_DG, f.dg, k, N, v);
+        // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
+        const double s00 = d[0], s11 = d[4], s22 = d[8];
+        const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
+        ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
+        ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
+        ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
+        ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
+        ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
+        ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) w[k] = ep[k] - ec[k];
+        pro.qh = -ec[5];
+    }
+    double sn = 0.0;
+#pragma unroll
+    for (int k = 0; k < 6; ++k) sn += lam6[k] * lam6[k];
+    pro.rsig2 = 1.0 / sn;
+    pro.lamh = lam6[5];
+    // u = S066 lam (deviatoric block, sample frame)
+    double u[5];
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double acc = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) acc += cst->S066[i][k] * lam6[k];
+        u[i] = acc;
+    }
+    // crystal frame: lam' = Q^T lam, q' = Q^T w, r0 = q' - Q^T u
+#pragma unroll
+    for (int i = 0; i < 5; ++i) {
+        double al = 0.0, aq = 0.0, au = 0.0;
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            al += Q[k][i] * lam6[k];
+            aq += Q[k][i] * w[k];
+            au += Q[k][i] * u[k];
         }
-        double sn = 0.0;
-#pragma unroll
-        for (int k = 0; k < 6; ++k) sn += lam6[k] * lam6[k];
-        rsig2 = 1.0 / sn;
-        // crystal frame: lam' = Q^T lam, q' = Q^T (eps_p - eps_c)
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            double al = 0.0, aq = 0.0;
-#pragma unroll
-            for (int k = 0; k < 5; ++k) {
-                al += Q[k][i] * lam6[k];
-                aq += Q[k][i] * (ep[k] - ec[k]);
-            }
-            lam[i] = al;
-            q[i] = aq;
-        }
-        lamh = lam6[5];
-        qh = -ec[5];
-        sc_rotate(&cst->S066[0][0], Q, A);
-#pragma unroll
-        for (int i = 0; i < 5; ++i) {
-            double acc = q[i];
-#pragma unroll
-            for (int j = 0; j < 5; ++j) acc -= A[ps(i, j)] * lam[j];
-            r0[i] = acc;
-        }
+        pro.lam[i] = al;
+        pro.q[i] = aq;
+        pro.r0[i] = aq - au;
+    }
+    if (FIRST) {
+        sc_rotate(&cst->S066[0][0], Q, pro.A);
         if (ROT) {
             double a0[9], Sp[15];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a0[k] = nr_in<STAGED>(stg, ST_TRAN0, f.tran0, k, N, v);
+            for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
             sprime(a, a0, P.sinv, Sp);
 #pragma unroll
-            for (int k = 0; k < 15; ++k) A[k] += Sp[k];
+            for (int k = 0; k < 15; ++k) pro.A[k] += Sp[k];
         } else {
 #pragma unroll
-            for (int i = 0; i < 5; ++i) A[ps(i, i)] += P.sinv[i];
+            for (int i = 0; i < 5; ++i) pro.A[ps(i, i)] += P.sinv[i];
         }
+#pragma unroll
+        for (int k = 0; k < 15; ++k) f.acr[k * N + v] = pro.A[k];
     }
+}
+
+// WS: Newton start 0 = lam (the reference), 1 = lam + the previous call's change (extrapolated),
+// 2 = lam + the learned prediction `wsp` (crystal frame).  FIRST: the call computes its prologue
+// and per-system constants and stores them for the increment; otherwise the caller has put the
+// prologue in `pro` and the constants in the LDS slots NRC_*.
+template <bool FIRST, bool ROT, int PH, int WS, bool GND>
+__device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
+                                         double *lds, NrPro &pro, const float *wsp, NrResult &res) {
+    // compile-time phase index: every material constant is a scalar load at a fixed offset
+    const DevPhase &P = cst->ph[PH];
+    const double dt = cst->dt;
+    if (FIRST) nr_prologue<true, ROT, PH>(cst, f, v, N, nullptr, pro);
+    double (&lam)[5] = pro.lam;
+    double (&q)[5] = pro.q;
+    double (&A)[15] = pro.A;
+    double (&r0)[5] = pro.r0;
+    const double lamh = pro.lamh, qh = pro.qh, rsig2 = pro.rsig2;
+    double ws[5] = {0, 0, 0, 0, 0}, wsh = 0.0;   // warm start: crystal-frame offset of the first iterate
     const double Shh = cst->S066[5][5];
     const double Ah = Shh + P.sinv[5];
     const double rAh = 1.0 / Ah;
@@ -1240,16 +1283,18 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 
     // per-system constants, kept in LDS (lane-major, conflict free): tau_pass * inv, inv and
     // gamma_0 * dt (the slip rates enter the residual and the Jacobian times dt); the first call of
-    // an increment uses the lagged rho_m in tau_pass
-    double *tp = lds + 0 * NSYS * NR_BLOCK + threadIdx.x, *inv = lds + 1 * NSYS * NR_BLOCK + threadIdx.x;
-    double *g0 = lds + 2 * NSYS * NR_BLOCK + threadIdx.x, *tpo = lds + 3 * NSYS * NR_BLOCK + threadIdx.x;
-    {
+    // an increment computes them (with the lagged rho_m in tau_pass for its own first iterate) and
+    // stores them in f.nrc, from where the caller copies them into LDS for the later calls
+    double *tp = lds + NRC_TP * NR_BLOCK + threadIdx.x, *inv = lds + NRC_INV * NR_BLOCK + threadIdx.x;
+    double *g0 = lds + NRC_G0 * NR_BLOCK + threadIdx.x, *tpo = lds + NRC_COMPS * NR_BLOCK + threadIdx.x;
+    if (FIRST) {
         const int nsys = P.nsys;
         double rs[NSYS];
 #pragma unroll
         for (int s = 0; s < NSYS; ++s) rs[s] = f.rho_s[s * N + v];
 #pragma unroll
         for (int s = 0; s < NSYS; ++s) {
+            double ct, ci, cg;
             if (s < nsys) {
                 double rP, rF;
                 rho_pf(P, rs, s, rP, rF);
@@ -1257,19 +1302,23 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
                 const double rmn = rho_mobile(P, rP, rF);
                 double t, iv, g;
                 sys_const(P, rP, rF, rmn, rmn, t, iv, g);
-                tp[s * NR_BLOCK] = t * iv;
-                inv[s * NR_BLOCK] = iv;
-                g0[s * NR_BLOCK] = g * dt;
-                if (FIRST) {
-                    const double rmo = f.rho_m[s * N + v];
-                    tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo)) * iv;
-                    f.rho_m_old[s * N + v] = rmo;
-                    f.rho_m[s * N + v] = rmn;
-                }
+                ct = t * iv;
+                ci = iv;
+                cg = g * dt;
+                const double rmo = f.rho_m[s * N + v];
+                tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo)) * iv;
+                f.rho_m_old[s * N + v] = rmo;
+                f.rho_m[s * N + v] = rmn;
             } else {
-                g0[s * NR_BLOCK] = 0.0; tp[s * NR_BLOCK] = 1e300; inv[s * NR_BLOCK] = 0.0;
-                if (FIRST) tpo[s * NR_BLOCK] = 1e300;
+                ct = 1e300; ci = 0.0; cg = 0.0;
+                tpo[s * NR_BLOCK] = 1e300;
             }
+            tp[s * NR_BLOCK] = ct;
+            inv[s * NR_BLOCK] = ci;
+            g0[s * NR_BLOCK] = cg;
+            f.nrc[(NRC_TP + s) * N + v] = ct;
+            f.nrc[(NRC_INV + s) * N + v] = ci;
+            f.nrc[(NRC_G0 + s) * N + v] = cg;
         }
     }
 
@@ -1500,28 +1549,22 @@ __device__ __forceinline__ void copy_to_lds(const void *src, double *lds, int by
 #endif
 }
 
-// Copies components [0, NC) of the staging layout for voxels [v0, v0 + 64) of this block into
-// `stage` (slot c at stage + 64 c).  With an even voxel count every component slice is 16-byte
-// aligned and one 16-byte load per lane moves two components; otherwise 4-byte loads.  Lanes
-// past the last voxel re-read the last valid element (the values are never used).
-template <int NC>
-__device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage) {
-    const size_t N = (size_t)nvox;
+// Copies components [0, NC) of a component-major array (component c at base(c) + voxel) for the
+// voxels [v0, v0 + 64) of this block into LDS slots (slot c at lds + 64 c), then waits for them.
+// With an even voxel count every component slice is 16-byte aligned and one 16-byte load per lane
+// moves two components; otherwise 4-byte loads.  Lanes past the last voxel re-read the last valid
+// element (the values are never used).  Every lane of the block must take part: a lane's copy
+// fills other lanes' elements.
+template <int NC, typename Base>
+__device__ __forceinline__ void copy_slots(Base base, int nvox, double *lds) {
     const int lane = threadIdx.x;
     const int v0 = blockIdx.x * NR_BLOCK;
-    auto base = [&](int c) -> const double * {
-        return c < ST_EPS ? f.sig + (size_t)(c - ST_SIG) * N
-             : c < ST_DG ? f.eps + (size_t)(c - ST_EPS) * N
-             : c < ST_TRAN ? f.dg + (size_t)(c - ST_DG) * N
-             : c < ST_TRAN0 ? f.tran + (size_t)(c - ST_TRAN) * N
-             : f.tran0 + (size_t)(c - ST_TRAN0) * N;
-    };
     if ((nvox & 1) == 0) {
         const int pair = min(v0 + 2 * (lane & 31), nvox - 2);
 #pragma unroll
         for (int c = 0; c < NC; c += 2) {
             const int cc = min(c + (lane >> 5), NC - 1);
-            copy_to_lds(base(cc) + pair, stage + c * NR_BLOCK, 16);
+            copy_to_lds(base(cc) + pair, lds + c * NR_BLOCK, 16);
         }
     } else {
 #pragma unroll
@@ -1531,11 +1574,25 @@ __device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *
                 const int d = h * 64 + lane;
                 const int vv = min(v0 + (d >> 1), nvox - 1);
                 const char *src = reinterpret_cast<const char *>(base(c) + vv) + 4 * (d & 1);
-                copy_to_lds(src, stage + c * NR_BLOCK + h * 32, 4);
+                copy_to_lds(src, lds + c * NR_BLOCK + h * 32, 4);
             }
     }
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+}
+
+// The Newton inputs of the block's voxels (stress, plastic strain, DisGrad, orientation) into the
+// staging slots ST_*.
+template <int NC>
+__device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage) {
+    const size_t N = (size_t)nvox;
+    auto base = [&](int c) -> const double * {
+        return c < ST_EPS ? f.sig + (size_t)(c - ST_SIG) * N
+             : c < ST_DG ? f.eps + (size_t)(c - ST_EPS) * N
+             : c < ST_TRAN ? f.dg + (size_t)(c - ST_DG) * N
+             : f.tran + (size_t)(c - ST_TRAN) * N;
+    };
+    copy_slots<NC>(base, nvox, stage);
 }
 
 // Learned start of the wavefront's voxels (every lane calls; non-crystalline lanes get zero).
@@ -1584,20 +1641,29 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
     // After the first call of an increment, the wavefront's inputs (stress, plastic strain,
-    // DisGrad, orientations) are copied to LDS by one burst of asynchronous global->LDS loads
-    // and a single wait, instead of several dependent round trips to HBM.  Each lane's
-    // per-system constants then overwrite its own, already consumed, staging slots, so the
-    // block needs no more LDS than the larger of the two (an even number of slots: the
-    // 16-byte copies fill slots in pairs).
-    constexpr int NST = FIRST ? 0 : (ROT ? ST_COMPS_ROT : ST_COMPS);
-    constexpr int NCONST = (FIRST ? 4 : 3) * NSYS;
+    // DisGrad, orientation) are copied to LDS by one burst of asynchronous global->LDS loads and a
+    // single wait, instead of several dependent round trips to HBM.  Once every lane has formed
+    // its prologue from them, the per-system constants the first call stored overwrite the
+    // staging slots (by a second burst), so the block needs no more LDS than the larger of the
+    // two.  The first call computes both and keeps its lagged tau_pass in four more slices.
+    constexpr int NST = FIRST ? 0 : ST_COMPS;
+    constexpr int NCONST = FIRST ? NRC_COMPS + NSYS : NRC_COMPS;
     constexpr int NSLOT = ((NST + 1) & ~1) > NCONST ? ((NST + 1) & ~1) : NCONST;
     __shared__ __align__(16) double lds[NSLOT * NR_BLOCK];
-    if constexpr (!FIRST) stage_inputs<NST>(f, nvox, lds);
-    const double *stg = lds + threadIdx.x;
+    NrPro pro;
     float wsp[6] = {0, 0, 0, 0, 0, 0};
-    if constexpr (!FIRST && WS == 2)
-        nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, stg, wsp);
+    if constexpr (!FIRST) {
+        stage_inputs<NST>(f, nvox, lds);
+        const double *stg = lds + threadIdx.x;
+        if constexpr (WS == 2)
+            nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, stg, wsp);
+        // every lane forms its prologue (lanes past the last voxel from the last one's inputs)
+        nr_prologue<false, ROT, 0>(cst, f, valid ? v : nvox - 1, N, stg, pro);
+        // the staging slots are consumed (all LDS reads returned) before the constants land on them
+        __builtin_amdgcn_s_waitcnt(0xc07f);
+        const double *nrc = f.nrc;
+        copy_slots<NRC_COMPS>([&](int c) { return nrc + (size_t)c * N; }, nvox, lds);
+    }
     NrResult r;
     r.cs = 0.0; r.ce = 0.0; r.it = 0; r.itot = 0; r.bad = 0;
 #pragma unroll
@@ -1610,9 +1676,9 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS, GND>(cst, f, v, N, lds, stg, wsp, r);
+                if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS, GND>(cst, f, v, N, lds, pro, wsp, r);
             } else {
-                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS, GND>(cst, f, v, N, lds, stg, wsp, r);
+                if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS, GND>(cst, f, v, N, lds, pro, wsp, r);
             }
         }
         todo &= ~__ballot(mine);
@@ -2639,6 +2705,38 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
                            0., nullptr);
     }
     check_launch("inv_z(test)");
+}
+
+// ---------------------------------------------------------------------------
+// HBM ceiling: a plain streaming copy, 16 bytes per lane per access, U independent accesses in
+// flight per lane, grid-stride over the buffer.  The measured rate is the ceiling the HBM-bound
+// passes above are compared with.
+// ---------------------------------------------------------------------------
+typedef double dvec2 __attribute__((ext_vector_type(2)));
+template <int U>
+__global__ void __launch_bounds__(256) k_stream_copy(const dvec2 *__restrict__ in, dvec2 *__restrict__ out,
+                                                      size_t n) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x;
+    for (; i + (U - 1) * stride < n; i += U * stride) {
+        dvec2 v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) v[u] = __builtin_nontemporal_load(in + i + u * stride);
+#pragma unroll
+        for (int u = 0; u < U; ++u) __builtin_nontemporal_store(v[u], out + i + u * stride);
+    }
+    for (; i < n; i += stride) out[i] = in[i];
+}
+
+void launch_stream_copy(hipStream_t s, const void *in, void *out, size_t n16, int unroll, int blocks) {
+    const dim3 grid(blocks), block(256);
+    if (unroll == 4)
+        hipLaunchKernelGGL(k_stream_copy<4>, grid, block, 0, s, (const dvec2 *)in, (dvec2 *)out, n16);
+    else if (unroll == 2)
+        hipLaunchKernelGGL(k_stream_copy<2>, grid, block, 0, s, (const dvec2 *)in, (dvec2 *)out, n16);
+    else
+        hipLaunchKernelGGL(k_stream_copy<1>, grid, block, 0, s, (const dvec2 *)in, (dvec2 *)out, n16);
+    check_launch("stream_copy");
 }
 
 }  // namespace srx

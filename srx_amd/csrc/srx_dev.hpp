@@ -183,6 +183,10 @@ struct Fields {
     double *rho_dipole, *rho_forest, *rho_inplane;                          // [12][N]
     double *rho_tot;   // [N] rho_tot of the last hardening update (evolution.c:1030-1048)
     double *fl0;       // [9][N] fluctuation carried over DisGrad resets (creep runs only, else null)
+    double *acr;       // [15][N] crystal-frame Newton matrix A = Q^T S066 Q + S' (packed symmetric 5x5) of the
+                       //   increment, written by its first Newton call, read by the later ones
+    double *nrc;       // [36][N] per-system Newton constants of the increment, written by its first call:
+                       //   tau_pass/(tau_0+tau_cut) [12], 1/(tau_0+tau_cut) [12], gamma_0 dt [12]
     float *dsig;       // [6][N] last Newton call's stress change, B basis, crystal frame (warm start only)
     const MlpPacked *mlp;   // learned Newton start (non-null: the start comes from the network)
     int *phase;        // [N]
@@ -256,6 +260,9 @@ int setup_block_count(int nvox);
 void launch_setup(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, const double *eul, double deg,
                   const double *e9, const double *sc9, int creep, double *partials, double *out);
 int nr_block_count(int nvox);
+// Streaming copy of n16 16-byte elements (the HBM ceiling measurement): `unroll` (1, 2, 4)
+// accesses in flight per lane, `blocks` blocks of 256 threads, grid-stride.
+void launch_stream_copy(hipStream_t s, const void *in, void *out, size_t n16, int unroll, int blocks);
 int end_block_count(int nvox);
 // Field conversion helpers used by srx_get_field (device -> AoS tensor on device scratch)
 void launch_b6_to_tensor(hipStream_t s, const double *v, int ncomp, int nvox, double *out9);
