@@ -46,6 +46,7 @@ enum {
     SRX_E_SIZE = 2008,          /* caller buffer too small */
     SRX_E_FIELD = 2009,         /* unknown field id */
     SRX_E_ARG = 2010,           /* invalid argument */
+    SRX_E_SCRATCH = 2011,       /* problem larger than the static scratch of srx_fft_forward_noalloc */
     SRX_E_NAN = 3001,           /* NaN in the constitutive update (the reference asserts, constitutive.c:660,683,707) */
     SRX_E_UPDATE_FLAG = 3002    /* Update_Flag 0 leaves Schm_gr uninitialised at istep 0 (init.c:815): rejected */
 };
@@ -265,6 +266,20 @@ int srx_fft_forward(const double *in, int ncomp, int n0, int n1, int n2, int pre
 
 /* Inverse of srx_fft_forward for real output: out = IFFT(in) * scale. */
 int srx_fft_inverse(const double *in, int ncomp, int n0, int n1, int n2, int precision, double scale, double *out);
+
+/* srx_fft_forward without any allocation.  No counterpart in the reference: it is for callers that
+   cannot allocate (code under HIP graph capture, tight serving loops, harnesses that forbid
+   hipMalloc).  d_in (ncomp*n0*n1*n2 doubles) and d_out (ncomp*n0*n1*(n2/2+1) complex values as
+   (re, im) doubles, srx_fft_forward's order) are device buffers of the caller; the two half spectra
+   and the twiddle tables live in one static device array of SRX_NOALLOC_SCRATCH_BYTES per GPU, held
+   until the process ends.  Needs 3 tables of n complex values (each rounded up to 256 bytes) plus
+   2 x ncomp*n0*n1*(n2/2+1) complex values (16 B FP64, 8 B mixed); larger problems return
+   SRX_E_SCRATCH.  Calls that overlap on one GPU share that array and race.  Synchronous: the
+   result is in d_out on return.  Twiddles are computed on the device, so results agree with
+   srx_fft_forward to rounding (not bitwise). */
+#define SRX_NOALLOC_SCRATCH_BYTES (32u << 20)
+int srx_fft_forward_noalloc(const double *d_in, int ncomp, int n0, int n1, int n2, int precision,
+                            double *d_out);
 
 #ifdef __cplusplus
 }

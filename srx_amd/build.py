@@ -29,11 +29,13 @@ def build(verbose_resources: bool = False) -> None:
     hobj = os.path.join(objdir, "host.o")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
     mobj = os.path.join(objdir, "mlp.o")
+    nobj = os.path.join(objdir, "noalloc.o")
     run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "kernels.hip"), "-o", kobj])
     run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "mlp.hip"), "-o", mobj])
+    run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "noalloc.hip"), "-o", nobj])
     run([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
          os.path.join(CSRC, "host.cpp"), "-o", hobj])
-    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", LIB, kobj, mobj, hobj])
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", LIB, kobj, mobj, nobj, hobj])
     run([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
          os.path.join(CSRC, "cli.cpp"), "-o", CLI, "-L" + HERE, "-lsrx", "-Wl,-rpath,$ORIGIN",
          "-L/opt/rocm/lib", "-lamdhip64"])

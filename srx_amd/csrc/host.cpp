@@ -403,6 +403,12 @@ bool grid_supported(const FftPlan pl[3], const int n[3], int prec) {
     return n[0] >= 1 && n[1] >= 1 && n[2] >= 1;
 }
 
+}  // namespace
+
+bool srx::srx_grid_supported(const FftPlan pl[3], const int n[3], int prec) { return grid_supported(pl, n, prec); }
+
+namespace {
+
 template <typename T>
 int upload_twiddles(int n, void **dst, hipStream_t st = nullptr) {
     std::vector<T> h(2 * std::max(n, 1));
@@ -856,6 +862,7 @@ const char *srx_strerror(int code) {
     case SRX_E_HIP: return "HIP runtime error";
     case SRX_E_SYSTEMS: return "more than 12 slip systems";
     case SRX_E_NOMEM: return "out of memory";
+    case SRX_E_SCRATCH: return "problem larger than the static scratch of the allocation-free entry point";
     case SRX_E_DONE: return "all increments done";
     case SRX_E_CREEP_STOP: return "creep run stopped at e_vm > 0.3";
     case SRX_E_SIZE: return "buffer too small";
