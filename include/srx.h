@@ -168,26 +168,35 @@ typedef struct {
 
 /* Runs every increment of n independent decks (an ensemble of RVEs, each the equivalent of one
    `trial <deck>` job) on this GPU.  Every member is set up first (all stay resident on the GPU),
-   then the increments run `concurrency` members at a time (0: 4), each on its own stream from its
-   own host thread, largest grids first; then all are closed.  Output files go to workdirs[i] (NULL:
-   each deck's directory).  results[i] is filled for every deck; returns 0 or the first member's
-   error code.  No counterpart in the reference, which runs one RVE per mpirun job. */
+   then the increments run, then all are closed.  concurrency <= 0: lockstep -- the members are
+   dealt into max(1, -concurrency) groups; the members of a group advance one increment at a time
+   together, the Newton solve, its reduction and the outer-iteration control of all of them running
+   as one launch each over the group, their FFT passes on up to 8 / groups streams; the groups run
+   concurrently, each from its own host thread.  concurrency k > 0: k members at a time, each on
+   its own stream from its own host thread, largest grids first.  Every mode gives each member the
+   results of its deck run alone.
+   Output files go to workdirs[i] (NULL: each deck's directory).  results[i] is filled for every
+   deck; returns 0 or the first member's error code.  No counterpart in the reference, which runs
+   one RVE per mpirun job. */
 int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int n, const srx_options *opt,
                      int concurrency, srx_ensemble_result *results);
 
 /* ---- learned Newton start (no counterpart in the reference) ----------------------------
    A 24-64-64-6 ReLU perceptron, evaluated in bf16 on the matrix cores with FP32 accumulation
-   inside the Newton kernel, corrects each voxel's extrapolated start: the solve starts at
-   lam + |d| (d / |d| + c), d the previous Newton call's stress change, c the network's output
-   (limited to |c| <= 1; c = 0 is the warm_start 1 start).  The solve still iterates to its own
+   inside the Newton kernel, sets the step length of each voxel's extrapolated start: the solve
+   starts at lam + alpha d, d the previous Newton call's stress change, alpha = 1 + c . d / |d|
+   with c the network's output, limited to 0 <= alpha <= 1 (c = 0 is the warm_start 1 start).
+   The solve still iterates to its own
    tolerance (evolution.c:217-223) and is repeated from lam if it fails to converge, so the
    network only changes how many iterations it takes.
    Parameters travel as one FP32 blob of SRX_MLP_BLOB floats: W1[24][64], b1[64], W2[64][64],
    b2[64], W3[64][6], b3[6] (W[k][i]: input k -> unit i), feature mean[24] and inverse spread[24],
    target mean[6] and spread[6].  Features per voxel (raw): orientation quaternion (w >= 0),
    direction of d (crystal frame, B basis), the start stress lam in the crystal frame / 100 MPa,
-   log10 rho_s of systems 1-7, log10(|d| / |lam|).  Target: the coming call's change over |d|,
-   minus d / |d|. */
+   the forcing of the coming solve e / (|d| + |e|), e = 1e5 MPa x (eps_c - eps_c of the previous
+   call) in the crystal frame (eps_c = sym(DisGrad), the strain the solve is driven by,
+   evolution.c:192-201), log10(|d| / |lam|) and log10 of the mean rho_s.  Target: the coming
+   call's change over |d|, minus d / |d|. */
 #define SRX_MLP_BLOB 6210
 #define SRX_MLP_NPARAM 6150
 

@@ -124,29 +124,42 @@ def crystal_stress(a9, lam6) -> np.ndarray:
     return np.einsum("nij,kij->nk", sc, BASIS)
 
 
-def features(a9, lam6, d6, rho7) -> tuple[np.ndarray, np.ndarray]:
-    """Raw features [n][24] and previous-change magnitudes [n] (definition in include/srx.h)."""
+E_SCALE = 1.0e5   # MPa, MLP_E_SCALE of mlp_dev.hpp
+
+
+def features(a9, lam6, d6, dec6, rho12) -> tuple[np.ndarray, np.ndarray]:
+    """Raw features [n][24] and previous-change magnitudes [n] (definition in include/srx.h):
+    quaternion, direction of d, lam / 100 MPa (crystal frame), the forcing e / (|d| + |e|) with
+    e = E_SCALE * (eps_c - eps_c of the previous call) in the crystal frame (dec6 is that strain
+    difference in the sample frame), log10(|d| / |lam|), log10 of the mean rho_s."""
     q = quaternion(a9)
     lc = crystal_stress(a9, lam6)
+    ec = crystal_stress(a9, dec6)
     d = np.asarray(d6, np.float64)
     nd = np.sqrt((d * d).sum(1))
     u = np.where(nd[:, None] > 0, d / np.where(nd > 0, nd, 1.0)[:, None], 0.0)
     ln = np.sqrt((lc * lc).sum(1))
-    x = np.concatenate([q, u, lc * 0.01, np.log10(np.maximum(np.asarray(rho7, np.float64), 1e-30)),
-                        np.log10(np.maximum(nd, 1e-300) / np.maximum(ln, 1e-300))[:, None]], 1)
+    en = E_SCALE * np.sqrt((ec * ec).sum(1))
+    den = nd + en
+    f = np.where(den[:, None] > 0, E_SCALE * ec / np.where(den > 0, den, 1.0)[:, None], 0.0)
+    rho = np.asarray(rho12, np.float64).mean(1)
+    x = np.concatenate([q, u, lc * 0.01, f, np.log10(np.maximum(nd, 1e-300) / np.maximum(ln, 1e-300))[:, None],
+                        np.log10(np.maximum(rho, 1e-30))[:, None]], 1)
     return x.astype(np.float32), nd.astype(np.float32)
 
 
 def start(y, u, nd) -> np.ndarray:
-    """Newton start offsets [n][6]: nd * (u + c), c = the network's de-normalised correction y
-    limited to |c| <= 1, u the previous change's direction; zero where nd is zero or not finite."""
-    c = np.asarray(y, np.float64)
-    n = np.sqrt((c * c).sum(1))
-    f = np.where(n <= 1.0, 1.0, np.where(np.isfinite(n) & (n > 0), 1.0 / np.where(n > 0, n, 1.0), 0.0))
+    """Newton start offsets [n][6]: alpha nd u, alpha = 1 + y . u (the network's de-normalised
+    correction projected on the previous change's direction u) limited to [0, 1], 1 where not
+    finite; zero where nd is zero or not finite."""
+    y = np.asarray(y, np.float64)
+    u = np.asarray(u, np.float64)
+    with np.errstate(invalid="ignore"):
+        alpha = 1.0 + (y * u).sum(1)
+    alpha = np.where(np.isfinite(alpha), np.clip(alpha, 0.0, 1.0), 1.0)
     nd = np.asarray(nd, np.float64)
     ok = (nd > 0) & (nd < 1e30)
-    w = nd[:, None] * (np.asarray(u, np.float64) + np.where(f[:, None] > 0, c * f[:, None], 0.0))
-    return np.where(ok[:, None], w, 0.0)
+    return np.where(ok[:, None], (alpha * nd)[:, None] * u, 0.0)
 
 
 def random_blob(seed: int = 0, xm=None, xs=None, tm=None, ts=None) -> np.ndarray:

@@ -348,7 +348,9 @@ def mlp_forward(blob: np.ndarray, x: np.ndarray) -> np.ndarray:
 
 
 def run_ensemble(decks, workdirs=None, concurrency: int = 4, **opt) -> list[dict]:
-    """srx_run_ensemble: every increment of each deck, `concurrency` RVEs at a time on this GPU.
+    """srx_run_ensemble: every increment of each deck on this GPU, `concurrency` RVEs at a time on
+    their own streams, or (concurrency 0) all in lockstep with the per-voxel kernels of all members
+    in one launch each.
     Returns one dict per deck (rc, increments, voxels, setup_seconds, run_seconds, run_phase_seconds
     (wall time of all members' increments, setup excluded), last report);
     raises SrxError if any member failed."""

@@ -696,6 +696,10 @@ int setup(srx_ctx *c) {
         HIPCK(hipMalloc(&F.dsig, sizeof(float) * 6 * N));
         if ((rc = ctx_zero(st, F.dsig, sizeof(float) * 6 * N))) return rc;
     }
+    if (c->opt.warm_start == 2) {   // the learned start's forcing feature (srx_mlp_record adds it too)
+        HIPCK(hipMalloc(&F.ecp, sizeof(float) * 6 * N));
+        if ((rc = ctx_zero(st, F.ecp, sizeof(float) * 6 * N))) return rc;
+    }
     HIPCK(hipMalloc(&F.phase, sizeof(int) * N));
     HIPCK(hipMalloc(&F.nr_it, sizeof(int) * N));
     if ((rc = ctx_zero(st, F.nr_it, sizeof(int) * N)) ||
@@ -1117,17 +1121,16 @@ static int write_field_dumps(srx_ctx *c, int step) {
     return 0;
 }
 
-int srx_step(srx_ctx *c, srx_step_report *rep) {
-    if (!c) return SRX_E_ARG;
-    if (c->istep >= c->nsteps) return SRX_E_DONE;
-    if (c->creep == 1 && c->e_vm > 0.3) return SRX_E_CREEP_STOP;
-    if (c->opt.warm_start == 2 && !c->mlp) return SRX_E_ARG;   // srx_mlp_set first
+// One increment (one pass of the istep loop of evolution.c:726-1520) in three phases, so that an
+// ensemble can run the outer iterations of all its members together (run_lockstep):
+// step_begin: the increment prologue and the control block of its outer iterations;
+// step_iterate: the outer iterations of this context alone; step_finish: everything after them.
+static int step_begin(srx_ctx *c) {
     const Deck &d = c->dk;
     const double dt = d.TimeStep;
     const int prec = c->opt.precision;
     const int N = (int)c->nvox;
-    srx_step_report r;
-    std::memset(&r, 0, sizeof(r));
+    (void)dt; (void)prec; (void)N; (void)d;
     if (c->ferr && c->nsteps != 1) std::fprintf(c->ferr, "STEP = %d\n", c->istep);
 
     // evolution.c:1115-1137
@@ -1140,7 +1143,6 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     if (c->creep == 1) for (int k = 0; k < 9; ++k) c->uniform[k] = set[k];
     else for (int k = 0; k < 9; ++k) c->uniform[k] += add[k];
     for (int k = 0; k < 9; ++k) c->dDisGradAvg[k] = c->acum[k] = 0.0;
-    const int rotated = c->istep > 2 ? 1 : 0;   // update_orient first runs at the end of istep 2
     // outer fixed point (evolution.c:1150-1256), controlled on the device: the host enqueues
     // iterations in batches and reads the control block once per batch
     IterCtl &ctl = *c->hctl;
@@ -1169,6 +1171,17 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     for (int k = 0; k < 9; ++k) ctl.uniform[k] = c->uniform[k];
     ctl.done = ctl_stop(ctl);
     HIPCK(hipMemcpyAsync(c->dctl, &ctl, sizeof(IterCtl), hipMemcpyHostToDevice, c->stream));
+    return 0;
+}
+
+static int step_iterate(srx_ctx *c) {
+    const Deck &d = c->dk;
+    const double dt = d.TimeStep;
+    const int prec = c->opt.precision;
+    const int N = (int)c->nvox;
+    (void)dt; (void)prec; (void)N; (void)d;
+    const int rotated = c->istep > 2 ? 1 : 0;
+    IterCtl &ctl = *c->hctl;
     const double divisor = (double)c->nvox;
     const int batch = c->opt.verbose ? 1 : ITER_BATCH;
     const int most = std::max(c->itermax, c->opt.fixed_iters);
@@ -1197,7 +1210,7 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
             if (rec)
                 launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 0, c->rec_x, c->rec_t, c->rec_nd,
                                   c->rec_count, c->rec_cap, g);
-            { Timer t(c, 5, launched); launch_nr(c->stream, c->dc, c->f, N, launched == 1, rotated, c->partials, g, c->opt.gnd); }
+            { Timer t(c, 5, launched); launch_nr(c->stream, c->dc, c->f, N, launched == 1, rotated, c->partials, g, c->opt.gnd, c->hc.ph[1].xtal); }
             if (rec)
                 launch_mlp_record(c->stream, c->dc, c->f, N, c->rec_stride, 1, c->rec_x, c->rec_t, c->rec_nd,
                                   c->rec_count, c->rec_cap, g);
@@ -1221,6 +1234,19 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
         HIPCK(hipStreamSynchronize(c->stream));
         if (c->opt.verbose) std::printf("iter %d Err_e %e Err_s %e\n", ctl.iter, ctl.Err_e, ctl.Err_s);
     }
+    return 0;
+}
+
+static int step_finish(srx_ctx *c, srx_step_report *rep) {
+    const Deck &d = c->dk;
+    const double dt = d.TimeStep;
+    const int prec = c->opt.precision;
+    const int N = (int)c->nvox;
+    (void)dt; (void)prec; (void)N; (void)d;
+    const int rotated = c->istep > 2 ? 1 : 0;
+    IterCtl &ctl = *c->hctl;
+    srx_step_report r;
+    std::memset(&r, 0, sizeof(r));
     if (ctl.nan) return SRX_E_NAN;
     const int iter = ctl.iter, nrmax = ctl.nrmax;
     const long nrtot = (long)ctl.nrtot;
@@ -1328,6 +1354,17 @@ int srx_step(srx_ctx *c, srx_step_report *rep) {
     return frc;
 }
 
+int srx_step(srx_ctx *c, srx_step_report *rep) {
+    if (!c) return SRX_E_ARG;
+    if (c->istep >= c->nsteps) return SRX_E_DONE;
+    if (c->creep == 1 && c->e_vm > 0.3) return SRX_E_CREEP_STOP;
+    if (c->opt.warm_start == 2 && !c->mlp) return SRX_E_ARG;   // srx_mlp_set first
+    int rc = step_begin(c);
+    if (!rc) rc = step_iterate(c);
+    if (!rc) rc = step_finish(c, rep);
+    return rc;
+}
+
 int srx_run(srx_ctx *c) {
     if (!c) return SRX_E_ARG;
     while (c->istep < c->nsteps) {
@@ -1336,6 +1373,130 @@ int srx_run(srx_ctx *c) {
         if (rc) return rc;
     }
     return 0;
+}
+
+// Lockstep ensemble run: every member advances one increment at a time together.  Each member's
+// prologue and end of increment run in turn, its FFT passes on one of up to eight side streams
+// (members spread over them), and the Newton solve, its reduction and the iteration control of all
+// members run as one launch each over the member table; the host reads back only the members' stop
+// flags, once per batch of ITER_BATCH iterations.
+static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> &ctx, srx_ensemble_result *results,
+                        int max_side) {
+    const int M = (int)members.size();
+    if (M == 0) return 0;
+    hipStream_t S = ctx[members[0]]->stream;
+    const int NS = std::max(1, std::min(max_side, M));
+    std::vector<hipStream_t> side(NS, nullptr);
+    std::vector<hipEvent_t> ev(NS, nullptr);
+    hipEvent_t fork = nullptr;
+    EnsMember *dtab = nullptr, *htab = nullptr;
+    int *ddone = nullptr, *hdone = nullptr;
+    std::vector<hipStream_t> own(M);
+    int rc = 0;
+    auto ok = [&](hipError_t e) {
+        if (e != hipSuccess && !rc) rc = e == hipErrorOutOfMemory ? SRX_E_NOMEM : SRX_E_HIP;
+        return rc == 0;
+    };
+    for (int q = 0; q < NS && ok(hipStreamCreateWithFlags(&side[q], hipStreamNonBlocking)); ++q)
+        ok(hipEventCreateWithFlags(&ev[q], hipEventDisableTiming));
+    ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
+    ok(hipMalloc(&dtab, sizeof(EnsMember) * M));
+    ok(hipHostMalloc((void **)&htab, sizeof(EnsMember) * M, hipHostMallocDefault));
+    ok(hipMalloc(&ddone, sizeof(int) * M));
+    ok(hipHostMalloc((void **)&hdone, sizeof(int) * M, hipHostMallocDefault));
+    // the members order their own copies and kernels on the main stream during the run
+    for (int j = 0; j < M; ++j) {
+        own[j] = ctx[members[j]]->stream;
+        ctx[members[j]]->stream = S;
+    }
+    while (!rc) {
+        std::vector<int> act;
+        for (int k : members) {
+            srx_ctx *c = ctx[k];
+            if (results[k].rc) continue;
+            if (c->istep >= c->nsteps || (c->creep == 1 && c->e_vm > 0.3)) continue;
+            if (c->opt.warm_start == 2 && !c->mlp) { results[k].rc = SRX_E_ARG; continue; }
+            act.push_back(k);
+        }
+        if (act.empty()) break;
+        const int nact = (int)act.size();
+        const int istep = ctx[act[0]]->istep;
+        for (int k : act)
+            if (ctx[k]->istep != istep) rc = SRX_E_ARG;   // members advance together from istep 0
+        if (rc) break;
+        const int rotated = istep > 2 ? 1 : 0;
+        int nb = 0, most = 0;
+        for (int j = 0; j < nact && !rc; ++j) {
+            srx_ctx *c = ctx[act[j]];
+            if (int e = step_begin(c)) rc = e;
+            htab[j].cst = c->dc;
+            htab[j].f = c->f;
+            htab[j].nvox = (int)c->nvox;
+            htab[j].blk0 = nb;
+            htab[j].nblk = c->nr_blocks;
+            htab[j].pad = 0;
+            htab[j].partials = c->partials;
+            htab[j].redseg = c->redseg;
+            htab[j].ctl = c->dctl;
+            nb += c->nr_blocks;
+            most = std::max(most, std::max(c->itermax, c->opt.fixed_iters));
+        }
+        if (rc || !ok(hipMemcpyAsync(dtab, htab, sizeof(EnsMember) * nact, hipMemcpyHostToDevice, S))) break;
+        const srx_options &o = ctx[act[0]]->opt;
+        int ph2 = 0;
+        for (int k : act) ph2 |= ctx[k]->hc.ph[1].xtal;
+        int launched = 0;
+        bool all_done = false;
+        while (!all_done && !rc) {
+            if (launched > most) { rc = SRX_E_HIP; break; }
+            for (int b = 0; b < ITER_BATCH && ok(hipEventRecord(fork, S)); ++b) {
+                ++launched;
+                for (int q = 0; q < NS; ++q) ok(hipStreamWaitEvent(side[q], fork, 0));
+                for (int j = 0; j < nact; ++j) {
+                    srx_ctx *c = ctx[act[j]];
+                    hipStream_t st = side[j % NS];
+                    const int prec = c->opt.precision;
+                    const int *tl = c->tiles;
+                    IterCtl *g = c->dctl;
+                    launch_fft_fwd_z(st, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], g, tl[0]);
+                    launch_fft_y(st, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], g, tl[1]);
+                    launch_green_x(st, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, g, tl[2]);
+                    launch_fft_y(st, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], g, tl[3]);
+                    launch_fft_inv_z_update(st, prec, c->specB, c->f.dg, c->n, c->pl[2], c->tw[2], g,
+                                            (double)c->nvox, tl[4]);
+                }
+                for (int q = 0; q < NS; ++q) ok(hipEventRecord(ev[q], side[q])) && ok(hipStreamWaitEvent(S, ev[q], 0));
+                launch_nr_ens(S, dtab, nact, nb, launched == 1, rotated, o.warm_start, o.gnd, ph2);
+                launch_reduce_nr_ens(S, dtab, nact);
+                launch_control_ens(S, dtab, nact, ddone);
+            }
+            if (!ok(hipMemcpyAsync(hdone, ddone, sizeof(int) * nact, hipMemcpyDeviceToHost, S)) ||
+                !ok(hipStreamSynchronize(S)))
+                break;
+            all_done = true;
+            for (int j = 0; j < nact; ++j) all_done = all_done && hdone[j] != 0;
+        }
+        if (rc) break;
+        for (int j = 0; j < nact; ++j)
+            ok(hipMemcpyAsync(ctx[act[j]]->hctl, ctx[act[j]]->dctl, sizeof(IterCtl), hipMemcpyDeviceToHost, S));
+        if (!ok(hipStreamSynchronize(S))) break;
+        for (int k : act) {
+            const int e = step_finish(ctx[k], &results[k].last);
+            if (e) results[k].rc = e;
+            else ++results[k].increments;
+        }
+    }
+    for (int j = 0; j < M; ++j) ctx[members[j]]->stream = own[j];
+    for (int q = 0; q < NS; ++q) {
+        if (side[q]) (void)hipStreamDestroy(side[q]);
+        if (ev[q]) (void)hipEventDestroy(ev[q]);
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (dtab) (void)hipFree(dtab);
+    if (ddone) (void)hipFree(ddone);
+    if (htab) (void)hipHostFree(htab);
+    if (hdone) (void)hipHostFree(hdone);
+    return rc;
 }
 
 int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int n, const srx_options *opt,
@@ -1377,8 +1538,29 @@ int srx_run_ensemble(const char *const *decks, const char *const *workdirs, int 
         results[i].rc = srx_open(decks[i], workdirs ? workdirs[i] : nullptr, &o, &ctx[i]);
         results[i].setup_seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
     });
-    // 2. the increments, `workers` members at a time, each on its own stream
+    // 2. the increments: all members in lockstep (concurrency 0) or `workers` members at a time, each
+    // on its own stream
     const auto r0 = std::chrono::steady_clock::now();
+    if (concurrency <= 0) {
+        // lockstep groups: members dealt round-robin (largest first) into -concurrency groups (0: one),
+        // each group in lockstep on its own streams from its own host thread, the groups concurrently
+        const int ngroups = std::max(1, std::min(-concurrency, n));
+        std::vector<std::vector<int>> groups(ngroups);
+        int gi = 0;
+        for (int k : order)
+            if (results[k].rc == 0) groups[gi++ % ngroups].push_back(k);
+        std::vector<int> grc(ngroups, 0);
+        std::vector<std::thread> th;
+        for (int g = 0; g < ngroups; ++g)
+            th.emplace_back([&, g]() { grc[g] = run_lockstep(groups[g], ctx, results, std::max(1, 8 / ngroups)); });
+        for (auto &t : th) t.join();
+        const double t = std::chrono::duration<double>(std::chrono::steady_clock::now() - r0).count();
+        for (int g = 0; g < ngroups; ++g)
+            for (int k : groups[g]) {
+                results[k].run_seconds = t;
+                if (grc[g] && !results[k].rc) results[k].rc = grc[g];
+            }
+    } else
     pool([&](int i) {
         srx_ensemble_result &r = results[i];
         if (r.rc) return;
@@ -1617,6 +1799,7 @@ void srx_close(srx_ctx *c) {
     for (double *p : ptrs)
         if (p) (void)hipFree(p);
     if (c->f.dsig) (void)hipFree(c->f.dsig);
+    if (c->f.ecp) (void)hipFree(c->f.ecp);
     if (c->hc.gbc) (void)hipFree(c->hc.gbc);
     if (c->mlp) (void)hipFree(c->mlp);
     if (c->mlp_blob) (void)hipFree(c->mlp_blob);
@@ -1670,6 +1853,12 @@ int srx_mlp_record(srx_ctx *c, int stride, size_t capacity) {
     free_recorder(c);
     if (stride == 0) return 0;
     if (!c->f.dsig || capacity == 0 || capacity > ((size_t)1 << 28)) return SRX_E_ARG;
+    if (!c->f.ecp) {
+        // the features need the previous Newton call's eps_c; graphs captured without it are dropped
+        HIPCK(hipMalloc(&c->f.ecp, sizeof(float) * 6 * (size_t)c->nvox));
+        if (int e = ctx_zero(c->stream, c->f.ecp, sizeof(float) * 6 * (size_t)c->nvox)) return e;
+        drop_graphs(c);
+    }
     const size_t cap = capacity;
     HIPCK(hipMalloc(&c->rec_x, sizeof(float) * 24 * cap));
     HIPCK(hipMalloc(&c->rec_t, sizeof(float) * 6 * cap));

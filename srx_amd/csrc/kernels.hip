@@ -1154,6 +1154,18 @@ constexpr int ST_COMPS = 29;
 // component order of Fields::nrc
 constexpr int NRC_TP = 0, NRC_INV = NSYS, NRC_G0 = 2 * NSYS, NRC_COMPS = 3 * NSYS;
 
+// eps_c = (DisGrad + DisGrad^T) / 2 in the B basis (chg_basis opt 2), from the 9 row-major components
+__device__ __forceinline__ void ec_from_dg(const double d[9], double ec[6]) {
+    const double s00 = d[0], s11 = d[4], s22 = d[8];
+    const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
+    ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
+    ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
+    ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
+    ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
+    ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
+    ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
+}
+
 // Per-voxel values every Newton call of an increment starts from, in the crystal frame of the
 // current orientation: lam' = Q^T lam, q' = Q^T (eps_p - eps_c), the matrix A = Sc + S' (packed),
 // r0 = q' - Sc lam' (= Q^T (eps_p - eps_c - S066 lam), Q being orthogonal), the hydrostatic parts
@@ -1176,9 +1188,11 @@ __device__ __forceinline__ double nr_in(const double *stg, int slot, const doubl
 // assembled from the orientations (Sc = Q^T S066 Q and the rotated grain compliance S') and stored
 // in f.acr for the increment's later calls.  Otherwise: inputs from the LDS staging slots (stg
 // points at this lane's element of slot 0) and A from f.acr.
-template <bool FIRST, bool ROT, int PH>
+// KEEP_EC (warm starts): eps_c is also stored in f.ecp where `store` is set, for the next call's
+// learned-start features.
+template <bool FIRST, bool ROT, int PH, bool KEEP_EC>
 __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                                            const double *stg, NrPro &pro) {
+                                            const double *stg, NrPro &pro, bool store = true) {
     constexpr bool STAGED = !FIRST;
     const DevPhase &P = cst->ph[PH];
     if (!FIRST) {
@@ -1201,14 +1215,11 @@ __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, co
 #pragma unroll
         for (int k = 0; k < 9; ++k) d[k] = nr_in<STAGED>(stg, ST_DG, f.dg, k, N, v);
         // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
-        const double s00 = d[0], s11 = d[4], s22 = d[8];
-        const double s12 = (d[5] + d[7]) / 2.0, s02 = (d[2] + d[6]) / 2.0, s01 = (d[1] + d[3]) / 2.0;
-        ec[0] = s00 * (-SRX_R2) + s11 * SRX_R2;
-        ec[1] = s00 * (-SRX_R6) + s11 * (-SRX_R6) + s22 * (2. * SRX_R6);
-        ec[2] = s12 * SRX_R2 + s12 * SRX_R2;
-        ec[3] = s02 * SRX_R2 + s02 * SRX_R2;
-        ec[4] = s01 * SRX_R2 + s01 * SRX_R2;
-        ec[5] = s00 * SRX_R3 + s11 * SRX_R3 + s22 * SRX_R3;
+        ec_from_dg(d, ec);
+        if (KEEP_EC && store && f.ecp) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) f.ecp[k * N + v] = (float)ec[k];
+        }
 #pragma unroll
         for (int k = 0; k < 5; ++k) w[k] = ep[k] - ec[k];
         pro.qh = -ec[5];
@@ -1269,7 +1280,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     // compile-time phase index: every material constant is a scalar load at a fixed offset
     const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
-    if (FIRST) nr_prologue<true, ROT, PH>(cst, f, v, N, nullptr, pro);
+    if (FIRST) nr_prologue<true, ROT, PH, (WS != 0)>(cst, f, v, N, nullptr, pro);
     double (&lam)[5] = pro.lam;
     double (&q)[5] = pro.q;
     double (&A)[15] = pro.A;
@@ -1556,9 +1567,9 @@ __device__ __forceinline__ void copy_to_lds(const void *src, double *lds, int by
 // element (the values are never used).  Every lane of the block must take part: a lane's copy
 // fills other lanes' elements.
 template <int NC, typename Base>
-__device__ __forceinline__ void copy_slots(Base base, int nvox, double *lds) {
+__device__ __forceinline__ void copy_slots(Base base, int nvox, double *lds, int blk) {
     const int lane = threadIdx.x;
-    const int v0 = blockIdx.x * NR_BLOCK;
+    const int v0 = blk * NR_BLOCK;
     if ((nvox & 1) == 0) {
         const int pair = min(v0 + 2 * (lane & 31), nvox - 2);
 #pragma unroll
@@ -1584,7 +1595,7 @@ __device__ __forceinline__ void copy_slots(Base base, int nvox, double *lds) {
 // The Newton inputs of the block's voxels (stress, plastic strain, DisGrad, orientation) into the
 // staging slots ST_*.
 template <int NC>
-__device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage) {
+__device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage, int blk) {
     const size_t N = (size_t)nvox;
     auto base = [&](int c) -> const double * {
         return c < ST_EPS ? f.sig + (size_t)(c - ST_SIG) * N
@@ -1592,7 +1603,7 @@ __device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *
              : c < ST_TRAN ? f.dg + (size_t)(c - ST_DG) * N
              : f.tran + (size_t)(c - ST_TRAN) * N;
     };
-    copy_slots<NC>(base, nvox, stage);
+    copy_slots<NC>(base, nvox, stage, blk);
 }
 
 // Learned start of the wavefront's voxels (every lane calls; non-crystalline lanes get zero).
@@ -1601,18 +1612,25 @@ __device__ __forceinline__ void nr_learned_start(const DevConst *__restrict__ cs
                                                  bool xt, const double *stg, float wsp[6]) {
     float xf[MLP_IN], nd = 0.0f;
     if (xt) {
-        double a[9], lam6[6], lamc[6], rho[7];
+        double a[9], lam6[6], lamc[6], dg[9], de[6], dec[6], rho = 0.0;
         float d[6];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[k] = stg[(ST_TRAN + k) * NR_BLOCK];
 #pragma unroll
         for (int k = 0; k < 6; ++k) lam6[k] = stg[(ST_SIG + k) * NR_BLOCK];
         crystal_stress(a, lam6, lamc);
+        // forcing: eps_c now minus eps_c of the previous call, crystal frame
+#pragma unroll
+        for (int k = 0; k < 9; ++k) dg[k] = stg[(ST_DG + k) * NR_BLOCK];
+        ec_from_dg(dg, de);
+#pragma unroll
+        for (int k = 0; k < 6; ++k) de[k] -= (double)f.ecp[k * N + v];
+        crystal_stress(a, de, dec);
 #pragma unroll
         for (int k = 0; k < 6; ++k) d[k] = f.dsig[k * N + v];
 #pragma unroll
-        for (int k = 0; k < 7; ++k) rho[k] = f.rho_s[k * N + v];
-        mlp_features(a, lamc, d, rho, xf, nd);
+        for (int k = 0; k < NSYS; ++k) rho += f.rho_s[k * N + v];
+        mlp_features(a, lamc, d, dec, rho / NSYS, xf, nd);
     } else {
 #pragma unroll
         for (int k = 0; k < MLP_IN; ++k) xf[k] = 0.0f;
@@ -1630,13 +1648,15 @@ __device__ __forceinline__ void nr_learned_start(const DevConst *__restrict__ cs
     for (int k = 0; k < 6; ++k) wsp[k] = (float)w6[k];
 }
 
-template <bool FIRST, bool ROT, int WS, bool GND = false>
-__global__ void __launch_bounds__(NR_BLOCK)
-__attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8)))
-k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ partials,
-     const IterCtl *__restrict__ ctl) {
+// One 64-voxel block of the Newton kernel: block `blk` of the `nblk` blocks of a grid of nvox
+// voxels (partials: NR_NPART x nblk block partial sums).
+// PH2: the second phase is crystalline (else its voxels are gas and its Newton path is not compiled).
+template <bool FIRST, bool ROT, int WS, bool GND, bool PH2>
+__device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const Fields &f, int nvox,
+                                         double *__restrict__ partials, const IterCtl *__restrict__ ctl, int blk,
+                                         int nblk) {
     if (ctl && ctl->done) return;
-    const int v = blockIdx.x * NR_BLOCK + threadIdx.x;
+    const int v = blk * NR_BLOCK + threadIdx.x;
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
@@ -1653,16 +1673,16 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
     NrPro pro;
     float wsp[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (!FIRST) {
-        stage_inputs<NST>(f, nvox, lds);
+        stage_inputs<NST>(f, nvox, lds, blk);
         const double *stg = lds + threadIdx.x;
         if constexpr (WS == 2)
             nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, stg, wsp);
         // every lane forms its prologue (lanes past the last voxel from the last one's inputs)
-        nr_prologue<false, ROT, 0>(cst, f, valid ? v : nvox - 1, N, stg, pro);
+        nr_prologue<false, ROT, 0, (WS != 0)>(cst, f, valid ? v : nvox - 1, N, stg, pro, valid);
         // the staging slots are consumed (all LDS reads returned) before the constants land on them
         __builtin_amdgcn_s_waitcnt(0xc07f);
         const double *nrc = f.nrc;
-        copy_slots<NRC_COMPS>([&](int c) { return nrc + (size_t)c * N; }, nvox, lds);
+        copy_slots<NRC_COMPS>([&](int c) { return nrc + (size_t)c * N; }, nvox, lds, blk);
     }
     NrResult r;
     r.cs = 0.0; r.ce = 0.0; r.it = 0; r.itot = 0; r.bad = 0;
@@ -1677,7 +1697,7 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
         if (mine) {
             if (pu == 1) {
                 if (cst->ph[0].xtal) nr_voxel<FIRST, ROT, 0, WS, GND>(cst, f, v, N, lds, pro, wsp, r);
-            } else {
+            } else if constexpr (PH2) {
                 if (cst->ph[1].xtal) nr_voxel<FIRST, ROT, 1, WS, GND>(cst, f, v, N, lds, pro, wsp, r);
             }
         }
@@ -1711,8 +1731,28 @@ k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ 
     red[17] = red[15];
     if (lane == 0) {
 #pragma unroll
-        for (int k = 0; k < NR_NPART; ++k) partials[(size_t)k * gridDim.x + blockIdx.x] = red[k];
+        for (int k = 0; k < NR_NPART; ++k) partials[(size_t)k * nblk + blk] = red[k];
     }
+}
+
+template <bool FIRST, bool ROT, int WS, bool GND, bool PH2>
+__global__ void __launch_bounds__(NR_BLOCK)
+__attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8)))
+k_nr(const DevConst *__restrict__ cst, Fields f, int nvox, double *__restrict__ partials,
+     const IterCtl *__restrict__ ctl) {
+    nr_block<FIRST, ROT, WS, GND, PH2>(cst, f, nvox, partials, ctl, blockIdx.x, gridDim.x);
+}
+
+// The Newton kernel over every member of an ensemble in one launch: member m owns the blocks
+// [tab[m].blk0, tab[m].blk0 + tab[m].nblk) of the grid.
+template <bool FIRST, bool ROT, int WS, bool GND, bool PH2>
+__global__ void __launch_bounds__(NR_BLOCK)
+__attribute__((amdgpu_waves_per_eu(FIRST ? SRX_NR_FIRST_WAVES_PER_EU : SRX_NR_WAVES_PER_EU, 8)))
+k_nr_ens(const EnsMember *__restrict__ tab, int nmem) {
+    int m = 0;
+    while (m + 1 < nmem && (int)blockIdx.x >= tab[m + 1].blk0) ++m;
+    const EnsMember &e = tab[m];
+    nr_block<FIRST, ROT, WS, GND, PH2>(e.cst, e.f, e.nvox, e.partials, e.ctl, blockIdx.x - e.blk0, e.nblk);
 }
 
 // fixed-order reduction of block partials stored component-major:
@@ -1758,11 +1798,10 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__
 // wavefronts, then the four wavefront results), into seg[k * RED_SEGS + sg].  k_control finishes.
 constexpr int RED_SEGS = 16, RED_SEG_BLOCK = 256;
 static_assert(NR_RED_SEG_DOUBLES == NR_NPART * RED_SEGS, "segment buffer size");
-__global__ void __launch_bounds__(RED_SEG_BLOCK) k_reduce_seg(const double *__restrict__ part, int nblocks,
-                                                              unsigned long long maxmask, double *__restrict__ seg,
-                                                              const IterCtl *__restrict__ ctl) {
+__device__ __forceinline__ void reduce_seg_block(const double *__restrict__ part, int nblocks,
+                                                 unsigned long long maxmask, double *__restrict__ seg,
+                                                 const IterCtl *__restrict__ ctl, int k, int sg) {
     if (ctl && ctl->done) return;
-    const int k = blockIdx.x, sg = blockIdx.y;
     const bool ismax = (maxmask >> k) & 1ull;
     const int chunk = (nblocks + RED_SEGS - 1) / RED_SEGS;
     const int b1 = min(nblocks, (sg + 1) * chunk);
@@ -1788,13 +1827,27 @@ __global__ void __launch_bounds__(RED_SEG_BLOCK) k_reduce_seg(const double *__re
     }
 }
 
+__global__ void __launch_bounds__(RED_SEG_BLOCK) k_reduce_seg(const double *__restrict__ part, int nblocks,
+                                                              unsigned long long maxmask, double *__restrict__ seg,
+                                                              const IterCtl *__restrict__ ctl) {
+    reduce_seg_block(part, nblocks, maxmask, seg, ctl, blockIdx.x, blockIdx.y);
+}
+
+// the same for every member of an ensemble: grid (NR_NPART, RED_SEGS, members)
+__global__ void __launch_bounds__(RED_SEG_BLOCK) k_reduce_seg_ens(const EnsMember *__restrict__ tab,
+                                                                  unsigned long long maxmask) {
+    const EnsMember &e = tab[blockIdx.z];
+    reduce_seg_block(e.partials, e.nblk, maxmask, e.redseg, e.ctl, blockIdx.x, blockIdx.y);
+}
+
 // One outer iteration's bookkeeping on the device (the host loop of evolution.c:1150-1256):
 // the dDisGradAvg this iteration's inverse-z pass applied joins the uniform part, the reduced
 // sums give the errors and get_smacro's averages and correction, and `done` decides whether the
 // next iteration runs.  One thread; the iteration's other kernels have finished (stream order).
 __device__ __forceinline__ void control_step(const DevConst *__restrict__ cst, IterCtl &c, const double *red);
 
-__global__ void k_control(const DevConst *__restrict__ cst, const double *__restrict__ seg, IterCtl *ctl) {
+__device__ __forceinline__ void control_block(const DevConst *__restrict__ cst, const double *__restrict__ seg,
+                                              IterCtl *ctl) {
     if (ctl->done) return;
     // The control block is copied to LDS by the whole block, worked on there by one thread (no
     // dependent HBM round trips) and copied back.
@@ -1817,6 +1870,19 @@ __global__ void k_control(const DevConst *__restrict__ cst, const double *__rest
     __syncthreads();
     for (int i = threadIdx.x; i < NW; i += blockDim.x)
         reinterpret_cast<unsigned long long *>(ctl)[i] = reinterpret_cast<const unsigned long long *>(&lc)[i];
+}
+
+__global__ void k_control(const DevConst *__restrict__ cst, const double *__restrict__ seg, IterCtl *ctl) {
+    control_block(cst, seg, ctl);
+}
+
+// the same for every member of an ensemble (one block each); done[m] receives the member's stop
+// flag, the only value the host reads between batches of iterations
+__global__ void k_control_ens(const EnsMember *__restrict__ tab, int *__restrict__ done) {
+    const EnsMember &e = tab[blockIdx.x];
+    control_block(e.cst, e.redseg, e.ctl);
+    __syncthreads();
+    if (threadIdx.x == 0) done[blockIdx.x] = e.ctl->done;
 }
 
 // one outer iteration's bookkeeping on the reduced values red[] (thread 0 of k_control)
@@ -2220,14 +2286,18 @@ __global__ void k_mlp_rec_x(const DevConst *__restrict__ cst, Fields f, int nvox
     const int ph = f.phase[v];
     float x[MLP_IN], nd = 0.0f;
     if (ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal) {
-        double a[9], lam6[6], lamc[6], rho[7];
+        double a[9], lam6[6], lamc[6], dg[9], de[6], dec[6], rho = 0.0;
         float d[6];
         for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
         for (int k = 0; k < 6; ++k) lam6[k] = f.sig[k * N + v];
         crystal_stress(a, lam6, lamc);
+        for (int k = 0; k < 9; ++k) dg[k] = f.dg[k * N + v];
+        ec_from_dg(dg, de);
+        for (int k = 0; k < 6; ++k) de[k] -= (double)f.ecp[k * N + v];
+        crystal_stress(a, de, dec);
         for (int k = 0; k < 6; ++k) d[k] = f.dsig[k * N + v];
-        for (int k = 0; k < 7; ++k) rho[k] = f.rho_s[k * N + v];
-        mlp_features(a, lamc, d, rho, x, nd);
+        for (int k = 0; k < NSYS; ++k) rho += f.rho_s[k * N + v];
+        mlp_features(a, lamc, d, dec, rho / NSYS, x, nd);
     } else {
         for (int k = 0; k < MLP_IN; ++k) x[k] = 0.0f;
     }
@@ -2590,30 +2660,38 @@ void launch_setup(hipStream_t s, const DevConst *cst, const Fields &f, int nvox,
 int nr_block_count(int nvox) { return (nvox + NR_BLOCK - 1) / NR_BLOCK; }
 int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
+// Kernel variant of a Newton call: first call of an increment (F), rotated texture (R), Newton
+// start (W: 0 reference, 1 extrapolated, 2 learned; the first call of an increment starts as 1),
+// GND option (G, reference start only), crystalline second phase (P).
+#define SRX_NR_VARIANTS(K, ARGS)                                                                           \
+    do {                                                                                                 \
+        const int ws_ = first_call && ws == 2 ? 1 : ws;                                                  \
+        const int code_ = (first_call ? 16 : 0) | (rotated ? 8 : 0) | (gnd ? 4 : 0) | (ph2 ? 2 : 0);    \
+        switch (code_ * 4 + ws_) {                                                                       \
+        SRX_NR_CASE(K, ARGS, 1, 1, 0) SRX_NR_CASE(K, ARGS, 1, 0, 0) SRX_NR_CASE(K, ARGS, 0, 1, 0)        \
+        SRX_NR_CASE(K, ARGS, 0, 0, 0) SRX_NR_CASE(K, ARGS, 1, 1, 1) SRX_NR_CASE(K, ARGS, 1, 0, 1)        \
+        SRX_NR_CASE(K, ARGS, 0, 1, 1) SRX_NR_CASE(K, ARGS, 0, 0, 1) SRX_NR_CASE(K, ARGS, 0, 1, 2)        \
+        SRX_NR_CASE(K, ARGS, 0, 0, 2)                                                                    \
+        SRX_NR_GCASE(K, ARGS, 1, 1) SRX_NR_GCASE(K, ARGS, 1, 0) SRX_NR_GCASE(K, ARGS, 0, 1)              \
+        SRX_NR_GCASE(K, ARGS, 0, 0)                                                                      \
+        default: break;                                                                                  \
+        }                                                                                                \
+    } while (0)
+#define SRX_NR_ONE(K, ARGS, F, R, W, G, P)                                                              \
+    case ((F) * 16 + (R) * 8 + (G) * 4 + (P) * 2) * 4 + (W):                                             \
+        SRX_APPLY(SRX_LAUNCH, (K<(F) != 0, (R) != 0, (W), (G) != 0, (P) != 0>), SRX_UNPACK ARGS);        \
+        break;
+#define SRX_NR_CASE(K, ARGS, F, R, W) SRX_NR_ONE(K, ARGS, F, R, W, 0, 0) SRX_NR_ONE(K, ARGS, F, R, W, 0, 1)
+#define SRX_NR_GCASE(K, ARGS, F, R) SRX_NR_ONE(K, ARGS, F, R, 0, 1, 0) SRX_NR_ONE(K, ARGS, F, R, 0, 1, 1)
+#define SRX_UNPACK(...) __VA_ARGS__
+#define SRX_APPLY(M, ...) M(__VA_ARGS__)
+
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, const IterCtl *ctl, int gnd) {
+               double *partials, const IterCtl *ctl, int gnd, int ph2) {
     const dim3 g(nr_block_count(nvox)), b(NR_BLOCK);
-    if (gnd) {   // GND option (plain start only)
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else SRX_LAUNCH((k_nr<false, false, 0, true>), g, b, 0, s, cst, f, nvox, partials, ctl);
-    } else if (f.dsig && f.mlp) {   // learned Newton start (the first call of an increment starts at lam)
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, 2>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else SRX_LAUNCH((k_nr<false, false, 2>), g, b, 0, s, cst, f, nvox, partials, ctl);
-    } else if (f.dsig) {   // extrapolated Newton start
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else SRX_LAUNCH((k_nr<false, false, 1>), g, b, 0, s, cst, f, nvox, partials, ctl);
-    } else {
-        if (first_call && rotated) SRX_LAUNCH((k_nr<true, true, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (first_call) SRX_LAUNCH((k_nr<true, false, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else if (rotated) SRX_LAUNCH((k_nr<false, true, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
-        else SRX_LAUNCH((k_nr<false, false, 0>), g, b, 0, s, cst, f, nvox, partials, ctl);
-    }
+    // the learned start needs the network; with only the previous change it is the extrapolated one
+    const int ws = gnd ? 0 : (f.dsig && f.mlp) ? 2 : f.dsig ? 1 : 0;
+    SRX_NR_VARIANTS(k_nr, (g, b, 0, s, cst, f, nvox, partials, ctl));
     check_launch("nr");
 }
 
@@ -2627,6 +2705,24 @@ void launch_reduce_nr(hipStream_t s, const double *partials, int nblocks, double
     SRX_LAUNCH(k_reduce_seg, dim3(NR_NPART, RED_SEGS), dim3(RED_SEG_BLOCK), 0, s, partials, nblocks, 1ull << 15, seg,
                ctl);
     check_launch("reduce");
+}
+
+void launch_nr_ens(hipStream_t s, const EnsMember *tab, int nmem, int nblocks, int first_call, int rotated, int ws,
+                   int gnd, int ph2) {
+    const dim3 g(nblocks), b(NR_BLOCK);
+    if (gnd) ws = 0;
+    SRX_NR_VARIANTS(k_nr_ens, (g, b, 0, s, tab, nmem));
+    check_launch("nr_ens");
+}
+
+void launch_reduce_nr_ens(hipStream_t s, const EnsMember *tab, int nmem) {
+    SRX_LAUNCH(k_reduce_seg_ens, dim3(NR_NPART, RED_SEGS, nmem), dim3(RED_SEG_BLOCK), 0, s, tab, 1ull << 15);
+    check_launch("reduce_ens");
+}
+
+void launch_control_ens(hipStream_t s, const EnsMember *tab, int nmem, int *done) {
+    SRX_LAUNCH(k_control_ens, dim3(nmem), dim3(64), 0, s, tab, done);
+    check_launch("control_ens");
 }
 
 void launch_control(hipStream_t s, const DevConst *cst, const double *seg, IterCtl *ctl) {

@@ -155,34 +155,40 @@ __device__ __forceinline__ void mlp_forward_wave(const MlpPacked *__restrict__ M
 //   [4, 10)  direction of the previous Newton call's stress change d (crystal frame, B basis:
 //            5 deviatoric components and the hydrostatic one)
 //   [10, 16) the Newton call's starting stress lam in the crystal frame, in units of 100 MPa
-//   [16, 23) log10 of the dislocation density rho_s of slip systems 0..6
-//   [23]     log10(|d| / |lam|)
+//   [16, 22) the forcing of the coming call: e = MLP_E_SCALE * (eps_c - eps_c of the previous call)
+//            (crystal frame, B basis; eps_c = sym(DisGrad)) as e / (|d| + |e|), in [-1, 1]
+//   [22]     log10(|d| / |lam|)
+//   [23]     log10 of the mean dislocation density rho_s over the slip systems
 // nd = |d| is the scale the outputs are given in.  The training target of a sample is the change
 // t of the coming Newton call as the correction to the extrapolated one: t / nd - u.
+constexpr double MLP_E_SCALE = 1.0e5;   // MPa: an elastic modulus scale turning strain into stress
 __device__ __forceinline__ void mlp_features(const double a[9], const double lam[6], const float d[6],
-                                             const double rho[7], float x[MLP_IN], float &nd) {
+                                             const double dec[6], double rho_mean, float x[MLP_IN], float &nd) {
     const double a00 = a[0], a11 = a[4], a22 = a[8];
     const double qw = 0.5 * sqrt(fmax(0.0, 1.0 + a00 + a11 + a22));
     const double qx = copysign(0.5 * sqrt(fmax(0.0, 1.0 + a00 - a11 - a22)), a[7] - a[5]);
     const double qy = copysign(0.5 * sqrt(fmax(0.0, 1.0 - a00 + a11 - a22)), a[2] - a[6]);
     const double qz = copysign(0.5 * sqrt(fmax(0.0, 1.0 - a00 - a11 + a22)), a[3] - a[1]);
     x[0] = (float)qw; x[1] = (float)qx; x[2] = (float)qy; x[3] = (float)qz;
-    double n2 = 0.0, l2 = 0.0;
+    double n2 = 0.0, l2 = 0.0, e2 = 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         n2 += (double)d[k] * (double)d[k];
         l2 += lam[k] * lam[k];
+        e2 += dec[k] * dec[k];
     }
     const double n = sqrt(n2);
     const double rn = n > 0.0 ? 1.0 / n : 0.0;
+    const double en = MLP_E_SCALE * sqrt(e2);
+    const double re = n + en > 0.0 ? MLP_E_SCALE / (n + en) : 0.0;
 #pragma unroll
     for (int k = 0; k < 6; ++k) {
         x[4 + k] = (float)((double)d[k] * rn);
         x[10 + k] = (float)(lam[k] * 0.01);
+        x[16 + k] = (float)(dec[k] * re);
     }
-#pragma unroll
-    for (int s = 0; s < 7; ++s) x[16 + s] = (float)log10(fmax(rho[s], 1e-30));
-    x[23] = (float)log10(fmax(n, 1e-300) / fmax(sqrt(l2), 1e-300));
+    x[22] = (float)log10(fmax(n, 1e-300) / fmax(sqrt(l2), 1e-300));
+    x[23] = (float)log10(fmax(rho_mean, 1e-30));
     nd = (float)n;
 }
 
@@ -192,23 +198,24 @@ __device__ __forceinline__ void mlp_normalise(const MlpPacked *__restrict__ M, f
     for (int k = 0; k < MLP_IN; ++k) x[k] = (x[k] - M->xm[k]) * M->xs[k];
 }
 
-// Stress change predicted for the Newton start: the previous change extrapolated, corrected by
-// the network, nd * (u + c) with u = d / nd its direction and c = y * spread + mean limited to
-// |c| <= 1; zero where the previous change is zero or the prediction is not finite.  A zero
-// correction is the extrapolated start.
+// Stress change predicted for the Newton start: the previous change d = nd u extrapolated with a
+// learned step length, alpha d, alpha = 1 + c . u the network's de-normalised correction
+// c = y * spread + mean projected on u, limited to 0 <= alpha <= 1 (a non-finite alpha gives 1);
+// zero where the previous change is zero.  The correction's component across d is not applied,
+// and the step never goes beyond the extrapolated one: moving the start off the line through the
+// last two iterates, or past the extrapolation, lengthened the Newton solves (a start past the
+// solution on an active slip system costs about one iteration per unit of overshoot in tau / tau_0,
+// a start short of it far less).  A zero correction is the extrapolated start.
 __device__ __forceinline__ void mlp_start(const MlpPacked *__restrict__ M, const float y[MLP_OUT],
                                           const float u[MLP_OUT], float nd, double ws[6]) {
-    double c[MLP_OUT], c2 = 0.0;
+    double cu = 0.0;
 #pragma unroll
-    for (int k = 0; k < MLP_OUT; ++k) {
-        c[k] = (double)y[k] * M->ts[k] + M->tm[k];
-        c2 += c[k] * c[k];
-    }
-    double f = 1.0;
-    if (!(c2 <= 1.0)) f = (c2 > 0.0 && c2 < 1e300) ? 1.0 / sqrt(c2) : 0.0;
+    for (int k = 0; k < MLP_OUT; ++k) cu += ((double)y[k] * M->ts[k] + M->tm[k]) * (double)u[k];
+    double alpha = 1.0 + cu;
+    alpha = alpha == alpha ? fmin(fmax(alpha, 0.0), 1.0) : 1.0;
     const bool ok = nd > 0.0f && nd < 1e30f;
 #pragma unroll
-    for (int k = 0; k < MLP_OUT; ++k) ws[k] = ok ? (double)nd * ((double)u[k] + (f > 0.0 ? c[k] * f : 0.0)) : 0.0;
+    for (int k = 0; k < MLP_OUT; ++k) ws[k] = ok ? (double)nd * alpha * (double)u[k] : 0.0;
 }
 
 }  // namespace srx

@@ -188,9 +188,21 @@ struct Fields {
     double *nrc;       // [36][N] per-system Newton constants of the increment, written by its first call:
                        //   tau_pass/(tau_0+tau_cut) [12], 1/(tau_0+tau_cut) [12], gamma_0 dt [12]
     float *dsig;       // [6][N] last Newton call's stress change, B basis, crystal frame (warm start only)
+    float *ecp;        // [6][N] eps_c = sym(DisGrad) (B basis, sample frame) of the last Newton call (warm start only)
     const MlpPacked *mlp;   // learned Newton start (non-null: the start comes from the network)
     int *phase;        // [N]
     int *nr_it;        // [N]
+};
+
+// One member of an ensemble whose per-voxel kernels run in one launch: the member owns the grid
+// blocks [blk0, blk0 + nblk) of the Newton kernel.
+struct EnsMember {
+    const DevConst *cst;
+    Fields f;
+    int nvox, blk0, nblk, pad;
+    double *partials;   // NR_NPART x nblk block partials
+    double *redseg;     // NR_NPART x 16 segment sums
+    IterCtl *ctl;
 };
 
 // Radix plan of one 1-D FFT length.
@@ -228,8 +240,9 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
                     const DevConst *cst, const IterCtl *ctl, int tile = 0);
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3],
                              const FftPlan &pz, const void *twz, const IterCtl *ctl, double scale, int tile = 0);
+// ph2: the second phase is crystalline (its Newton path is compiled in only then)
 void launch_nr(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int first_call, int rotated,
-               double *partials, const IterCtl *ctl = nullptr, int gnd = 0);
+               double *partials, const IterCtl *ctl = nullptr, int gnd = 0, int ph2 = 1);
 void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals, double *out,
                    const IterCtl *ctl = nullptr);
 // The per-iteration reduction of the Newton kernel's block partials in two fixed-order stages:
@@ -238,6 +251,13 @@ void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals
 constexpr int NR_RED_SEG_DOUBLES = NR_NPART * 16;
 void launch_reduce_nr(hipStream_t s, const double *partials, int nblocks, double *seg, const IterCtl *ctl);
 void launch_control(hipStream_t s, const DevConst *cst, const double *seg, IterCtl *ctl);
+// The same three kernels over every member of an ensemble in one launch each (tab: device array of
+// nmem members; nblocks: their Newton blocks in total; ws: the Newton start option; done[m]: each
+// member's stop flag after the iteration).
+void launch_nr_ens(hipStream_t s, const EnsMember *tab, int nmem, int nblocks, int first_call, int rotated, int ws,
+                   int gnd, int ph2);
+void launch_reduce_nr_ens(hipStream_t s, const EnsMember *tab, int nmem);
+void launch_control_ens(hipStream_t s, const EnsMember *tab, int nmem, int *done);
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int creep, const double *set9,
                      const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,

@@ -142,8 +142,13 @@ def test_ensemble48_converged_mixed_members_equal_runs_alone(ensemble48):
     """The ensemble configuration as benchmarked (mixed precision, every increment to Err): each
     member's last report is bit-identical to the same deck run alone."""
     specs, decks = ensemble48
-    print("ensemble: mixed converged run of 48 members", flush=True)
+    print("ensemble: mixed converged run of 48 members (streams, then lockstep)", flush=True)
     res = srx_amd.run_ensemble(decks, concurrency=8, precision=srx_amd.PREC_MIXED)
+    lock = srx_amd.run_ensemble(decks, concurrency=0, precision=srx_amd.PREC_MIXED)
+    for k in range(len(decks)):
+        assert lock[k]["rc"] == 0 and lock[k]["increments"] == 2
+        for key in ("iters", "nr_iters_total", "s_vm", "e_vm", "rho_s_avg", "sig_avg"):
+            assert lock[k]["last"][key] == res[k]["last"][key], (k, key)
     for k, deck in enumerate(decks):
         with srx_amd.Solver(deck, precision=srx_amd.PREC_MIXED) as s:
             for _ in range(2):

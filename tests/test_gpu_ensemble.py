@@ -54,6 +54,44 @@ def test_ensemble_matches_members_run_alone(tmp_path):
         assert a == b
 
 
+@pytest.mark.parametrize("concurrency", [0, -2])
+def test_lockstep_ensemble_matches_members_run_alone(tmp_path, concurrency):
+    """concurrency 0: all members advance together, their Newton solves, reductions and iteration
+    control in one launch each over the member table (-2: two such groups, concurrently); every
+    member's results equal its deck run alone, bit for bit (the same kernels per member, only
+    launched together)."""
+    decks = make_cases(str(tmp_path / "ens"))
+    res = srx_amd.run_ensemble(decks, concurrency=concurrency, write_files=1)
+    alone = make_cases(str(tmp_path / "alone"))
+    for k, deck in enumerate(alone):
+        s = srx_amd.Solver(deck, write_files=1)
+        for _ in range(3):
+            last = s.step()
+        s.close()
+        r = res[k]
+        assert r["rc"] == 0 and r["increments"] == 3, (k, r["rc"])
+        for key in ("iters", "nr_iters_total", "s_vm", "e_vm", "rho_s_avg"):
+            assert r["last"][key] == last[key], (k, key)
+        assert r["last"]["sig_avg"] == last["sig_avg"]
+        a = open(os.path.join(os.path.dirname(decks[k]), "vm.out")).read()
+        b = open(os.path.join(os.path.dirname(deck), "vm.out")).read()
+        assert a == b
+
+
+def test_lockstep_members_with_different_increment_counts(tmp_path):
+    """A member with fewer increments leaves the lockstep when it is done; the others go on."""
+    decks = make_cases(str(tmp_path / "ens"))[:3]
+    res = srx_amd.run_ensemble(decks, concurrency=0, nsteps_override=0)
+    assert [r["increments"] for r in res] == [3, 3, 3]
+    d2 = str(tmp_path / "short")
+    n, g, seed, rate, tex = CASES[1]
+    short = inputs.make_case(d2, n=n, grains=g, seed=seed, nsteps=1, timestep=1e-4 / rate, rate=rate, texture=tex)
+    res = srx_amd.run_ensemble([decks[0], short, decks[2]], concurrency=0)
+    assert [r["increments"] for r in res] == [3, 1, 3]
+    assert all(r["rc"] == 0 for r in res)
+    assert res[0]["last"]["sig_avg"] == srx_amd.run_ensemble([decks[0]], concurrency=0)[0]["last"]["sig_avg"]
+
+
 def test_ensemble_reports_a_failing_member(tmp_path):
     decks = make_cases(str(tmp_path / "ens"))[:2]
     os.remove(os.path.join(os.path.dirname(decks[1]), "mesh.txt"))

@@ -89,8 +89,11 @@ def test_recorder_features(tmp_path):
     # the initial ones (1e12 m^-2: log10 of 1.0 in the deck's units)
     np.testing.assert_allclose(x[:, :4].reshape(-1, q.shape[0], 4), np.broadcast_to(q, (len(x) // len(q),) + q.shape),
                                atol=1e-6)
-    np.testing.assert_array_equal(x[:, 16:23], 0.0)
+    np.testing.assert_array_equal(x[:, 23], 0.0)
     np.testing.assert_allclose(np.linalg.norm(x[:, 4:10], axis=1), 1.0, rtol=1e-5)
+    # the forcing e / (|d| + |e|): bounded by one, and non-zero while the outer iteration moves DisGrad
+    fn = np.linalg.norm(x[:, 16:22], axis=1)
+    assert (fn <= 1.0 + 1e-6).all() and (fn > 0).mean() > 0.9
     assert np.isfinite(t).all()
 
 
@@ -132,13 +135,9 @@ def test_zero_correction_is_the_extrapolated_start(tmp_path):
     assert np.abs(sa - sb).max() <= 1e-8 * np.abs(sa).max()
 
 
-def test_trained_start_converges_to_the_plain_result_and_is_reported(tmp_path):
-    """The network is trained on the solver's own states (recorded from an extrapolated-start run),
-    then the same deck runs with the plain, extrapolated and learned starts.  The learned start
-    converges to the plain result within the Newton tolerance and takes fewer Newton iterations
-    than the plain start (what the trained network gains over the method without it).  Its own
-    correction is measured against the extrapolated start it refines: on these polycrystals it
-    does not reduce the iteration count further (printed, not asserted)."""
+def _three_starts(tmp_path):
+    """Train on the solver's own states (recorded from an extrapolated-start run of a 24^3 deck),
+    then run the deck with the plain, extrapolated and learned starts."""
     deck = make_deck(str(tmp_path), n=24, steps=6)
     s = srx_amd.Solver(deck, warm_start=1)
     s.mlp_record(2, 2000000)
@@ -152,6 +151,13 @@ def test_trained_start_converges_to_the_plain_result_and_is_reported(tmp_path):
     rp, sp = run(deck, 6, warm_start=0)
     re, se = run(deck, 6, warm_start=1)
     rl, sl = run(deck, 6, warm_start=2, blob=blob)
+    return rp, sp, re, se, rl, sl
+
+
+def test_trained_start_converges_to_the_plain_result_and_is_reported(tmp_path):
+    """The learned start converges to the plain result within the Newton tolerance and takes fewer
+    Newton iterations than the plain start (the reference's, the method without the network)."""
+    rp, sp, re, se, rl, sl = _three_starts(tmp_path)
     for a, b in zip(rp, rl):
         assert a["iters"] == b["iters"]
         assert abs(a["s_vm"] - b["s_vm"]) <= 1e-8 * abs(a["s_vm"])
@@ -160,3 +166,14 @@ def test_trained_start_converges_to_the_plain_result_and_is_reported(tmp_path):
     wave = {k: sum(r["nr_wave_iters"] for r in v) for k, v in (("plain", rp), ("extrapolated", re), ("learned", rl))}
     print("Newton iterations (lane sum): %s; wavefront sum: %s" % (tot, wave))
     assert tot["learned"] < tot["plain"]
+
+
+@pytest.mark.skip(reason="measured, not met: on a 24^3 polycrystal the learned step length takes 401 k wavefront "
+                         "Newton iterations against 373 k for the extrapolated start it refines (2.750 vs 2.718 "
+                         "lane iterations per voxel and outer iteration at 128^3); the network does not improve "
+                         "on the extrapolation")
+def test_learned_start_takes_fewer_wavefront_iterations_than_the_extrapolated_start(tmp_path):
+    """What the network adds over the extrapolated start it refines, in the unit the GPU pays for:
+    wavefront Newton iterations (a wavefront runs until its slowest lane has converged)."""
+    rp, sp, re, se, rl, sl = _three_starts(tmp_path)
+    assert sum(r["nr_wave_iters"] for r in rl) < sum(r["nr_wave_iters"] for r in re)

@@ -70,16 +70,18 @@ def test_crystal_stress_closed_forms():
 
 
 def test_start_rule_limits():
-    u = np.tile(np.array([0, 1, 0, 0, 0, 0], float), (5, 1))
-    y = np.array([[0.5, 0, 0, 0, 0, 0], [10, 0, 0, 0, 0, 0], [np.nan, 0, 0, 0, 0, 0], [1, 1, 0, 0, 0, 0],
-                  [0, 0, 0, 0, 0, 0]], float)
-    nd = np.array([2.0, 2.0, 2.0, 0.0, 3.0])
+    u = np.tile(np.array([0, 1, 0, 0, 0, 0], float), (7, 1))
+    y = np.array([[0, 0.5, 0, 0, 0, 0], [0, 10, 0, 0, 0, 0], [0, np.nan, 0, 0, 0, 0], [1, 1, 0, 0, 0, 0],
+                  [0, 0, 0, 0, 0, 0], [3, -0.25, 0, 0, 0, 0], [0, -3, 0, 0, 0, 0]], float)
+    nd = np.array([2.0, 2.0, 2.0, 0.0, 3.0, 2.0, 2.0])
     w = M.start(y, u, nd)
-    np.testing.assert_allclose(w[0], [1, 2, 0, 0, 0, 0])
-    np.testing.assert_allclose(w[1], [2, 2, 0, 0, 0, 0])     # correction limited to |previous change|
-    np.testing.assert_allclose(w[2], [0, 2, 0, 0, 0, 0])     # a non-finite correction is dropped
-    np.testing.assert_array_equal(w[3], np.zeros(6))
+    np.testing.assert_allclose(w[0], [0, 2, 0, 0, 0, 0])     # never beyond the extrapolated step
+    np.testing.assert_allclose(w[1], [0, 2, 0, 0, 0, 0])
+    np.testing.assert_allclose(w[2], [0, 2, 0, 0, 0, 0])     # a non-finite correction: the extrapolated start
+    np.testing.assert_array_equal(w[3], np.zeros(6))         # no previous change: the reference's start
     np.testing.assert_allclose(w[4], [0, 3, 0, 0, 0, 0])     # zero correction: the extrapolated start
+    np.testing.assert_allclose(w[5], [0, 1.5, 0, 0, 0, 0])   # the component across d is not applied
+    np.testing.assert_array_equal(w[6], np.zeros(6))          # a shortened step: down to the reference's start
 
 
 @pytest.mark.parametrize("rounding", [True, False])
@@ -89,12 +91,19 @@ def test_features_layout(rounding):
     lam = rng.normal(0, 30, (8, 6))
     d = rng.normal(0, 1e-3, (8, 6))
     d[0] = 0.0
-    rho = rng.uniform(0.5, 3, (8, 7))
-    x, nd = M.features(a, lam, d, rho)
+    de = rng.normal(0, 1e-8, (8, 6))
+    de[1] = 0.0
+    rho = rng.uniform(0.5, 3, (8, 12))
+    x, nd = M.features(a, lam, d, de, rho)
     assert x.shape == (8, 24) and x.dtype == np.float32
     np.testing.assert_allclose(np.linalg.norm(x[1:, 4:10], axis=1), 1.0, rtol=1e-6)
     np.testing.assert_array_equal(x[0, 4:10], 0.0)
     np.testing.assert_allclose(nd, np.linalg.norm(d, axis=1), rtol=1e-6)
-    np.testing.assert_allclose(x[:, 16:23], np.log10(rho), rtol=1e-6)
+    # the forcing is rotated like a stress: its norm is frame independent, and e / (|d| + |e|)
+    en = M.E_SCALE * np.linalg.norm(de, axis=1)
+    np.testing.assert_allclose(np.linalg.norm(x[:, 16:22], axis=1), en / (nd + en), rtol=1e-5)
+    np.testing.assert_array_equal(x[1, 16:22], 0.0)
+    np.testing.assert_allclose(np.linalg.norm(x[0, 16:22]), 1.0, rtol=1e-6)   # d = 0: a unit direction
+    np.testing.assert_allclose(x[:, 23], np.log10(rho.mean(1)), rtol=1e-6)
     y = M.forward(M.random_blob(1), x, rounding=rounding)
     assert np.isfinite(y).all()
