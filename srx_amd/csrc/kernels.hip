@@ -57,6 +57,11 @@ __device__ __forceinline__ cx<T> cscale(cx<T> a, T s) { return {a.x * s, a.y * s
 // multiply by -i*sign ... helpers: i*a = (-a.y, a.x)
 template <typename T>
 __device__ __forceinline__ cx<T> mul_i(cx<T> a) { return {-a.y, a.x}; }
+// two complex values (a pair of components loaded together)
+template <typename T>
+struct cx2 {
+    cx<T> a, b;
+};
 
 // ---------------------------------------------------------------------------
 // radix butterflies; sign = -1 forward (e^{-2 pi i/R}), +1 inverse
@@ -150,6 +155,25 @@ struct FastDiv {
     __device__ __forceinline__ explicit FastDiv(int dd) : m(dd > 1 ? 0xFFFFFFFFu / (unsigned)dd + 1u : 0u), d(dd) {}
     __device__ __forceinline__ int div(int n) const { return d > 1 ? (int)__umulhi((unsigned)n, m) : n; }
 };
+
+// Element loop of a pass's global-memory side: elements [0, tot) dealt to the block's threads,
+// taken in chunks of CH per thread whose loads ld(i) are all issued before the first store
+// st(i, value) waits on them (one round trip to HBM per chunk, not per element).  Threads past the
+// end load element tot - 1 again and store nothing.
+constexpr int FFT_CHUNK = 8;
+template <typename V, typename Ld, typename St>
+__device__ __forceinline__ void chunked_copy(int tot, Ld ld, St st) {
+    for (int b = 0; b < tot; b += FFT_CHUNK * FFT_BLOCK) {
+        V v[FFT_CHUNK];
+#pragma unroll
+        for (int e = 0; e < FFT_CHUNK; ++e) v[e] = ld(min(b + e * FFT_BLOCK + (int)threadIdx.x, tot - 1));
+#pragma unroll
+        for (int e = 0; e < FFT_CHUNK; ++e) {
+            const int i = b + e * FFT_BLOCK + (int)threadIdx.x;
+            if (i < tot) st(i, v[e]);
+        }
+    }
+}
 
 // One Stockham stage (radix R, sub-transform span L) over `npen` pencils of
 // length N stored pencil-major in LDS.  Every butterfly of the stage is held
@@ -334,13 +358,14 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
         const int cb = ca + 1;
         const bool two = cb < ncomp;
         const double ma = ca < 6 ? mean[ca] : 0.0, mb = (two && cb < 6) ? mean[cb] : 0.0;
-        for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
-            const int yy = dn2.div(i), z = i - yy * n2;
-            const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
-            const double va = in[ca * N + g] - ma;
-            const double vb = two ? in[cb * N + g] - mb : 0.0;
-            buf[yy * P + z] = cx<T>{T(va), T(vb)};
-        }
+        const double *ia = in + ca * N, *ib = in + (two ? cb : ca) * N;
+        const size_t g0 = ((size_t)x * n1 + y0) * n2;   // the block's rows are contiguous
+        chunked_copy<cx<double>>(
+            ty * n2, [&](int i) { return cx<double>{ia[g0 + i], ib[g0 + i]}; },
+            [&](int i, cx<double> v) {
+                const int yy = dn2.div(i), z = i - yy * n2;
+                buf[yy * P + z] = cx<T>{T(v.x - ma), T(two ? v.y - mb : 0.0)};
+            });
         __syncthreads();
         lds_fft<T, 2, false>(buf, ty, pz, stw, -1, P);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
@@ -380,15 +405,24 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     const size_t base = ((size_t)c * H + kz) * plane;
     const FastDiv dn1(n1), dtx(tx);
     if (b_to_c) {
-        for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
-            const int xx = dn1.div(i), y = i - xx * n1;
-            buf[xx * P + y] = in[base + (size_t)(x0 + xx) * n1 + y];
-        }
+        // the block's x rows are contiguous in B
+        const cx<T> *src = in + base + (size_t)x0 * n1;
+        chunked_copy<cx<T>>(tx * n1, [&](int i) { return src[i]; },
+                            [&](int i, cx<T> v) {
+                                const int xx = dn1.div(i), y = i - xx * n1;
+                                buf[xx * P + y] = v;
+                            });
     } else {
-        for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
-            const int y = dtx.div(i), xx = i - y * tx;
-            buf[xx * P + y] = in[base + (size_t)y * n0 + x0 + xx];
-        }
+        chunked_copy<cx<T>>(
+            tx * n1,
+            [&](int i) {
+                const int y = dtx.div(i), xx = i - y * tx;
+                return in[base + (size_t)y * n0 + x0 + xx];
+            },
+            [&](int i, cx<T> v) {
+                const int y = dtx.div(i), xx = i - y * tx;
+                buf[xx * P + y] = v;
+            });
     }
     __syncthreads();
     lds_fft<T, 2>(buf, tx, py, stw, b_to_c ? -1 : +1, P);
@@ -635,10 +669,14 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
     const FastDiv dslot(slot), dn0(n0);
     cx<T> *stw = buf + (mode == 1 ? 9 : ncomp) * slot;
     for (int i = threadIdx.x; i < n0; i += FFT_BLOCK) stw[i] = tw[i];
-    for (int i = threadIdx.x; i < cin * slot; i += FFT_BLOCK) {
-        const int c = dslot.div(i), r = i - c * slot, yy = dn0.div(r), x = r - yy * n0;
-        buf[c * slot + yy * n0 + x] = spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x];
-    }
+    // the block's rows of one component are contiguous: y0 * n0 .. (y0 + ty) * n0 of plane kz
+    chunked_copy<cx<T>>(
+        cin * slot,
+        [&](int i) {
+            const int c = dslot.div(i), r = i - c * slot;
+            return spec[((size_t)c * H + kz) * plane + (size_t)y0 * n0 + r];
+        },
+        [&](int i, cx<T> v) { buf[i] = v; });
     __syncthreads();
     if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, stw, -1, n0);
     if (mode == 1) {
@@ -699,13 +737,27 @@ __device__ __forceinline__ cx<T> shfl_xor_cx(cx<T> v) {
 template <typename T>
 __device__ __forceinline__ cx<T> conj_cx(cx<T> a) { return cx<T>{a.x, -a.y}; }
 
-// Cross-lane stage ST (partner lane ^ (32 >> ST)) and the ones after it, for C components at once:
-// all partner values of a stage are requested before any is used, so one wait covers them.
+// Cross-lane stage ST (partner lane ^ h, h = 32 >> ST) and the ones after it, for C components at
+// once: all partner values of a stage are requested before any is used, so one wait covers them.
+// Branch free: with s = +1 in the lower lane of a pair and -1 in the upper one, and the upper
+// lane's twiddle w (the lower lane's is 1),
+//   decimation in frequency: lower x_lo + x_hi, upper (x_lo - x_hi) w  =  (u + s v) w'
+//   decimation in time:      lower x_lo + w x_hi, upper x_lo - w x_hi  =  m_partner + s m,
+//                            m = w' v (each upper lane multiplies its own value before the exchange)
+// (u the partner's value, v the lane's own, w' = w in the upper lane and 1 in the lower one; a
+// product by exactly 1 returns its operand, so the results equal the two-sided forms').  The last
+// DIF and first DIT stage (h = 1) has w = 1 in every lane and no product.
+template <typename T>
+__device__ __forceinline__ cx<T> pair_twiddle(const cx<T> w, bool hi) {
+    return cx<T>{hi ? w.x : T(1), hi ? w.y : T(0)};
+}
 template <typename T, int NPL, int C, int ST>
 __device__ __forceinline__ void wave_dif_stage(cx<T> (&v)[C][NPL], const cx<T> ws[6], int lane) {
     if constexpr (ST < 6) {
         constexpr int h = 32 >> ST;
         const bool hi = lane & h;
+        const T s = hi ? T(-1) : T(1);
+        const cx<T> w = pair_twiddle(ws[ST], hi);
         cx<T> u[C][NPL];
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -714,7 +766,10 @@ __device__ __forceinline__ void wave_dif_stage(cx<T> (&v)[C][NPL], const cx<T> w
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-            for (int k = 0; k < NPL; ++k) v[c][k] = hi ? cmul(u[c][k] - v[c][k], ws[ST]) : v[c][k] + u[c][k];
+            for (int k = 0; k < NPL; ++k) {
+                const cx<T> t{fma(s, v[c][k].x, u[c][k].x), fma(s, v[c][k].y, u[c][k].y)};
+                v[c][k] = h == 1 ? t : cmul(t, w);
+            }
         wave_dif_stage<T, NPL, C, ST + 1>(v, ws, lane);
     }
 }
@@ -723,7 +778,14 @@ __device__ __forceinline__ void wave_dit_stage(cx<T> (&v)[C][NPL], const cx<T> w
     if constexpr (ST >= 0) {
         constexpr int h = 32 >> ST;
         const bool hi = lane & h;
-        const cx<T> w = conj_cx(ws[ST]);
+        const T s = hi ? T(-1) : T(1);
+        if constexpr (h != 1) {
+            const cx<T> w = pair_twiddle(conj_cx(ws[ST]), hi);
+#pragma unroll
+            for (int c = 0; c < C; ++c)
+#pragma unroll
+                for (int k = 0; k < NPL; ++k) v[c][k] = cmul(v[c][k], w);
+        }
         cx<T> u[C][NPL];
 #pragma unroll
         for (int c = 0; c < C; ++c)
@@ -732,8 +794,7 @@ __device__ __forceinline__ void wave_dit_stage(cx<T> (&v)[C][NPL], const cx<T> w
 #pragma unroll
         for (int c = 0; c < C; ++c)
 #pragma unroll
-            for (int k = 0; k < NPL; ++k)
-                v[c][k] = hi ? u[c][k] - cmul(v[c][k], w) : v[c][k] + cmul(u[c][k], w);
+            for (int k = 0; k < NPL; ++k) v[c][k] = cx<T>{fma(s, v[c][k].x, u[c][k].x), fma(s, v[c][k].y, u[c][k].y)};
         wave_dit_stage<T, NPL, C, ST - 1>(v, ws, lane);
     }
 }
@@ -764,8 +825,12 @@ __device__ __forceinline__ void wave_fft_inv(cx<T> (&v)[C][NPL], const cx<T> wl[
     }
 }
 
+// blocks of the wavefront Green kernel per compute unit the register allocator targets
+#ifndef SRX_GREEN_MIN_BLOCKS
+#define SRX_GREEN_MIN_BLOCKS 1
+#endif
 template <typename T, int NPL>
-__global__ void __launch_bounds__(256) k_green_wave(cx<T> *__restrict__ spec, int n1, int n2,
+__global__ void __launch_bounds__(256, SRX_GREEN_MIN_BLOCKS) k_green_wave(cx<T> *__restrict__ spec, int n1, int n2,
                                                     const cx<T> *__restrict__ tw, const DevConst *__restrict__ cst,
                                                     double dc0, double dc1, double dc2, double dc3, double dc4,
                                                     double dc5, const IterCtl *__restrict__ ctl) {
@@ -850,26 +915,58 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
     for (int ca = 2 * blockIdx.z; ca < ncomp && ca < 2 * blockIdx.z + 2; ca += 2) {
         const int cb = ca + 1;
         const bool two = cb < ncomp;
-        for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
-            const int kz = dty.div(i), yy = i - kz * ty;
-            const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
-            const cx<T> A = specB[ca * HS + o];
-            const cx<T> B = two ? specB[cb * HS + o] : cx<T>{T(0), T(0)};
-            buf[yy * P + kz] = cx<T>{A.x - B.y, A.y + B.x};
-            if (kz >= 1 && kz <= n2 - H) buf[yy * P + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
+        const cx<T> *sa = specB + ca * HS, *sb = specB + (two ? cb : ca) * HS;
+        chunked_copy<cx2<T>>(
+            H * ty,
+            [&](int i) {
+                const int kz = dty.div(i), yy = i - kz * ty;
+                const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
+                return cx2<T>{sa[o], sb[o]};
+            },
+            [&](int i, cx2<T> v) {
+                const int kz = dty.div(i), yy = i - kz * ty;
+                const cx<T> A = v.a;
+                const cx<T> B = two ? v.b : cx<T>{T(0), T(0)};
+                buf[yy * P + kz] = cx<T>{A.x - B.y, A.y + B.x};
+                if (kz >= 1 && kz <= n2 - H) buf[yy * P + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
+            });
+        // DisGrad of the block's rows (contiguous): the first chunk of each thread's values is
+        // loaded before the transform, so its round trip to HBM overlaps the LDS work
+        double *da = dg + ca * N, *db = dg + (two ? cb : ca) * N;
+        const size_t g0 = ((size_t)x * n1 + y0) * n2;
+        const int tot = ty * n2;
+        cx<double> pre[FFT_CHUNK];
+        if (accumulate) {
+#pragma unroll
+            for (int e = 0; e < FFT_CHUNK; ++e) {
+                const int i = min(e * FFT_BLOCK + (int)threadIdx.x, tot - 1);
+                pre[e] = cx<double>{da[g0 + i], db[g0 + i]};
+            }
         }
         __syncthreads();
         lds_fft<T, 2, false>(buf, ty, pz, stw, +1, P);
-        for (int i = threadIdx.x; i < ty * n2; i += FFT_BLOCK) {
-            const int yy = dn2.div(i), z = i - yy * n2;
-            const size_t g = ((size_t)x * n1 + y0 + yy) * n2 + z;
-            const cx<T> v = buf[yy * P + z];
-            if (accumulate) {
-                dg[ca * N + g] += add[ca] + (double)v.x * rdiv;
-                if (two) dg[cb * N + g] += add[cb] + (double)v.y * rdiv;
-            } else {
-                dg[ca * N + g] = (double)v.x * rdiv;
-                if (two) dg[cb * N + g] = (double)v.y * rdiv;
+        for (int b = 0; b < tot; b += FFT_CHUNK * FFT_BLOCK) {
+            if (accumulate && b > 0) {
+#pragma unroll
+                for (int e = 0; e < FFT_CHUNK; ++e) {
+                    const int i = min(b + e * FFT_BLOCK + (int)threadIdx.x, tot - 1);
+                    pre[e] = cx<double>{da[g0 + i], db[g0 + i]};
+                }
+            }
+#pragma unroll
+            for (int e = 0; e < FFT_CHUNK; ++e) {
+                const int i = b + e * FFT_BLOCK + (int)threadIdx.x;
+                if (i < tot) {
+                    const int yy = dn2.div(i), z = i - yy * n2;
+                    const cx<T> v = buf[yy * P + z];
+                    if (accumulate) {
+                        da[g0 + i] = pre[e].x + (add[ca] + (double)v.x * rdiv);
+                        if (two) db[g0 + i] = pre[e].y + (add[cb] + (double)v.y * rdiv);
+                    } else {
+                        da[g0 + i] = (double)v.x * rdiv;
+                        if (two) db[g0 + i] = (double)v.y * rdiv;
+                    }
+                }
             }
         }
         __syncthreads();
