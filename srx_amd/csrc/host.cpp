@@ -400,7 +400,9 @@ bool grid_supported(const FftPlan pl[3], const int n[3], int prec) {
         const long lds = ax == 0 ? 10L * p.n * csz : (2L * p.n + 1) * csz;
         if (lds > 48 * 1024) return false;
     }
-    return n[0] >= 1 && n[1] >= 1 && n[2] >= 1;
+    if (n[0] < 1 || n[1] < 1 || n[2] < 1) return false;
+    // the per-voxel kernels address an element of a field by a 32-bit byte offset from its component
+    return (long long)n[0] * n[1] * n[2] <= SRX_MAX_VOXELS;
 }
 
 }  // namespace

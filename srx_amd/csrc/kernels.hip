@@ -1276,7 +1276,7 @@ struct NrPro {
 // lane's element of slot 0) when STAGED, else from global memory.
 template <bool STAGED>
 __device__ __forceinline__ double nr_in(const double *stg, int slot, const double *g, int k, size_t N, int v) {
-    return STAGED ? stg[(slot + k) * NR_BLOCK] : g[k * N + v];
+    return STAGED ? stg[(slot + k) * NR_BLOCK] : ldc(g + (size_t)k * N, N, 8u * (unsigned)v);
 }
 
 // WS: Newton start 0 = lam (the reference), 1 = lam + the previous call's change (extrapolated),
@@ -1292,9 +1292,10 @@ __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, co
                                             const double *stg, NrPro &pro, bool store = true) {
     constexpr bool STAGED = !FIRST;
     const DevPhase &P = cst->ph[PH];
+    const unsigned o8 = 8u * (unsigned)v;
     if (!FIRST) {
 #pragma unroll
-        for (int k = 0; k < 15; ++k) pro.A[k] = f.acr[k * N + v];
+        for (int k = 0; k < 15; ++k) pro.A[k] = ldc(f.acr + (size_t)(k) * N, N, o8);
     }
     double a[9];
 #pragma unroll
@@ -1354,7 +1355,7 @@ __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, co
         if (ROT) {
             double a0[9], Sp[15];
 #pragma unroll
-            for (int k = 0; k < 9; ++k) a0[k] = f.tran0[k * N + v];
+            for (int k = 0; k < 9; ++k) a0[k] = ldc(f.tran0 + (size_t)(k) * N, N, o8);
             sprime(a, a0, P.sinv, Sp);
 #pragma unroll
             for (int k = 0; k < 15; ++k) pro.A[k] += Sp[k];
@@ -1363,7 +1364,7 @@ __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, co
             for (int i = 0; i < 5; ++i) pro.A[ps(i, i)] += P.sinv[i];
         }
 #pragma unroll
-        for (int k = 0; k < 15; ++k) f.acr[k * N + v] = pro.A[k];
+        for (int k = 0; k < 15; ++k) stc(f.acr + (size_t)(k) * N, N, o8, pro.A[k]);
     }
 }
 
@@ -1377,6 +1378,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     // compile-time phase index: every material constant is a scalar load at a fixed offset
     const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
+    const unsigned o8 = 8u * (unsigned)v;
     if (FIRST) nr_prologue<true, ROT, PH, (WS != 0)>(cst, f, v, N, nullptr, pro);
     double (&lam)[5] = pro.lam;
     double (&q)[5] = pro.q;
@@ -1399,7 +1401,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         const int nsys = P.nsys;
         double rs[NSYS];
 #pragma unroll
-        for (int s = 0; s < NSYS; ++s) rs[s] = f.rho_s[s * N + v];
+        for (int s = 0; s < NSYS; ++s) rs[s] = ldc(f.rho_s + (size_t)(s) * N, N, o8);
 #pragma unroll
         for (int s = 0; s < NSYS; ++s) {
             double ct, ci, cg;
@@ -1413,10 +1415,10 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
                 ct = t * iv;
                 ci = iv;
                 cg = g * dt;
-                const double rmo = f.rho_m[s * N + v];
+                const double rmo = ldc(f.rho_m + (size_t)(s) * N, N, o8);
                 tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo)) * iv;
-                f.rho_m_old[s * N + v] = rmo;
-                f.rho_m[s * N + v] = rmn;
+                stc(f.rho_m_old + (size_t)(s) * N, N, o8, rmo);
+                stc(f.rho_m + (size_t)(s) * N, N, o8, rmn);
             } else {
                 ct = 1e300; ci = 0.0; cg = 0.0;
                 tpo[s * NR_BLOCK] = 1e300;
@@ -1424,9 +1426,9 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             tp[s * NR_BLOCK] = ct;
             inv[s * NR_BLOCK] = ci;
             g0[s * NR_BLOCK] = cg;
-            f.nrc[(NRC_TP + s) * N + v] = ct;
-            f.nrc[(NRC_INV + s) * N + v] = ci;
-            f.nrc[(NRC_G0 + s) * N + v] = cg;
+            stc(f.nrc + (size_t)((NRC_TP + s)) * N, N, o8, ct);
+            stc(f.nrc + (size_t)((NRC_INV + s)) * N, N, o8, ci);
+            stc(f.nrc + (size_t)((NRC_G0 + s)) * N, N, o8, cg);
         }
     }
 
@@ -1571,7 +1573,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         // the staging slots now hold the per-system constants: the orientation from global memory
         double a[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+        for (int k = 0; k < 9; ++k) a[k] = ldc(f.tran + (size_t)(k) * N, N, o8);
         double Q[5][5];
         q5_from(a, Q);
         // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre.  With the texture
@@ -1627,7 +1629,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     }
     res.sig[5] = sh;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) f.sig_pre[k * N + v] = pre[k];
+    for (int k = 0; k < 5; ++k) stc(f.sig_pre + (size_t)(k) * N, N, o8, pre[k]);
     res.it = it;
     res.itot = itot;
     bool bad = false;
@@ -1726,7 +1728,7 @@ __device__ __forceinline__ void nr_learned_start(const DevConst *__restrict__ cs
 #pragma unroll
         for (int k = 0; k < 6; ++k) d[k] = f.dsig[k * N + v];
 #pragma unroll
-        for (int k = 0; k < NSYS; ++k) rho += f.rho_s[k * N + v];
+        for (int k = 0; k < NSYS; ++k) rho += ldc(f.rho_s + (size_t)(k) * N, N, 8u * (unsigned)v);
         mlp_features(a, lamc, d, dec, rho / NSYS, xf, nd);
     } else {
 #pragma unroll
@@ -1801,9 +1803,10 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
         todo &= ~__ballot(mine);
     }
     if (valid) {
+        const unsigned o8 = 8u * (unsigned)v;
         // gas voxels: Sig = Edot = 0 (evolution.c:418-423)
 #pragma unroll
-        for (int k = 0; k < 6; ++k) f.sig[k * N + v] = r.sig[k];
+        for (int k = 0; k < 6; ++k) stc(f.sig + (size_t)(k) * N, N, o8, r.sig[k]);
         f.nr_it[v] = r.it;
     }
     // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan, wavefront nr_max
@@ -2042,19 +2045,17 @@ constexpr int END_BLOCK = 128;
 
 template <int PH, bool GND>
 __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                                          int lagged_possible, int do_orient, int istep, double *vals) {
+                                          int lagged_possible, int do_orient, int istep, double *vals, double *rsl) {
     const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
+    const unsigned o8 = 8u * (unsigned)v;   // byte offset of this voxel's element of a component
     double a[9];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) a[k] = f.tran[k * N + v];
+    for (int k = 0; k < 9; ++k) a[k] = ldc(f.tran + (size_t)(k) * N, N, o8);
     const bool lagged = lagged_possible && f.nr_it[v] == 1;
-    double rs[NSYS];
-#pragma unroll
-    for (int s = 0; s < NSYS; ++s) rs[s] = f.rho_s[s * N + v];
     double pre[5];
 #pragma unroll
-    for (int k = 0; k < 5; ++k) pre[k] = f.sig_pre[k * N + v];
+    for (int k = 0; k < 5; ++k) pre[k] = ldc(f.sig_pre + (size_t)(k) * N, N, o8);
     // final stress in the crystal frame (Schmid tensors of this increment)
     double sf[5];
     {
@@ -2062,7 +2063,7 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
         q5_from(a, Q);
         double s6[5];
 #pragma unroll
-        for (int k = 0; k < 5; ++k) s6[k] = f.sig[k * N + v];
+        for (int k = 0; k < 5; ++k) s6[k] = ldc(f.sig + (size_t)(k) * N, N, o8);
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
             double acc = 0.0;
@@ -2071,62 +2072,113 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             sf[i] = acc;
         }
     }
+    const int nsys = P.nsys;
+    // rho_s at the start of the increment into this lane's LDS column (rsl, stride END_BLOCK): every
+    // system's rho_F and rho_P sum all of them, and the system loop below updates one at a time;
+    // rho_tot of evolution.c:1040-1046, from the densities at the start of the increment
+    double rho_tot = 0.0;
+#pragma unroll
+    for (int s = 0; s < NSYS; ++s) {
+        const double rs = ldc(f.rho_s + (size_t)(s) * N, N, o8);
+        rsl[s * END_BLOCK] = rs;
+        if (s < nsys) {
+            double g1 = 0.0, g2 = 0.0;
+            if (GND) {
+                g1 = ldc(cst->rho_g1 + (size_t)(s) * N, N, o8);
+                g2 = ldc(cst->rho_g2 + (size_t)(s) * N, N, o8);
+            }
+            rho_tot += rs + ldc(f.rho_m_old + (size_t)(s) * N, N, o8) + sqrt(g1 * g1 + g2 * g2 + 0.0 * 0.0);
+        }
+    }
+    stc(f.rho_tot, N, o8, rho_tot);
     double edc[5] = {0, 0, 0, 0, 0};
-    double sumF = 0, sumP = 0, sumM = 0, rho_tot = 0.0;
+    double sumF = 0, sumP = 0, sumM = 0;
+    double sR = 0, sD = 0, sFo = 0, sI = 0;
     // plastic spin sum_s b_s (x) n_s gamdot_s of update_orient (constitutive.c:215-250), with the
     // pre-update TranMat, accumulated with the slip rates so they need not stay in registers
     double lp[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
     bool bad = false;
-    const int nsys = P.nsys;
-    // rho_tot of evolution.c:1040-1046, from the densities at the start of the increment
-#pragma unroll
-    for (int s = 0; s < NSYS; ++s)
-        if (s < nsys) {
-            double g1 = 0.0, g2 = 0.0;
-            if (GND) {
-                g1 = cst->rho_g1[s * N + v];
-                g2 = cst->rho_g2[s * N + v];
-            }
-            rho_tot += rs[s] + f.rho_m_old[s * N + v] + sqrt(g1 * g1 + g2 * g2 + 0.0 * 0.0);
+    const double G = P.G, nu = cst->nu, T = cst->T, D0 = cst->D0;
+    // One trip per system: slip rate (StrainRate_Orowan at the pre-update iterate, constitutive.c
+    // 507-728) and the hardening update of DislocationEvolution (istep <= 2e6 branch,
+    // constitutive.c:1400-1426) with that slip rate and rho_F.  The per-system arrays of the next
+    // system are loaded one trip ahead.
+    double nx_m = ldc(f.rho_m, N, o8), nx_dip = ldc(f.rho_dipole, N, o8), nx_for = ldc(f.rho_forest, N, o8),
+           nx_inp = ldc(f.rho_inplane, N, o8), nx_acc = ldc(f.gam_acum, N, o8);
+#pragma unroll 1
+    for (int s = 0; s < nsys; ++s) {
+        const double rmn = nx_m, rdip0 = nx_dip, rfor0 = nx_for, rinp0 = nx_inp, gacc = nx_acc;
+        {
+            const size_t c = (size_t)min(s + 1, nsys - 1) * N;
+            nx_m = ldc(f.rho_m + c, N, o8);
+            nx_dip = ldc(f.rho_dipole + c, N, o8);
+            nx_for = ldc(f.rho_forest + c, N, o8);
+            nx_inp = ldc(f.rho_inplane + c, N, o8);
+            nx_acc = ldc(f.gam_acum + c, N, o8);
         }
-    f.rho_tot[v] = rho_tot;
-#pragma unroll SRX_END_UNROLL
-    for (int s = 0; s < NSYS; ++s) {
-        if (s < nsys) {
-            double rP, rF;
-            rho_pf(P, rs, s, rP, rF);
-            if (GND) gnd_add(cst, s, N, v, rP, rF);
-            const double rmn = f.rho_m[s * N + v];
-            const double rmo = f.rho_m_old[s * N + v];
-            double tp, inv, g0;
-            sys_const(P, rP, rF, lagged ? rmo : rmn, rmn, tp, inv, g0);
-            const double *p = P.p[s];
-            const double rss = p[0] * pre[0] + p[1] * pre[1] + p[2] * pre[2] + p[3] * pre[3] + p[4] * pre[4];
-            double g, g2;
-            slip_rate(rss, fma(fabs(rss), inv, -(tp * inv)), inv, g0, g, g2);
-            bad |= !isfinite(g);
+        double rP = 0.0, rF = 0.0;
 #pragma unroll
-            for (int i = 0; i < 5; ++i) edc[i] += p[i] * g;
-            if (do_orient) {
-                double bs[3], ns[3];
-                for (int i = 0; i < 3; ++i) {
-                    bs[i] = a[3 * i] * P.bb[s][0] + a[3 * i + 1] * P.bb[s][1] + a[3 * i + 2] * P.bb[s][2];
-                    ns[i] = a[3 * i] * P.bn[s][0] + a[3 * i + 1] * P.bn[s][1] + a[3 * i + 2] * P.bn[s][2];
-                }
-                for (int i = 0; i < 3; ++i)
-                    for (int j = 0; j < 3; ++j) lp[i][j] += bs[i] * ns[j] * g;
-            }
-            f.rho_F[s * N + v] = rF;
-            f.rho_P[s * N + v] = rP;
-            f.gamdot[s * N + v] = g;
-            sumF += rF;
-            sumP += rP;
-            sumM += rmn;
+        for (int j = 0; j < NSYS; ++j) {
+            const double rj = rsl[j * END_BLOCK];
+            rF += P.chi1[s][j] * rj;
+            rP += P.chi[s][j] * rj;
         }
+        if (GND) gnd_add(cst, s, N, v, rP, rF);
+        const double rmo = lagged ? ldc(f.rho_m_old + (size_t)(s) * N, N, o8) : rmn;
+        double tp, inv, g0;
+        sys_const(P, rP, rF, rmo, rmn, tp, inv, g0);
+        const double *p = P.p[s];
+        const double rss = p[0] * pre[0] + p[1] * pre[1] + p[2] * pre[2] + p[3] * pre[3] + p[4] * pre[4];
+        double g, g2;
+        slip_rate(rss, fma(fabs(rss), inv, -(tp * inv)), inv, g0, g, g2);
+        bad |= !isfinite(g);
+#pragma unroll
+        for (int i = 0; i < 5; ++i) edc[i] += p[i] * g;
+        if (do_orient) {
+            double bs[3], ns[3];
+            for (int i = 0; i < 3; ++i) {
+                bs[i] = a[3 * i] * P.bb[s][0] + a[3 * i + 1] * P.bb[s][1] + a[3 * i + 2] * P.bb[s][2];
+                ns[i] = a[3 * i] * P.bn[s][0] + a[3 * i + 1] * P.bn[s][1] + a[3 * i + 2] * P.bn[s][2];
+            }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) lp[i][j] += bs[i] * ns[j] * g;
+        }
+        stc(f.rho_F + (size_t)(s) * N, N, o8, rF);
+        stc(f.rho_P + (size_t)(s) * N, N, o8, rP);
+        stc(f.gamdot + (size_t)(s) * N, N, o8, g);
+        sumF += rF;
+        sumP += rP;
+        sumM += rmn;
+        // hardening, with the final stress in the crystal frame
+        const double rssf = p[0] * sf[0] + p[1] * sf[1] + p[2] * sf[2] + p[3] * sf[3] + p[4] * sf[4];
+        const double bb = P.b;
+        const double d_dip = sqrt(3.0) * G * bb / 16.0 / 3.14159265359 / (1 - nu) / (fabs(rssf) + 1.0E-4);
+        const double hydro = G * bb / 3.14159265359 / (1 - nu) / d_dip;
+        double r = rsl[s * END_BLOCK];
+        double rdip = rdip0, rfor = rfor0, rinp = rinp0;
+        const double gd = g;
+        const double rFs = rF;
+        if (istep <= 2000000) {
+            const double climb = 1E-17 * rho_tot * 0.5 * 1E24 * (d_dip) * (D0 / bb) * pow(r, 2.0) *
+                                 (exp(fabs(hydro) * bb * bb * bb * 100 / (1.38 * T)) - 1.0) * P.expmqb;
+            rdip += climb * dt;
+            rfor += fabs(gd) * (P.c4 * sqrt(rFs)) * dt;
+            rinp += fabs(gd) * (P.c5 * r) * dt;
+            const double dot = fabs(gd) * (P.c4 * sqrt(rFs) + P.c6 * (d_dip) * rmn - P.c5 * r) - climb;
+            r += dot * dt;
+        }
+        stc(f.rho_s + (size_t)(s) * N, N, o8, r);
+        stc(f.rho_dipole + (size_t)(s) * N, N, o8, rdip);
+        stc(f.rho_forest + (size_t)(s) * N, N, o8, rfor);
+        stc(f.rho_inplane + (size_t)(s) * N, N, o8, rinp);
+        stc(f.gam_acum + (size_t)(s) * N, N, o8, gacc + gd * dt);
+        sR += r;
+        sD += rdip;
+        sFo += rfor;
+        sI += rinp;
     }
     // Edot in the sample frame and Eps += Edot dt (the rotation is rebuilt rather than kept in
-    // registers through the slip-rate loop)
-    double eds[5];
+    // registers through the system loop)
     double Q[5][5];
     q5_from(a, Q);
 #pragma unroll
@@ -2134,31 +2186,29 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
         double acc = 0.0;
 #pragma unroll
         for (int k = 0; k < 5; ++k) acc += Q[i][k] * edc[k];
-        eds[i] = acc;
-        f.edot[i * N + v] = acc;
-        const double e = f.eps[i * N + v] + acc * dt;
-        f.eps[i * N + v] = e;
+        stc(f.edot + (size_t)(i) * N, N, o8, acc);
+        const double e = ldc(f.eps + (size_t)(i) * N, N, o8) + acc * dt;
+        stc(f.eps + (size_t)(i) * N, N, o8, e);
         vals[(i) * END_BLOCK] = e;
         vals[(6 + i) * END_BLOCK] = acc;
     }
     vals[(5) * END_BLOCK] = 0.0;
     vals[(11) * END_BLOCK] = 0.0;
-    (void)eds;
     // reorientation (update_orient), with the pre-update TranMat for b and n
     if (do_orient) {
         double rl[3][3];
         double vg[9];
 #pragma unroll
-        for (int k = 0; k < 9; ++k) vg[k] = (f.dg[k * N + v] - f.dg0[k * N + v]) / dt;
+        for (int k = 0; k < 9; ++k) vg[k] = (ldc(f.dg + (size_t)(k) * N, N, o8) - ldc(f.dg0 + (size_t)(k) * N, N, o8)) / dt;
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j) rl[i][j] = (vg[3 * i + j] - vg[3 * j + i]) / 2.;
-        double rot[3][3], rsl[3][3];
+        double rot[3][3], rsw[3][3];
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) rsl[i][j] = (lp[i][j] - lp[j][i]) / 2.;
-        vals[(18) * END_BLOCK] = sqrt(rsl[2][1] * rsl[2][1] + rsl[0][2] * rsl[0][2] + rsl[1][0] * rsl[1][0]);
+            for (int j = 0; j < 3; ++j) rsw[i][j] = (lp[i][j] - lp[j][i]) / 2.;
+        vals[(18) * END_BLOCK] = sqrt(rsw[2][1] * rsw[2][1] + rsw[0][2] * rsw[0][2] + rsw[1][0] * rsw[1][0]);
         vals[(19) * END_BLOCK] = sqrt(rl[2][1] * rl[2][1] + rl[0][2] * rl[0][2] + rl[1][0] * rl[1][0]);
         for (int i = 0; i < 3; ++i)
-            for (int j = 0; j < 3; ++j) rot[i][j] = (cst->Udot_a[3 * i + j] + rl[i][j] - rsl[i][j]) * dt;
+            for (int j = 0; j < 3; ++j) rot[i][j] = (cst->Udot_a[3 * i + j] + rl[i][j] - rsw[i][j]) * dt;
         // Orientation (constitutive.c:142-194)
         double w[3] = {rot[2][1], rot[0][2], rot[1][0]};
         double wn = sqrt(w[0] * w[0] + w[1] * w[1] + w[2] * w[2]);
@@ -2174,42 +2224,7 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
             for (int j = 0; j < 3; ++j) rot[i][j] = (double)(i == j) + 2. * (om[i][j] + om2[i][j]) / (1. + wt);
         for (int i = 0; i < 3; ++i)
             for (int j = 0; j < 3; ++j)
-                f.tran[(3 * i + j) * N + v] = rot[i][0] * a[j] + rot[i][1] * a[3 + j] + rot[i][2] * a[6 + j];
-    }
-    // DislocationEvolution, istep <= 2e6 branch; the slip rates and rho_F of this increment are
-    // read back from the arrays written above (one system at a time keeps the kernel's registers
-    // within two wavefronts per SIMD)
-    const double G = P.G, nu = cst->nu, T = cst->T, D0 = cst->D0;
-    double sR = 0, sD = 0, sFo = 0, sI = 0;
-#pragma unroll 1
-    for (int s = 0; s < nsys; ++s) {
-        const double *p = P.p[s];
-        const double rss = p[0] * sf[0] + p[1] * sf[1] + p[2] * sf[2] + p[3] * sf[3] + p[4] * sf[4];
-        const double bb = P.b;
-        const double d_dip = sqrt(3.0) * G * bb / 16.0 / 3.14159265359 / (1 - nu) / (fabs(rss) + 1.0E-4);
-        const double hydro = G * bb / 3.14159265359 / (1 - nu) / d_dip;
-        double r = f.rho_s[s * N + v];
-        double rdip = f.rho_dipole[s * N + v], rfor = f.rho_forest[s * N + v], rinp = f.rho_inplane[s * N + v];
-        const double gd = f.gamdot[s * N + v];
-        const double rFs = f.rho_F[s * N + v];
-        if (istep <= 2000000) {
-            const double climb = 1E-17 * rho_tot * 0.5 * 1E24 * (d_dip) * (D0 / bb) * pow(r, 2.0) *
-                                 (exp(fabs(hydro) * bb * bb * bb * 100 / (1.38 * T)) - 1.0) * P.expmqb;
-            rdip += climb * dt;
-            rfor += fabs(gd) * (P.c4 * sqrt(rFs)) * dt;
-            rinp += fabs(gd) * (P.c5 * r) * dt;
-            const double dot = fabs(gd) * (P.c4 * sqrt(rFs) + P.c6 * (d_dip) * f.rho_m[s * N + v] - P.c5 * r) - climb;
-            r += dot * dt;
-        }
-        f.rho_s[s * N + v] = r;
-        f.rho_dipole[s * N + v] = rdip;
-        f.rho_forest[s * N + v] = rfor;
-        f.rho_inplane[s * N + v] = rinp;
-        f.gam_acum[s * N + v] += gd * dt;
-        sR += r;
-        sD += rdip;
-        sFo += rfor;
-        sI += rinp;
+                stc(f.tran + (size_t)((3 * i + j)) * N, N, o8, rot[i][0] * a[j] + rot[i][1] * a[3 + j] + rot[i][2] * a[6 + j]);
     }
     vals[(12) * END_BLOCK] = sumF;
     vals[(13) * END_BLOCK] = sumP;
@@ -2234,6 +2249,7 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
     const int ph = valid ? f.phase[v] : 0;
     // this lane's partial values, in an LDS column (stride END_BLOCK) rather than registers
     __shared__ double lv[END_NPART * END_BLOCK];
+    __shared__ double lrs[NSYS * END_BLOCK];
     double *vals = lv + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < END_NPART; ++k) vals[k * END_BLOCK] = 0.0;
@@ -2244,9 +2260,9 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
         const bool mine = valid && ph == pu;
         if (mine) {
             if (pu == 1) {
-                if (cst->ph[0].xtal) end_voxel<0, GND>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
+                if (cst->ph[0].xtal) end_voxel<0, GND>(cst, f, v, N, lagged_possible, do_orient, istep, vals, lrs + threadIdx.x);
             } else {
-                if (cst->ph[1].xtal) end_voxel<1, GND>(cst, f, v, N, lagged_possible, do_orient, istep, vals);
+                if (cst->ph[1].xtal) end_voxel<1, GND>(cst, f, v, N, lagged_possible, do_orient, istep, vals, lrs + threadIdx.x);
             }
         }
         todo &= ~__ballot(mine);
@@ -2256,8 +2272,8 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
         // gas: Edot = 0, Eps unchanged (it is zero), gamdot untouched
 #pragma unroll
         for (int i = 0; i < 5; ++i) {
-            f.edot[i * N + v] = 0.0;
-            vals[i * END_BLOCK] = f.eps[i * N + v];
+            stc(f.edot + (size_t)(i) * N, N, 8u * (unsigned)v, 0.0);
+            vals[i * END_BLOCK] = ldc(f.eps + (size_t)(i) * N, N, 8u * (unsigned)v);
         }
     }
     __shared__ double red[END_BLOCK / 64][END_NPART];

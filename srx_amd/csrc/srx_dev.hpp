@@ -14,6 +14,28 @@ namespace srx {
 constexpr int NSYS = 12;
 constexpr int NPH = 2;
 constexpr int MAX_RADIX_STAGES = 12;
+// Largest grid (voxels): the per-voxel kernels address element v of a component by the 32-bit byte
+// offset 8 v from the component's base.
+constexpr long long SRX_MAX_VOXELS = (1ll << 29) - 1;
+
+// Element v of one component of a field, read or written as a raw buffer access: the component's
+// base address and byte length in a buffer resource (scalar registers, wavefront-uniform) and the
+// lane's 32-bit byte offset o8 = 8 v.  A field access written as base[k * N + v] makes a 64-bit
+// address per lane and component, which the compiler computes early and keeps live; here the only
+// per-lane value is o8.  (An offset past the component reads 0 and writes nothing.)
+#if defined(__HIP__)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t comp_rsrc(const double *base, size_t n) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)(unsigned)(n * 8u),
+                                             0x00020000);
+}
+__device__ __forceinline__ double ldc(const double *base, size_t n, unsigned o8) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(comp_rsrc(base, n), o8, 0, 0));
+}
+typedef unsigned int srx_u32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void stc(double *base, size_t n, unsigned o8, double x) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(srx_u32x2, x), comp_rsrc(base, n), o8, 0, 0);
+}
+#endif
 
 // Material constants of one phase, precomputed on the host in the
 // reference's evaluation order where it matters for rounding.
