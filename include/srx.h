@@ -37,7 +37,7 @@ enum {
                                    extent CellDim[0] also within the x pass's LDS row (10 n0 complex values
                                    in 48 KB, 9 n0 / smallest radix <= 1024): up to 300 in FP64 (not 250,
                                    270) and up to 512 in mixed precision (not 250, 270, 360, 375, 384, 405,
-                                   432, 450, 480, 486, 500); at most 2^29 - 1 voxels in all */
+                                   432, 450, 480, 486, 500); at most 2^28 - 1 voxels in all */
     SRX_E_HIP = 2003,           /* HIP runtime error */
     SRX_E_SYSTEMS = 2004,       /* more than 12 slip systems in a mode */
     SRX_E_NOMEM = 2005,         /* allocation failed */

@@ -1245,8 +1245,8 @@ __device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const do
 }
 
 // Staging slots of the Newton kernel's inputs in LDS (one 64-voxel slice per component).
-constexpr int ST_SIG = 0, ST_EPS = 6, ST_DG = 11, ST_TRAN = 20;
-constexpr int ST_COMPS = 29;
+constexpr int ST_SIG = 0, ST_EPS = 6, ST_DG = 12, ST_TRAN = 22;   // each field from an even slot
+constexpr int ST_COMPS = 32;
 // LDS slots of the per-system constants (tau_pass * inv, inv, gamma_0 dt of 12 systems), also the
 // component order of Fields::nrc
 constexpr int NRC_TP = 0, NRC_INV = NSYS, NRC_G0 = 2 * NSYS, NRC_COMPS = 3 * NSYS;
@@ -1651,30 +1651,29 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// asynchronous global -> LDS copy of `bytes` (4 or 16) per lane to lds + lane * bytes
-__device__ __forceinline__ void copy_to_lds(const void *src, double *lds, int bytes) {
-#if __HIP_DEVICE_COMPILE__
-    if (bytes == 16) __builtin_amdgcn_global_load_lds(src, (lds_void *)lds, 16, 0, 0);
-    else __builtin_amdgcn_global_load_lds(src, (lds_void *)lds, 4, 0, 0);
-#endif
-}
-
-// Copies components [0, NC) of a component-major array (component c at base(c) + voxel) for the
-// voxels [v0, v0 + 64) of this block into LDS slots (slot c at lds + 64 c), then waits for them.
+// Asynchronous copies of the NC components of one field (component c at base + c N) for the voxels
+// [v0, v0 + 64) of this block into the LDS slots lds + 64 c, as raw buffer loads to LDS: the
+// component's base in a buffer resource (scalar registers) and one 32-bit byte offset per lane.
 // With an even voxel count every component slice is 16-byte aligned and one 16-byte load per lane
-// moves two components; otherwise 4-byte loads.  Lanes past the last voxel re-read the last valid
-// element (the values are never used).  Every lane of the block must take part: a lane's copy
-// fills other lanes' elements.
-template <int NC, typename Base>
-__device__ __forceinline__ void copy_slots(Base base, int nvox, double *lds, int blk) {
+// moves two components (lanes 32-63 the second, at byte offset 8 N further; a field with an odd
+// component count fills the slot after its last one with a copy of it); otherwise 4-byte loads.
+// Lanes past the last voxel re-read the last valid element (the values are never used).  Every
+// lane of the block must take part (a lane's copy fills other lanes' elements), and the caller
+// waits for the copies (copy_wait).
+template <int NC>
+__device__ __forceinline__ void copy_field(const double *base, size_t N, int nvox, double *lds, int blk) {
+#if __HIP_DEVICE_COMPILE__
     const int lane = threadIdx.x;
     const int v0 = blk * NR_BLOCK;
     if ((nvox & 1) == 0) {
-        const int pair = min(v0 + 2 * (lane & 31), nvox - 2);
+        const unsigned pb = 8u * (unsigned)min(v0 + 2 * (lane & 31), nvox - 2);
+        const unsigned hi = (lane >> 5) ? 8u * (unsigned)N : 0u;
 #pragma unroll
         for (int c = 0; c < NC; c += 2) {
-            const int cc = min(c + (lane >> 5), NC - 1);
-            copy_to_lds(base(cc) + pair, lds + c * NR_BLOCK, 16);
+            const bool two = c + 1 < NC;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(comp_rsrc(base + (size_t)c * N, two ? 2 * N : N),
+                                                     (lds_void *)(lds + c * NR_BLOCK), 16, pb + (two ? hi : 0u),
+                                                     0, 0, 0);
         }
     } else {
 #pragma unroll
@@ -1682,27 +1681,27 @@ __device__ __forceinline__ void copy_slots(Base base, int nvox, double *lds, int
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int d = h * 64 + lane;
-                const int vv = min(v0 + (d >> 1), nvox - 1);
-                const char *src = reinterpret_cast<const char *>(base(c) + vv) + 4 * (d & 1);
-                copy_to_lds(src, lds + c * NR_BLOCK + h * 32, 4);
+                const unsigned off = 8u * (unsigned)min(v0 + (d >> 1), nvox - 1) + 4u * (unsigned)(d & 1);
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(comp_rsrc(base + (size_t)c * N, N),
+                                                         (lds_void *)(lds + c * NR_BLOCK + h * 32), 4, off, 0, 0, 0);
             }
     }
+#endif
+}
+__device__ __forceinline__ void copy_wait() {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 }
 
 // The Newton inputs of the block's voxels (stress, plastic strain, DisGrad, orientation) into the
-// staging slots ST_*.
-template <int NC>
+// staging slots ST_* (each field from an even slot).
 __device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage, int blk) {
     const size_t N = (size_t)nvox;
-    auto base = [&](int c) -> const double * {
-        return c < ST_EPS ? f.sig + (size_t)(c - ST_SIG) * N
-             : c < ST_DG ? f.eps + (size_t)(c - ST_EPS) * N
-             : c < ST_TRAN ? f.dg + (size_t)(c - ST_DG) * N
-             : f.tran + (size_t)(c - ST_TRAN) * N;
-    };
-    copy_slots<NC>(base, nvox, stage, blk);
+    copy_field<6>(f.sig, N, nvox, stage + ST_SIG * NR_BLOCK, blk);
+    copy_field<5>(f.eps, N, nvox, stage + ST_EPS * NR_BLOCK, blk);
+    copy_field<9>(f.dg, N, nvox, stage + ST_DG * NR_BLOCK, blk);
+    copy_field<9>(f.tran, N, nvox, stage + ST_TRAN * NR_BLOCK, blk);
+    copy_wait();
 }
 
 // Learned start of the wavefront's voxels (every lane calls; non-crystalline lanes get zero).
@@ -1772,7 +1771,7 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
     NrPro pro;
     float wsp[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (!FIRST) {
-        stage_inputs<NST>(f, nvox, lds, blk);
+        stage_inputs(f, nvox, lds, blk);
         const double *stg = lds + threadIdx.x;
         if constexpr (WS == 2)
             nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, stg, wsp);
@@ -1781,7 +1780,8 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
         // the staging slots are consumed (all LDS reads returned) before the constants land on them
         __builtin_amdgcn_s_waitcnt(0xc07f);
         const double *nrc = f.nrc;
-        copy_slots<NRC_COMPS>([&](int c) { return nrc + (size_t)c * N; }, nvox, lds, blk);
+        copy_field<NRC_COMPS>(nrc, N, nvox, lds, blk);
+        copy_wait();
     }
     NrResult r;
     r.cs = 0.0; r.ce = 0.0; r.it = 0; r.itot = 0; r.bad = 0;

@@ -14,9 +14,9 @@ namespace srx {
 constexpr int NSYS = 12;
 constexpr int NPH = 2;
 constexpr int MAX_RADIX_STAGES = 12;
-// Largest grid (voxels): the per-voxel kernels address element v of a component by the 32-bit byte
-// offset 8 v from the component's base.
-constexpr long long SRX_MAX_VOXELS = (1ll << 29) - 1;
+// Largest grid (voxels): the per-voxel kernels address element v of a component, or of the
+// component after it, by a 32-bit byte offset (8 v, 8 (N + v)) from the component's base.
+constexpr long long SRX_MAX_VOXELS = (1ll << 28) - 1;
 
 // Element v of one component of a field, read or written as a raw buffer access: the component's
 // base address and byte length in a buffer resource (scalar registers, wavefront-uniform) and the

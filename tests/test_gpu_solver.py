@@ -56,6 +56,30 @@ def test_fixed_iterations_match_oracle(tmp_path, oracle_lib):
         compare_fields(g, o, 1e-9)
 
 
+@pytest.mark.parametrize("precision", [srx_amd.PREC_FP64, srx_amd.PREC_MIXED])
+def test_odd_voxel_count_matches_oracle(tmp_path, oracle_lib, precision):
+    """A grid of odd extents (15 x 9 x 5 = 675 voxels: an odd count, so the Newton kernel stages its
+    inputs with 4-byte copies, and a last 64-voxel block of 35) at fixed outer iterations, through
+    increments where the texture rotates: FP64 to 1e-9 with the oracle's exact Newton iteration
+    counts, mixed precision to the mixed bounds (1e-4 on the macroscopic stress, 1e-5 on rho_s)."""
+    deck = make(tmp_path, n=(15, 9, 5), grains=6, seed=21, nsteps=4)
+    o = oracle_lib.Oracle(deck, fixed_iters=4)
+    g = srx_amd.Solver(deck, fixed_iters=4, precision=precision)
+    for step in range(4):
+        ro, rg = o.step(), g.step()
+        assert rg["iters"] == ro["iters"] == 4
+        if precision == srx_amd.PREC_FP64:
+            assert rg["nr_iters_total"] == ro["nr_iters_total"]
+            compare_reports(rg, ro, 1e-10)
+            compare_fields(g, o, 1e-9)
+        else:
+            so, sm = np.array(ro["sig_avg"]), np.array(rg["sig_avg"])
+            assert np.linalg.norm(sm - so) / np.linalg.norm(so) < 1e-4
+            assert rel(g.field("rho_s"), o.field("rho_s")) < 1e-5
+    g.close()
+    o.close()
+
+
 def test_converged_increments_match_oracle(tmp_path, oracle_lib):
     """Increments iterated to the deck's Err = 2e-5, including istep >= 2 where the texture
     (and with it the crystal frame of the Newton solve) evolves."""
