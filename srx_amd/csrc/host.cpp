@@ -661,6 +661,12 @@ int setup(srx_ctx *c) {
     };
     Fields &F = c->f;
     int rc = 0;
+    {
+        // the Newton constants: 36 components at an even pitch (16-byte aligned component slices)
+        const size_t bytes = sizeof(double) * 36 * nrc_pitch(N);
+        HIPCK(hipMalloc(&F.nrc, bytes));
+        if ((rc = ctx_zero(st, F.nrc, bytes))) return rc;
+    }
     if ((rc = dalloc(&F.sig, 6, 0.0)) || (rc = dalloc(&F.sig_pre, 5, 0.0)) || (rc = dalloc(&F.eps, 5, 0.0)) ||
         (rc = dalloc(&F.edot, 5, 0.0)) || (rc = dalloc(&F.dg, 9, 0.0)) || (rc = dalloc(&F.dg0, 9, 0.0)) ||
         (rc = dalloc(&F.tran, 9, 0.0)) || (rc = dalloc(&F.tran0, 9, 0.0)) ||
@@ -669,7 +675,7 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.gamdot, 12, 0.0)) || (rc = dalloc(&F.gam_acum, 12, 0.0)) ||
         (rc = dalloc(&F.rho_dipole, 12, 0.0)) || (rc = dalloc(&F.rho_forest, 12, 0.0)) ||
         (rc = dalloc(&F.rho_inplane, 12, 0.0)) || (rc = dalloc(&F.rho_tot, 1, 0.0)) ||
-        (rc = dalloc(&F.acr, 15, 0.0)) || (rc = dalloc(&F.nrc, 36, 0.0)) ||
+        (rc = dalloc(&F.acr, 15, 0.0)) ||
         (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
     if (c->opt.gnd) {

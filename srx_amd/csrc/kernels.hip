@@ -1244,9 +1244,6 @@ __device__ __forceinline__ void sc_rotate(const double *__restrict__ S, const do
         }
 }
 
-// Staging slots of the Newton kernel's inputs in LDS (one 64-voxel slice per component).
-constexpr int ST_SIG = 0, ST_EPS = 6, ST_DG = 12, ST_TRAN = 22;   // each field from an even slot
-constexpr int ST_COMPS = 32;
 // LDS slots of the per-system constants (tau_pass * inv, inv, gamma_0 dt of 12 systems), also the
 // component order of Fields::nrc
 constexpr int NRC_TP = 0, NRC_INV = NSYS, NRC_G0 = 2 * NSYS, NRC_COMPS = 3 * NSYS;
@@ -1271,52 +1268,68 @@ struct NrPro {
     double lam[5], q[5], r0[5], A[15];
     double lamh, qh, rsig2;
 };
+// An empty statement that takes x in a register: x is computed before it, and the statement stays
+// in program order with the memory waits around it.
+__device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+// The prologue is formed while the copies issued before it are in flight, not after the wait for
+// them (the compiler would otherwise sink the arithmetic to its first use).
+__device__ __forceinline__ void pin_prologue(NrPro &p) {
+#pragma unroll
+    for (int i = 0; i < 5; ++i) { pin(p.lam[i]); pin(p.q[i]); pin(p.r0[i]); }
+    pin(p.lamh); pin(p.qh); pin(p.rsig2);
+}
 
-// Input component k of a field: from the LDS staging slice (lane-major, stg points at this
-// lane's element of slot 0) when STAGED, else from global memory.
-template <bool STAGED>
-__device__ __forceinline__ double nr_in(const double *stg, int slot, const double *g, int k, size_t N, int v) {
-    return STAGED ? stg[(slot + k) * NR_BLOCK] : ldc(g + (size_t)k * N, N, 8u * (unsigned)v);
+// The inputs of a Newton call for one voxel: orientation (TranMat), stress, plastic strain,
+// DisGrad and, after the first call of an increment, the increment's matrix A (packed).
+struct NrIn {
+    double a[9], lam6[6], ep[5], d[9], A[15];
+};
+template <bool LOAD_A>
+__device__ __forceinline__ void nr_load(const Fields &f, size_t N, unsigned o8, NrIn &in) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) in.a[k] = ldc(f.tran + (size_t)k * N, N, o8);
+#pragma unroll
+    for (int k = 0; k < 6; ++k) in.lam6[k] = ldc(f.sig + (size_t)k * N, N, o8);
+#pragma unroll
+    for (int k = 0; k < 5; ++k) in.ep[k] = ldc(f.eps + (size_t)k * N, N, o8);
+#pragma unroll
+    for (int k = 0; k < 9; ++k) in.d[k] = ldc(f.dg + (size_t)k * N, N, o8);
+    if (LOAD_A) {
+#pragma unroll
+        for (int k = 0; k < 15; ++k) in.A[k] = ldc(f.acr + (size_t)k * N, N, o8);
+    }
 }
 
 // WS: Newton start 0 = lam (the reference), 1 = lam + the previous call's change (extrapolated),
 // 2 = lam + the learned prediction `wsp` (crystal frame)
-// Prologue of a Newton call.  FIRST (the increment's first call): inputs from global memory, A
-// assembled from the orientations (Sc = Q^T S066 Q and the rotated grain compliance S') and stored
-// in f.acr for the increment's later calls.  Otherwise: inputs from the LDS staging slots (stg
-// points at this lane's element of slot 0) and A from f.acr.
+// Prologue of a Newton call from its inputs `in`.  FIRST (the increment's first call): A assembled
+// from the orientations (Sc = Q^T S066 Q and the rotated grain compliance S') and stored in f.acr
+// for the increment's later calls.  Otherwise A is in.A.
 // KEEP_EC (warm starts): eps_c is also stored in f.ecp where `store` is set, for the next call's
 // learned-start features.
 template <bool FIRST, bool ROT, int PH, bool KEEP_EC>
 __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, const Fields &f, int v, size_t N,
-                                            const double *stg, NrPro &pro, bool store = true) {
-    constexpr bool STAGED = !FIRST;
+                                            const NrIn &in, NrPro &pro, bool store = true) {
     const DevPhase &P = cst->ph[PH];
     const unsigned o8 = 8u * (unsigned)v;
     if (!FIRST) {
 #pragma unroll
-        for (int k = 0; k < 15; ++k) pro.A[k] = ldc(f.acr + (size_t)(k) * N, N, o8);
+        for (int k = 0; k < 15; ++k) pro.A[k] = in.A[k];
     }
-    double a[9];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) a[k] = nr_in<STAGED>(stg, ST_TRAN, f.tran, k, N, v);
+    const double *a = in.a;
     double Q[5][5];
     q5_from(a, Q);
     double lam6[6], w[5];
     {
-        double ec[6], ep[5];
+        double ec[6];
+        const double *ep = in.ep;
 #pragma unroll
-        for (int k = 0; k < 6; ++k) lam6[k] = nr_in<STAGED>(stg, ST_SIG, f.sig, k, N, v);
-#pragma unroll
-        for (int k = 0; k < 5; ++k) ep[k] = nr_in<STAGED>(stg, ST_EPS, f.eps, k, N, v);
-        double d[9];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) d[k] = nr_in<STAGED>(stg, ST_DG, f.dg, k, N, v);
+        for (int k = 0; k < 6; ++k) lam6[k] = in.lam6[k];
         // strain_aux = (DisGrad + DisGrad^T)/2 -> B basis (chg_basis opt 2)
-        ec_from_dg(d, ec);
+        ec_from_dg(in.d, ec);
         if (KEEP_EC && store && f.ecp) {
 #pragma unroll
-            for (int k = 0; k < 6; ++k) f.ecp[k * N + v] = (float)ec[k];
+            for (int k = 0; k < 6; ++k) stcf(f.ecp + (size_t)k * N, N, 4u * (unsigned)v, (float)ec[k]);
         }
 #pragma unroll
         for (int k = 0; k < 5; ++k) w[k] = ep[k] - ec[k];
@@ -1379,7 +1392,11 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     const DevPhase &P = cst->ph[PH];
     const double dt = cst->dt;
     const unsigned o8 = 8u * (unsigned)v;
-    if (FIRST) nr_prologue<true, ROT, PH, (WS != 0)>(cst, f, v, N, nullptr, pro);
+    if (FIRST) {
+        NrIn in;
+        nr_load<false>(f, N, o8, in);
+        nr_prologue<true, ROT, PH, (WS != 0)>(cst, f, v, N, in, pro);
+    }
     double (&lam)[5] = pro.lam;
     double (&q)[5] = pro.q;
     double (&A)[15] = pro.A;
@@ -1426,9 +1443,9 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
             tp[s * NR_BLOCK] = ct;
             inv[s * NR_BLOCK] = ci;
             g0[s * NR_BLOCK] = cg;
-            stc(f.nrc + (size_t)((NRC_TP + s)) * N, N, o8, ct);
-            stc(f.nrc + (size_t)((NRC_INV + s)) * N, N, o8, ci);
-            stc(f.nrc + (size_t)((NRC_G0 + s)) * N, N, o8, cg);
+            stc(f.nrc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8, ct);
+            stc(f.nrc + (size_t)(NRC_INV + s) * nrc_pitch(N), nrc_pitch(N), o8, ci);
+            stc(f.nrc + (size_t)(NRC_G0 + s) * nrc_pitch(N), nrc_pitch(N), o8, cg);
         }
     }
 
@@ -1436,8 +1453,8 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         // extrapolated start: lam + (lam - lam of the previous outer iteration), the change
         // kept in the crystal frame of this increment's orientation
 #pragma unroll
-        for (int i = 0; i < 5; ++i) ws[i] = (double)f.dsig[i * N + v];
-        wsh = (double)f.dsig[5 * N + v];
+        for (int i = 0; i < 5; ++i) ws[i] = (double)ldcf(f.dsig + (size_t)i * N, N, 4u * (unsigned)v);
+        wsh = (double)ldcf(f.dsig + 5 * N, N, 4u * (unsigned)v);
     } else if (!FIRST && WS == 2) {
 #pragma unroll
         for (int i = 0; i < 5; ++i) ws[i] = (double)wsp[i];
@@ -1561,8 +1578,8 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     if (WS) {
         // this call's stress change (crystal frame) for the next call's extrapolated start
 #pragma unroll
-        for (int i = 0; i < 5; ++i) f.dsig[i * N + v] = (float)(sg[i] - lam[i]);
-        f.dsig[5 * N + v] = (float)(sh - lamh);
+        for (int i = 0; i < 5; ++i) stcf(f.dsig + (size_t)i * N, N, 4u * (unsigned)v, (float)(sg[i] - lam[i]));
+        stcf(f.dsig + 5 * N, N, 4u * (unsigned)v, (float)(sh - lamh));
     }
     // errors of evolution.c:381-391 (rotation invariant)
     double cs2 = (sh - lamh) * (sh - lamh);
@@ -1651,40 +1668,26 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 
 typedef __attribute__((address_space(3))) void lds_void;
 
-// Asynchronous copies of the NC components of one field (component c at base + c N) for the voxels
-// [v0, v0 + 64) of this block into the LDS slots lds + 64 c, as raw buffer loads to LDS: the
-// component's base in a buffer resource (scalar registers) and one 32-bit byte offset per lane.
-// With an even voxel count every component slice is 16-byte aligned and one 16-byte load per lane
-// moves two components (lanes 32-63 the second, at byte offset 8 N further; a field with an odd
-// component count fills the slot after its last one with a copy of it); otherwise 4-byte loads.
-// Lanes past the last voxel re-read the last valid element (the values are never used).  Every
-// lane of the block must take part (a lane's copy fills other lanes' elements), and the caller
-// waits for the copies (copy_wait).
+// Asynchronous copies of the NC components of one field (component c at base + c P, the pitch P
+// even, so that every component slice is 16-byte aligned) for the voxels [v0, v0 + 64) of this
+// block into the LDS slots lds + 64 c, as raw buffer loads to LDS: the component's base in a buffer
+// resource (scalar registers) and one 32-bit byte offset per lane.  One 16-byte load per lane moves
+// two components (lanes 32-63 the second, at byte offset 8 P further; with an odd NC the slot after
+// the last one receives a copy of it).  Lanes past the last voxel read the pitch's padding or
+// re-read valid elements (the values are never used).  Every lane of the block must take part (a
+// lane's copy fills other lanes' elements), and the caller waits for the copies (copy_wait).
 template <int NC>
-__device__ __forceinline__ void copy_field(const double *base, size_t N, int nvox, double *lds, int blk) {
+__device__ __forceinline__ void copy_field(const double *base, size_t P, double *lds, int blk) {
 #if __HIP_DEVICE_COMPILE__
     const int lane = threadIdx.x;
     const int v0 = blk * NR_BLOCK;
-    if ((nvox & 1) == 0) {
-        const unsigned pb = 8u * (unsigned)min(v0 + 2 * (lane & 31), nvox - 2);
-        const unsigned hi = (lane >> 5) ? 8u * (unsigned)N : 0u;
+    const unsigned pb = 8u * (unsigned)min((size_t)(v0 + 2 * (lane & 31)), P - 2);
+    const unsigned hi = (lane >> 5) ? 8u * (unsigned)P : 0u;
 #pragma unroll
-        for (int c = 0; c < NC; c += 2) {
-            const bool two = c + 1 < NC;
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(comp_rsrc(base + (size_t)c * N, two ? 2 * N : N),
-                                                     (lds_void *)(lds + c * NR_BLOCK), 16, pb + (two ? hi : 0u),
-                                                     0, 0, 0);
-        }
-    } else {
-#pragma unroll
-        for (int c = 0; c < NC; ++c)
-#pragma unroll
-            for (int h = 0; h < 2; ++h) {
-                const int d = h * 64 + lane;
-                const unsigned off = 8u * (unsigned)min(v0 + (d >> 1), nvox - 1) + 4u * (unsigned)(d & 1);
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(comp_rsrc(base + (size_t)c * N, N),
-                                                         (lds_void *)(lds + c * NR_BLOCK + h * 32), 4, off, 0, 0, 0);
-            }
+    for (int c = 0; c < NC; c += 2) {
+        const bool two = c + 1 < NC;
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(comp_rsrc(base + (size_t)c * P, two ? 2 * P : P),
+                                                 (lds_void *)(lds + c * NR_BLOCK), 16, pb + (two ? hi : 0u), 0, 0, 0);
     }
 #endif
 }
@@ -1693,34 +1696,19 @@ __device__ __forceinline__ void copy_wait() {
     __syncthreads();
 }
 
-// The Newton inputs of the block's voxels (stress, plastic strain, DisGrad, orientation) into the
-// staging slots ST_* (each field from an even slot).
-__device__ __forceinline__ void stage_inputs(const Fields &f, int nvox, double *stage, int blk) {
-    const size_t N = (size_t)nvox;
-    copy_field<6>(f.sig, N, nvox, stage + ST_SIG * NR_BLOCK, blk);
-    copy_field<5>(f.eps, N, nvox, stage + ST_EPS * NR_BLOCK, blk);
-    copy_field<9>(f.dg, N, nvox, stage + ST_DG * NR_BLOCK, blk);
-    copy_field<9>(f.tran, N, nvox, stage + ST_TRAN * NR_BLOCK, blk);
-    copy_wait();
-}
-
 // Learned start of the wavefront's voxels (every lane calls; non-crystalline lanes get zero).
-// Features from the staged orientation and stress, the previous change and rho_s.
+// Features from the Newton call's inputs (orientation, stress, DisGrad), the previous change and
+// rho_s.
 __device__ __forceinline__ void nr_learned_start(const DevConst *__restrict__ cst, const Fields &f, size_t N, int v,
-                                                 bool xt, const double *stg, float wsp[6]) {
+                                                 bool xt, const NrIn &in, float wsp[6]) {
     float xf[MLP_IN], nd = 0.0f;
     if (xt) {
-        double a[9], lam6[6], lamc[6], dg[9], de[6], dec[6], rho = 0.0;
+        double lamc[6], de[6], dec[6], rho = 0.0;
+        const double *a = in.a;
         float d[6];
-#pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = stg[(ST_TRAN + k) * NR_BLOCK];
-#pragma unroll
-        for (int k = 0; k < 6; ++k) lam6[k] = stg[(ST_SIG + k) * NR_BLOCK];
-        crystal_stress(a, lam6, lamc);
+        crystal_stress(a, in.lam6, lamc);
         // forcing: eps_c now minus eps_c of the previous call, crystal frame
-#pragma unroll
-        for (int k = 0; k < 9; ++k) dg[k] = stg[(ST_DG + k) * NR_BLOCK];
-        ec_from_dg(dg, de);
+        ec_from_dg(in.d, de);
 #pragma unroll
         for (int k = 0; k < 6; ++k) de[k] -= (double)f.ecp[k * N + v];
         crystal_stress(a, de, dec);
@@ -1764,23 +1752,24 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
     // its prologue from them, the per-system constants the first call stored overwrite the
     // staging slots (by a second burst), so the block needs no more LDS than the larger of the
     // two.  The first call computes both and keeps its lagged tau_pass in four more slices.
-    constexpr int NST = FIRST ? 0 : ST_COMPS;
-    constexpr int NCONST = FIRST ? NRC_COMPS + NSYS : NRC_COMPS;
-    constexpr int NSLOT = ((NST + 1) & ~1) > NCONST ? ((NST + 1) & ~1) : NCONST;
+    constexpr int NSLOT = FIRST ? NRC_COMPS + NSYS : NRC_COMPS;
     __shared__ __align__(16) double lds[NSLOT * NR_BLOCK];
     NrPro pro;
     float wsp[6] = {0, 0, 0, 0, 0, 0};
     if constexpr (!FIRST) {
-        stage_inputs(f, nvox, lds, blk);
-        const double *stg = lds + threadIdx.x;
+        // After the first call of an increment: the call's inputs and A are loaded into registers,
+        // then the increment's per-system constants are copied into LDS by asynchronous buffer
+        // loads, so both arrive in one round trip to HBM and the prologue is formed while the
+        // constants are in flight.  Every lane forms a prologue (lanes past the last voxel from
+        // the last one's inputs).
+        const int vv = valid ? v : nvox - 1;
+        NrIn in;
+        nr_load<true>(f, N, 8u * (unsigned)vv, in);
         if constexpr (WS == 2)
-            nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, stg, wsp);
-        // every lane forms its prologue (lanes past the last voxel from the last one's inputs)
-        nr_prologue<false, ROT, 0, (WS != 0)>(cst, f, valid ? v : nvox - 1, N, stg, pro, valid);
-        // the staging slots are consumed (all LDS reads returned) before the constants land on them
-        __builtin_amdgcn_s_waitcnt(0xc07f);
-        const double *nrc = f.nrc;
-        copy_field<NRC_COMPS>(nrc, N, nvox, lds, blk);
+            nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, in, wsp);
+        copy_field<NRC_COMPS>(f.nrc, nrc_pitch(N), lds, blk);
+        nr_prologue<false, ROT, 0, (WS != 0)>(cst, f, vv, N, in, pro, valid);
+        pin_prologue(pro);
         copy_wait();
     }
     NrResult r;

@@ -14,6 +14,9 @@ namespace srx {
 constexpr int NSYS = 12;
 constexpr int NPH = 2;
 constexpr int MAX_RADIX_STAGES = 12;
+// Component pitch of Fields::nrc: the voxel count rounded up to even, so that every component
+// starts 16-byte aligned.
+__host__ __device__ inline size_t nrc_pitch(size_t n) { return (n + 1) & ~(size_t)1; }
 // Largest grid (voxels): the per-voxel kernels address element v of a component, or of the
 // component after it, by a 32-bit byte offset (8 v, 8 (N + v)) from the component's base.
 constexpr long long SRX_MAX_VOXELS = (1ll << 28) - 1;
@@ -30,6 +33,17 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t comp_rsrc(const double *base, 
 }
 __device__ __forceinline__ double ldc(const double *base, size_t n, unsigned o8) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(comp_rsrc(base, n), o8, 0, 0));
+}
+// The same for an FP32 field (lane byte offset o4 = 4 v).
+__device__ __forceinline__ float ldcf(const float *base, size_t n, unsigned o4) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(unsigned)(n * 4u), 0x00020000), o4, 0,
+        0));
+}
+__device__ __forceinline__ void stcf(float *base, size_t n, unsigned o4, float x) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x),
+                                          __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(unsigned)(n * 4u), 0x00020000),
+                                          o4, 0, 0);
 }
 typedef unsigned int srx_u32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ void stc(double *base, size_t n, unsigned o8, double x) {
@@ -207,7 +221,7 @@ struct Fields {
     double *fl0;       // [9][N] fluctuation carried over DisGrad resets (creep runs only, else null)
     double *acr;       // [15][N] crystal-frame Newton matrix A = Q^T S066 Q + S' (packed symmetric 5x5) of the
                        //   increment, written by its first Newton call, read by the later ones
-    double *nrc;       // [36][N] per-system Newton constants of the increment, written by its first call:
+    double *nrc;       // [36][nrc_pitch(N)] per-system Newton constants of the increment, written by its first call:
                        //   tau_pass/(tau_0+tau_cut) [12], 1/(tau_0+tau_cut) [12], gamma_0 dt [12]
     float *dsig;       // [6][N] last Newton call's stress change, B basis, crystal frame (warm start only)
     float *ecp;        // [6][N] eps_c = sym(DisGrad) (B basis, sample frame) of the last Newton call (warm start only)
