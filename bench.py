@@ -40,7 +40,7 @@ FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix), TFLOP/s
 F_NR = 2300.0                  # flops per voxel per Newton iteration (algorithmic count)
 # HBM bytes per voxel per launch of the Newton kernel, from rocprofv3 FETCH_SIZE (x2.0) and
 # WRITE_SIZE (x1.0) runs calibrated on a 1 GiB device copy (profiles/evp_fft_128cubed_pmc.md)
-NR_TRAFFIC_BYTES_PER_VOXEL = 852.0
+NR_TRAFFIC_BYTES_PER_VOXEL = 817.2
 METRIC = "converged voxel-increments/s at 128³, plain FP64 vs mixed-precision at equal stress error"
 
 
