@@ -1410,10 +1410,12 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 
     // per-system constants, kept in LDS (lane-major, conflict free): tau_pass * inv, inv and
     // gamma_0 * dt (the slip rates enter the residual and the Jacobian times dt); the first call of
-    // an increment computes them (with the lagged rho_m in tau_pass for its own first iterate) and
-    // stores them in f.nrc, from where the caller copies them into LDS for the later calls
+    // an increment computes them and stores them in f.nrc, from where the caller copies them into
+    // LDS for the later calls.  Its own first iterate takes tau_pass with the lagged rho_m: those
+    // values sit in the tau_pass slots for the first iteration, and the second reloads the
+    // increment's from f.nrc (this lane's own stores).
     double *tp = lds + NRC_TP * NR_BLOCK + threadIdx.x, *inv = lds + NRC_INV * NR_BLOCK + threadIdx.x;
-    double *g0 = lds + NRC_G0 * NR_BLOCK + threadIdx.x, *tpo = lds + NRC_COMPS * NR_BLOCK + threadIdx.x;
+    double *g0 = lds + NRC_G0 * NR_BLOCK + threadIdx.x;
     if (FIRST) {
         const int nsys = P.nsys;
         double rs[NSYS];
@@ -1433,14 +1435,13 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
                 ci = iv;
                 cg = g * dt;
                 const double rmo = ldc(f.rho_m + (size_t)(s) * N, N, o8);
-                tpo[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo)) * iv;
+                tp[s * NR_BLOCK] = P.c1Gb3 * fsqrt(fabs(rP + rmo)) * iv;
                 stc(f.rho_m_old + (size_t)(s) * N, N, o8, rmo);
                 stc(f.rho_m + (size_t)(s) * N, N, o8, rmn);
             } else {
                 ct = 1e300; ci = 0.0; cg = 0.0;
-                tpo[s * NR_BLOCK] = 1e300;
+                tp[s * NR_BLOCK] = 1e300;
             }
-            tp[s * NR_BLOCK] = ct;
             inv[s * NR_BLOCK] = ci;
             g0[s * NR_BLOCK] = cg;
             stc(f.nrc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8, ct);
@@ -1468,8 +1469,9 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     double err2 = 1E-18;
     // With the learned start (WS 2), a solve that does not converge (non-finite iterate, or 500
     // iterations) is repeated from lam, the reference's start; `itot` counts both attempts.  (The
-    // extrapolated start has not needed it; the second attempt costs it registers.)
-    for (int att = 0; att < (WS == 2 ? 2 : 1); ++att) {
+    // extrapolated start has not needed it; the second attempt costs it registers.  The first call
+    // of an increment starts from lam already.)
+    for (int att = 0; att < ((WS == 2 && !FIRST) ? 2 : 1); ++att) {
     if (att == 1) {
         if (err2 <= 1E-20) break;
 #pragma unroll
@@ -1487,7 +1489,12 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         for (int k = 0; k < 15; ++k) K[k] = A[k];
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
-        const double *tpu = (FIRST && it == 1) ? tpo : tp;
+        if (FIRST && it == 2) {
+#pragma unroll
+            for (int s = 0; s < NSYS; ++s)
+                tp[s * NR_BLOCK] = ldc(f.nrc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8);
+        }
+        const double *tpu = tp;
         // SRX_NR_GROUP systems per trip, under one wavefront-uniform branch: their slip-rate
         // evaluations are independent dependency chains the scheduler interleaves (an inactive
         // system of the group evaluates at 0 and adds exact zeros)
@@ -1655,15 +1662,15 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     res.bad = bad;
 }
 
-// Waves per SIMD the register allocator targets for the Newton kernel.  The kernel is bound by
-// FP64 dependency chains and LDS/scalar-load latency, not by HBM: a second wavefront per SIMD
-// to issue from is worth a few spilled registers (measured 31% faster than one wavefront with
-// no spills).  The first call of an increment (more inputs, more live state) keeps one.
+// Waves per SIMD the register allocator targets for the Newton kernel: a second wavefront per
+// SIMD to issue from (measured 31% faster than one), which the LDS of the per-system constants
+// (18 KB per 64-lane block) also allows; both the first call of an increment and the later ones
+// fit two without spills.
 #ifndef SRX_NR_WAVES_PER_EU
 #define SRX_NR_WAVES_PER_EU 2
 #endif
 #ifndef SRX_NR_FIRST_WAVES_PER_EU
-#define SRX_NR_FIRST_WAVES_PER_EU 1
+#define SRX_NR_FIRST_WAVES_PER_EU 2
 #endif
 
 typedef __attribute__((address_space(3))) void lds_void;
@@ -1752,7 +1759,7 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
     // its prologue from them, the per-system constants the first call stored overwrite the
     // staging slots (by a second burst), so the block needs no more LDS than the larger of the
     // two.  The first call computes both and keeps its lagged tau_pass in four more slices.
-    constexpr int NSLOT = FIRST ? NRC_COMPS + NSYS : NRC_COMPS;
+    constexpr int NSLOT = NRC_COMPS;
     __shared__ __align__(16) double lds[NSLOT * NR_BLOCK];
     NrPro pro;
     float wsp[6] = {0, 0, 0, 0, 0, 0};
