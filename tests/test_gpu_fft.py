@@ -1,4 +1,7 @@
-"""GPU FFT passes against numpy.fft (a library routine) through the C ABI."""
+"""GPU FFT passes against numpy.fft (a library routine) through the C ABI.
+
+Shapes include extents of 256 along z or y, where a z or y pass block moves 16 pencils of 256
+elements and its global loads run in two chunks of 8 per thread."""
 import numpy as np
 import pytest
 
@@ -8,7 +11,7 @@ pytestmark = pytest.mark.gpu
 
 
 @pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (4, 6, 10), (12, 20, 9), (32, 16, 64), (1, 8, 8),
-                                   (8, 1, 5), (96, 8, 4)])
+                                   (8, 1, 5), (96, 8, 4), (4, 32, 256), (16, 256, 8)])
 @pytest.mark.parametrize("ncomp", [1, 6, 9])
 def test_forward_matches_numpy_fp64(shape, ncomp):
     rng = np.random.default_rng(hash((shape, ncomp)) % 2**32)
@@ -20,7 +23,8 @@ def test_forward_matches_numpy_fp64(shape, ncomp):
     assert err < 1e-13, err
 
 
-@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (12, 20, 9), (32, 16, 64), (96, 8, 4)])
+@pytest.mark.parametrize("shape", [(8, 8, 8), (16, 16, 16), (12, 20, 9), (32, 16, 64), (96, 8, 4), (4, 32, 256),
+                                   (16, 256, 8)])
 def test_inverse_matches_numpy_fp64(shape):
     rng = np.random.default_rng(7)
     x = rng.standard_normal((9,) + shape)
