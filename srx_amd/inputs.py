@@ -31,8 +31,11 @@ FCC_SLIP = [
 PI_REF = 3.14159265359  # evp.h:84, used by the reader for deg->rad
 
 
-def write_sx(path: str, tau0: float = 0.1) -> None:
-    """Slip-mode file with one FCC mode (label 1), tau0xf = ``tau0`` MPa."""
+def write_sx(path: str, tau0: float = 0.1, nsys: int = 12) -> None:
+    """Slip-mode file with one FCC mode (label 1), tau0xf = ``tau0`` MPa: the first ``nsys`` of the
+    12 {111}<110> systems in Cu_p.sx's order."""
+    if not 1 <= nsys <= 12:
+        raise ValueError("nsys must be 1..12")
     lines = [
         "# Crystal system (XtalSys):", "CUBIC", "",
         "# Crystal axex (XtalAxis):", "1.0 1.0 1.0", "",
@@ -41,14 +44,14 @@ def write_sx(path: str, tau0: float = 0.1) -> None:
         "# Labels of the modes to be used (iMode):", "1", "",
         "# Mode# 1 -- <110>(111) SLIP",
         "\tnsmx\tnrsx\tgamd0x\ttwshx\tisectwx",
-        "\t12\t\t10.\t\t1.0\t\t0.0\t\t0",
+        "\t%d\t\t10.\t\t1.0\t\t0.0\t\t0" % nsys,
         "\ttau0xf\ttau0xb\ttau1x\tthet0\tthet1",
         f"\t{tau0!r}\t\t{tau0!r}\t\t5.\t\t400.\t250.",
         "\thselfx\thlatex",
         "\t1.0\t\t1.0",
         "\tSlip (n-b)",
     ]
-    for n, b in FCC_SLIP:
+    for n, b in FCC_SLIP[:nsys]:
         lines.append("\t%d %d %d\t\t%d %d %d" % (n + b))
     with open(path, "w") as f:
         f.write("\n".join(lines) + "\n")
@@ -225,10 +228,10 @@ def write_mesh_grains(path: str, n, euler_grain: np.ndarray, grain: np.ndarray, 
 
 def make_case(workdir: str, n=(16, 16, 16), grains: int = 8, seed: int = 1, nsteps: int = 20,
               texture: str = "random", mesh_name: str = "mesh.txt", deck_name: str = "deck.in",
-              **deck_kw) -> str:
+              nsys: int = 12, **deck_kw) -> str:
     """Write sx + mesh + deck for a Voronoi case into ``workdir``; return the deck path."""
     os.makedirs(workdir, exist_ok=True)
-    write_sx(os.path.join(workdir, "Cu_p.sx"))
+    write_sx(os.path.join(workdir, "Cu_p.sx"), nsys=nsys)
     eul, gid = voronoi_grains(n, grains, seed, texture)
     write_mesh_grains(os.path.join(workdir, mesh_name), n, eul, gid)
     deck = os.path.join(workdir, deck_name)

@@ -80,6 +80,22 @@ def test_odd_voxel_count_matches_oracle(tmp_path, oracle_lib, precision):
     o.close()
 
 
+def test_fewer_slip_systems_match_oracle(tmp_path, oracle_lib):
+    """A slip mode of 7 of the 12 systems (an odd count: the Newton kernel's last pair of systems
+    holds one padding system, and the end-of-increment kernel's system loop and its one-ahead
+    loads stop at 7) at fixed outer iterations, through increments where the texture rotates."""
+    deck = make(tmp_path, n=(16, 12, 10), grains=6, seed=22, nsteps=4, nsys=7)
+    o = oracle_lib.Oracle(deck, fixed_iters=4)
+    g = srx_amd.Solver(deck, fixed_iters=4)
+    for step in range(4):
+        ro, rg = o.step(), g.step()
+        assert rg["nr_iters_total"] == ro["nr_iters_total"]
+        compare_reports(rg, ro, 1e-10)
+        compare_fields(g, o, 1e-9)
+    g.close()
+    o.close()
+
+
 def test_converged_increments_match_oracle(tmp_path, oracle_lib):
     """Increments iterated to the deck's Err = 2e-5, including istep >= 2 where the texture
     (and with it the crystal frame of the Newton solve) evolves."""

@@ -227,6 +227,17 @@ def test_reference_deck_two_crystalline_phases(tmp_path):
     assert np.isfinite(r["s_vm"]) and r["iters"] == 1
 
 
+@pytest.mark.parametrize("nsys", [7, 12])
+def test_slip_mode_size_is_honoured(tmp_path, nsys):
+    """A mode of the first nsys systems: the average of rho_s summed over the systems starts at
+    nsys times the initial density (rho_SSD_initial = 1 per system in these decks)."""
+    deck = inputs.make_case(str(tmp_path), n=(8, 6, 4), grains=3, seed=2, nsteps=1, timestep=0.1, nsys=nsys)
+    o = Oracle(deck, fixed_iters=2)
+    r = o.step()
+    assert r["rho_s_avg"] == pytest.approx(float(nsys), rel=1e-6)
+    o.close()
+
+
 def test_error_codes(tmp_path):
     """Setup errors return the reference's PError codes instead of aborting (init.c:195-197,
     263-266, 865-867, 888-891)."""
