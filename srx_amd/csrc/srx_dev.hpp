@@ -27,9 +27,12 @@ constexpr long long SRX_MAX_VOXELS = (1ll << 28) - 1;
 // address per lane and component, which the compiler computes early and keeps live; here the only
 // per-lane value is o8.  (An offset past the component reads 0 and writes nothing.)
 #if defined(__HIP__)
+// Byte length of a buffer resource: saturated at 2^32 - 1 (every offset used stays below 2^32).
+__device__ __forceinline__ int rsrc_bytes(size_t bytes) {
+    return (int)(unsigned)(bytes > 0xFFFFFFFFull ? 0xFFFFFFFFull : bytes);
+}
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t comp_rsrc(const double *base, size_t n) {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, (int)(unsigned)(n * 8u),
-                                             0x00020000);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<double *>(base), (short)0, rsrc_bytes(n * 8u), 0x00020000);
 }
 __device__ __forceinline__ double ldc(const double *base, size_t n, unsigned o8) {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(comp_rsrc(base, n), o8, 0, 0));
@@ -37,12 +40,12 @@ __device__ __forceinline__ double ldc(const double *base, size_t n, unsigned o8)
 // The same for an FP32 field (lane byte offset o4 = 4 v).
 __device__ __forceinline__ float ldcf(const float *base, size_t n, unsigned o4) {
     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
-        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, (int)(unsigned)(n * 4u), 0x00020000), o4, 0,
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base), (short)0, rsrc_bytes(n * 4u), 0x00020000), o4, 0,
         0));
 }
 __device__ __forceinline__ void stcf(float *base, size_t n, unsigned o4, float x) {
     __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x),
-                                          __builtin_amdgcn_make_buffer_rsrc(base, (short)0, (int)(unsigned)(n * 4u), 0x00020000),
+                                          __builtin_amdgcn_make_buffer_rsrc(base, (short)0, rsrc_bytes(n * 4u), 0x00020000),
                                           o4, 0, 0);
 }
 typedef unsigned int srx_u32x2 __attribute__((ext_vector_type(2)));

@@ -63,6 +63,10 @@ def test_setup_error_codes(tmp_path):
     bad = os.path.join(d, "bad_grid.in")
     inputs.write_deck(bad, n=(7, 4, 4))
     assert open_code(bad) == 2002
+    # every extent supported, but 2^28 voxels: beyond the 32-bit byte offsets of the field accesses
+    bad = os.path.join(d, "bad_grid_size.in")
+    inputs.write_deck(bad, n=(256, 1024, 1024))
+    assert open_code(bad) == 2002
     bad = os.path.join(d, "bad_sx.in")
     inputs.write_deck(bad, n=(4, 4, 4), sx="nope.sx")
     assert open_code(bad) == 2001
