@@ -246,7 +246,7 @@ int srx_mlp_samples(srx_ctx *ctx, float *x, float *t, size_t max, size_t *n);
 int srx_fft_tiles(srx_ctx *ctx, int *tiles, double *us);
 
 /* Measurement helper (no counterpart in the reference): the HBM ceiling of a plain streaming
-   copy on the current GPU, against which the HBM-bound passes are judged.  Copies `bytes` (at
+   copy on the current GPU, the reference point for the HBM-bound passes.  Copies `bytes` (at
    least 1 MiB; two buffers of that size are allocated and freed) with 16-byte accesses for each of
    12 launch shapes (1, 2 or 4 accesses in flight per lane; 4, 8, 16 or 32 blocks of 256 threads
    per compute unit), `reps` timed launches each after a warm-up, and returns the best launch's
