@@ -228,30 +228,30 @@ def test_mixed_precision_against_the_oracle(tmp_path, oracle_lib):
         assert rel(g.field("rho_s"), o.field("rho_s")) < 1e-5
 
 
-def test_output_files_match_oracle(tmp_path, oracle_lib):
-    """vm.out / dd.out / err.out written by the GPU path carry the oracle's numbers (%e)."""
-    deck = make(tmp_path, n=(8, 8, 8), grains=3, seed=7, nsteps=3)
+@pytest.mark.parametrize("n,seed,fixed", [((8, 8, 8), 7, 6), ((16, 16, 16), 3, 6), ((8, 8, 8), 7, 0),
+                                          ((12, 10, 8), 11, 0)])
+def test_output_files_match_oracle(tmp_path, oracle_lib, n, seed, fixed):
+    """vm.out / dd.out / err.out written by the GPU path are the oracle's files byte for byte (the
+    reference's headers and %e columns), at a fixed number of outer iterations and converged."""
+    deck = make(tmp_path, n=n, grains=3, seed=seed, nsteps=3)
     go, oo = tmp_path / "gpu", tmp_path / "orc"
     go.mkdir()
     oo.mkdir()
     for f in ("Cu_p.sx", "mesh.txt"):
         shutil.copy(tmp_path / f, go / f)
         shutil.copy(tmp_path / f, oo / f)
-    g = srx_amd.Solver(deck, workdir=str(go), write_files=1, fixed_iters=6)
-    o = oracle_lib.Oracle(deck, workdir=str(oo), write_files=1, fixed_iters=6)
+    g = srx_amd.Solver(deck, workdir=str(go), write_files=1, fixed_iters=fixed)
+    o = oracle_lib.Oracle(deck, workdir=str(oo), write_files=1, fixed_iters=fixed)
     for _ in range(3):
         g.step()
         o.step()
     g.close()
     o.close()
-    for name in ("vm.out", "dd.out"):
-        A = open(go / name).read().splitlines()
-        B = open(oo / name).read().splitlines()
-        assert A[0] == B[0] and len(A) == len(B) == 4
-        a = np.array([[float(x) for x in l.split("\t") if x] for l in A[1:]])
-        b = np.array([[float(x) for x in l.split("\t") if x] for l in B[1:]])
-        np.testing.assert_allclose(a, b, rtol=2e-6)
-    assert open(go / "err.out").read() == open(oo / "err.out").read()
+    for name in ("vm.out", "dd.out", "err.out"):
+        A = open(go / name).read()
+        B = open(oo / name).read()
+        assert len(A.splitlines()) == 4, name
+        assert A == B, name
 
 
 def test_nan_is_reported_not_aborted(tmp_path):
