@@ -1450,13 +1450,10 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         }
     }
 
-    if (!FIRST && WS == 1) {
-        // extrapolated start: lam + (lam - lam of the previous outer iteration), the change
-        // kept in the crystal frame of this increment's orientation
-#pragma unroll
-        for (int i = 0; i < 5; ++i) ws[i] = (double)ldcf(f.dsig + (size_t)i * N, N, 4u * (unsigned)v);
-        wsh = (double)ldcf(f.dsig + 5 * N, N, 4u * (unsigned)v);
-    } else if (!FIRST && WS == 2) {
+    if (!FIRST && WS != 0) {
+        // extrapolated start (WS 1): lam + (lam - lam of the previous outer iteration), the change
+        // kept in the crystal frame of this increment's orientation, loaded by the caller with
+        // the other inputs; learned start (WS 2): the network's offset
 #pragma unroll
         for (int i = 0; i < 5; ++i) ws[i] = (double)wsp[i];
         wsh = (double)wsp[5];
@@ -1772,6 +1769,10 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
         const int vv = valid ? v : nvox - 1;
         NrIn in;
         nr_load<true>(f, N, 8u * (unsigned)vv, in);
+        if constexpr (WS == 1) {
+#pragma unroll
+            for (int k = 0; k < 6; ++k) wsp[k] = ldcf(f.dsig + (size_t)k * N, N, 4u * (unsigned)vv);
+        }
         if constexpr (WS == 2)
             nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, in, wsp);
         copy_field<NRC_COMPS>(f.nrc, nrc_pitch(N), lds, blk);
