@@ -1591,7 +1591,8 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     for (int i = 0; i < 5; ++i) cs2 += (sg[i] - lam[i]) * (sg[i] - lam[i]);
     res.cs = sqrt(cs2);
     {
-        // the staging slots now hold the per-system constants: the orientation from global memory
+        // the orientation again (the registers that held it were released for the Newton loop and
+        // the LDS holds the per-system constants)
         double a[9];
 #pragma unroll
         for (int k = 0; k < 9; ++k) a[k] = ldc(f.tran + (size_t)(k) * N, N, o8);
@@ -1750,12 +1751,8 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
-    // After the first call of an increment, the wavefront's inputs (stress, plastic strain,
-    // DisGrad, orientation) are copied to LDS by one burst of asynchronous global->LDS loads and a
-    // single wait, instead of several dependent round trips to HBM.  Once every lane has formed
-    // its prologue from them, the per-system constants the first call stored overwrite the
-    // staging slots (by a second burst), so the block needs no more LDS than the larger of the
-    // two.  The first call computes both and keeps its lagged tau_pass in four more slices.
+    // LDS: the 36 per-system constants of the block's 64 voxels (18 KB); the first call of an
+    // increment computes them here, the later ones copy them from f.nrc.
     constexpr int NSLOT = NRC_COMPS;
     __shared__ __align__(16) double lds[NSLOT * NR_BLOCK];
     NrPro pro;
