@@ -1,15 +1,19 @@
 // kernels.hip -- CDNA4 (gfx950) kernels of the EVP-FFT increment loop.
 //
 // FFT passes (evolution.c:1157-1201 replaced): LDS-resident Stockham
-// radix-2/3/4/5 transforms over batches of pencils, 256 threads per block.
+// radix-2/3/4/5/8 transforms over batches of pencils, 256 threads per block, their global
+// loads issued in chunks before the first wait.
 //   fwd_z   real [c][x][y][z]      -> half spectrum B [c][kz][x][y]   (two real comps per complex FFT)
 //   fft_y   B <-> C [c][kz][y][x]  (the y transform doubles as the pencil transpose)
-//   green_x C(6 stress comps) -> forward x -> Green operator -> inverse x -> C(9 comps), in place
+//   green   C(6 stress comps) -> forward x -> Green operator -> inverse x -> C(9 comps), in place
+//           (k_green_wave: one wavefront per row, cross-lane butterflies, for x extents 64/128/256;
+//           k_green_x: LDS transforms otherwise)
 //   inv_z   B(9 comps) -> c2r along z, fused with the FP64 DisGrad update (Eq. 15)
-// Per-voxel kernels (one voxel per lane, FP64):
+// Per-voxel kernels (one voxel per lane, FP64; fields read and written as raw buffer accesses,
+// one 32-bit lane offset from a scalar base):
 //   nr      the constitutive Newton solve of UpdateStress (evolution.c:157-427)
 //   end     Edot/Eps update, update_orient, DislocationEvolution (evolution.c:1268-1362)
-// Block partial sums are reduced by reduce_partials in a fixed order, so every
+// Block partial sums are reduced in a fixed order (k_reduce, k_reduce_seg, k_control), so every
 // sum is bitwise repeatable from run to run.
 #include "srx_dev.hpp"
 #include "mlp_dev.hpp"
