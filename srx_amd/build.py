@@ -1,6 +1,7 @@
 """Build libsrx.so (HIP kernels + host driver) and the `srx` CLI for gfx950, in-tree."""
 from __future__ import annotations
 
+import concurrent.futures
 import os
 import subprocess
 import sys
@@ -14,6 +15,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # Packed-FP32 instructions are disabled for all device code.
 DEV_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
+DEV_SOURCES = ["newton.hip", "increment.hip", "fft.hip", "mlp.hip", "noalloc.hip"]
 HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-fopenmp", "-Wall", "-Wno-unused-function"]
 
 
@@ -25,17 +27,19 @@ def run(cmd):
 def build(verbose_resources: bool = False) -> None:
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    kobj = os.path.join(objdir, "kernels.o")
     hobj = os.path.join(objdir, "host.o")
     extra = ["-Rpass-analysis=kernel-resource-usage"] if verbose_resources else []
-    mobj = os.path.join(objdir, "mlp.o")
-    nobj = os.path.join(objdir, "noalloc.o")
-    run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "kernels.hip"), "-o", kobj])
-    run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "mlp.hip"), "-o", mobj])
-    run([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, "noalloc.hip"), "-o", nobj])
-    run([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
-         os.path.join(CSRC, "host.cpp"), "-o", hobj])
-    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", LIB, kobj, mobj, nobj, hobj])
+    # the device sources compile independently; the Newton file takes longest
+    cmds, objs = [], []
+    for src in DEV_SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmds.append([HIPCC, *DEV_FLAGS, *extra, "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    cmds.append([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", "-c",
+                 os.path.join(CSRC, "host.cpp"), "-o", hobj])
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        list(ex.map(run, cmds))
+    run([HIPCC, "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", LIB, *objs, hobj])
     run([HIPCC, "-x", "c++", *HOST_FLAGS, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include",
          os.path.join(CSRC, "cli.cpp"), "-o", CLI, "-L" + HERE, "-lsrx", "-Wl,-rpath,$ORIGIN",
          "-L/opt/rocm/lib", "-lamdhip64"])

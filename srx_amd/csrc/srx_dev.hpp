@@ -261,7 +261,7 @@ bool make_plan(int n, FftPlan &p);
 bool srx_grid_supported(const FftPlan pl[3], const int n[3], int prec);
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
 
-// ---- launch wrappers (kernels.hip) -------------------------------------
+// ---- launch wrappers (fft.hip, newton.hip, increment.hip) ---------------
 // Spectrum buffers: specB [c][kz][x][y], specC [c][kz][y][x]; T = double or float.
 // With a non-null `ctl` the iteration's scalars (mean stress, k = 0 sums, dDisGradAvg) come from
 // the device control block and the kernels return at once when its `done` is set.
@@ -307,7 +307,7 @@ int gnd_block_count(int nvox);
 void launch_gnd(hipStream_t s, const DevConst *cst, const DevConst &hc, const Fields &f, int nvox, int grad,
                 double *partials);
 // Learned Newton start: operand packing (mlp.hip) and the recording of training samples around a
-// Newton call (kernels.hip): `after` = 0 writes the features of the sampled voxels before the
+// Newton call (newton.hip): `after` = 0 writes the features of the sampled voxels before the
 // call, 1 their targets after it and advances the sample count.
 void mlp_pack(hipStream_t s, const float *dblob, MlpPacked *M);
 size_t mlp_packed_bytes();
