@@ -82,7 +82,8 @@ typedef struct {
     int    reserved[7];      /* [0]: kernel timing (srx_kernel_times); [1] = 1: enqueue the outer
                                 iterations launch by launch instead of replaying each batch of 8
                                 from a HIP graph captured on first use (the default); [2] = 1: keep
-                                the default FFT tiles instead of timing the candidates at srx_open */
+                                the default FFT tiles instead of timing the candidates at srx_open;
+                                [3] = 1: also time the FFT radix plans (srx_fft_plan) */
 } srx_options;
 
 /* Everything the reference prints or writes per increment, plus counters it
@@ -244,6 +245,31 @@ int srx_mlp_samples(srx_ctx *ctx, float *x, float *t, size_t max, size_t *n);
    n0 = 64, 128 and 256, where it is also the default) and their timed launch durations in
    microseconds. */
 int srx_fft_tiles(srx_ctx *ctx, int *tiles, double *us);
+
+/* Most radix stages of one FFT plan. */
+#define SRX_FFT_MAX_STAGES 12
+
+/* FFT radix plans (no counterpart in the reference, whose FFTW plans are its own).  A plan is the
+   sequence of radices (2, 3, 4, 5 or 8) of the Stockham stages of one axis (0: x, 1: y, 2: z); the
+   default is one per extent (radix 8 as often as possible, then 4, one 2, 3s and 5s).  With
+   options.reserved[3] = 1, srx_open also times the candidates of srx_fft_plan_candidates on each
+   axis whose passes take a plan (not x where the wavefront Green pass runs) and keeps the fastest,
+   the same plan for every context of one grid shape and precision in the process.  Another radix
+   order rounds differently: results then differ from the default plans within the FFT's rounding
+   error, not bitwise.  The x pass of n0 = 64, 128 and 256 takes no plan (one wavefront per row). */
+int srx_fft_plan(srx_ctx *ctx, int axis, int *radices, int *nstages);
+
+/* Sets the plan of one axis: radices[0..nstages-1] from {2, 3, 4, 5, 8} whose product is the
+   extent (else SRX_E_ARG), within the passes' LDS and butterfly limits (else SRX_E_GRID).  That
+   axis's passes return to their default tiles, and the next increment captures its batches anew. */
+int srx_set_fft_plan(srx_ctx *ctx, int axis, const int *radices, int nstages);
+
+/* Candidate plans for extent n (host only, no GPU needed), the default first: every split of the
+   power of two into radix-8 and radix-4 stages with at most one radix-2 stage, times the 3s and 5s,
+   in the default order and reversed.  Writes up to max_plans rows of SRX_FFT_MAX_STAGES ints
+   (radices, then 0s) and their total number in *nplans; SRX_E_SIZE when there are more than
+   max_plans, SRX_E_GRID when n is not 2^a 3^b 5^c up to 1024. */
+int srx_fft_plan_candidates(int n, int *radices, int max_plans, int *nplans);
 
 /* Measurement helper (no counterpart in the reference): the HBM ceiling of a plain streaming
    copy on the current GPU, the reference point for the HBM-bound passes.  Copies `bytes` (at

@@ -23,6 +23,8 @@ LIB_PATH = os.path.join(HERE, "libsrx.so")
 
 PREC_FP64 = 0
 PREC_MIXED = 1
+SRX_E_GRID = 2002
+SRX_E_SIZE = 2008
 
 
 class Options(C.Structure):
@@ -67,9 +69,11 @@ FIELDS = {"sig": (0, 9), "eps": (1, 9), "edot": (2, 9), "disgrad": (3, 9), "velg
 EXPORTS = ["srx_open", "srx_step", "srx_run", "srx_get_field", "srx_set_field", "srx_dims", "srx_close",
            "srx_strerror", "srx_default_options", "srx_kernel_times", "srx_fft_forward", "srx_fft_inverse",
            "srx_run_ensemble", "srx_mlp_default_train_options", "srx_mlp_train", "srx_mlp_grad",
-           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples", "srx_fft_tiles", "srx_hbm_stream"]
+           "srx_mlp_forward", "srx_mlp_set", "srx_mlp_record", "srx_mlp_samples", "srx_fft_tiles", "srx_hbm_stream",
+           "srx_fft_plan", "srx_set_fft_plan", "srx_fft_plan_candidates"]
 
 MLP_BLOB = 6210     # SRX_MLP_BLOB
+FFT_MAX_STAGES = 12   # SRX_FFT_MAX_STAGES
 MLP_NPARAM = 6150   # SRX_MLP_NPARAM
 
 
@@ -124,6 +128,9 @@ def lib():
         L.srx_mlp_samples.argtypes = [C.c_void_p, F, F, C.c_size_t, P(C.c_size_t)]
         L.srx_fft_tiles.argtypes = [C.c_void_p, P(C.c_int), P(C.c_double)]
         L.srx_hbm_stream.argtypes = [C.c_size_t, C.c_int, P(C.c_double), P(C.c_double), P(C.c_int), P(C.c_int)]
+        L.srx_fft_plan.argtypes = [C.c_void_p, C.c_int, P(C.c_int), P(C.c_int)]
+        L.srx_set_fft_plan.argtypes = [C.c_void_p, C.c_int, P(C.c_int), C.c_int]
+        L.srx_fft_plan_candidates.argtypes = [C.c_int, P(C.c_int), C.c_int, P(C.c_int)]
         _lib = L
     return _lib
 
@@ -143,6 +150,21 @@ class SrxError(RuntimeError):
         self.code = code
         msg = lib().srx_strerror(code).decode()
         super().__init__("%s: %s (code %d)" % (where, msg, code))
+
+
+def fft_plan_candidates(n: int) -> list:
+    """Radix plans the FFT autotuner may time for extent n, the default first (host only)."""
+    L = lib()
+    cnt = C.c_int()
+    rc = L.srx_fft_plan_candidates(int(n), None, 0, C.byref(cnt))
+    if rc not in (0, SRX_E_SIZE):
+        raise SrxError(rc, "srx_fft_plan_candidates")
+    buf = (C.c_int * (cnt.value * FFT_MAX_STAGES))()
+    rc = L.srx_fft_plan_candidates(int(n), buf, cnt.value, C.byref(cnt))
+    if rc:
+        raise SrxError(rc, "srx_fft_plan_candidates")
+    rows = [list(buf[i * FFT_MAX_STAGES:(i + 1) * FFT_MAX_STAGES]) for i in range(cnt.value)]
+    return [[x for x in r if x] for r in rows]
 
 
 def options(**kw) -> Options:
@@ -216,6 +238,22 @@ class Solver:
         if rc:
             raise SrxError(rc, "srx_fft_tiles")
         return list(t), list(us)
+
+    def fft_plan(self, axis: int) -> list:
+        """Radices of the FFT plan of one axis (0: x, 1: y, 2: z)."""
+        r = (C.c_int * FFT_MAX_STAGES)()
+        ns = C.c_int()
+        rc = self.L.srx_fft_plan(self.h, int(axis), r, C.byref(ns))
+        if rc:
+            raise SrxError(rc, "srx_fft_plan")
+        return list(r)[:ns.value]
+
+    def set_fft_plan(self, axis: int, radices) -> None:
+        """Run the passes of one axis with these radices (product: the extent)."""
+        r = (C.c_int * max(1, len(radices)))(*[int(x) for x in radices])
+        rc = self.L.srx_set_fft_plan(self.h, int(axis), r, len(radices))
+        if rc:
+            raise SrxError(rc, "srx_set_fft_plan")
 
     def mlp_set(self, blob: np.ndarray) -> None:
         """Install the learned Newton start (context opened with warm_start=2)."""

@@ -268,20 +268,32 @@ __device__ __forceinline__ void lds_fft_pow2(cx<T> *buf, int npen, const cx<T> *
     }
 }
 
+// whether plan pl is make_plan's schedule for the power-of-two length N (the one the compile-time
+// stages run); a tuned or caller-set plan with other radices takes the runtime stages
+template <int N>
+__device__ __forceinline__ bool pow2_default(const FftPlan &pl) {
+    constexpr int ns = pow2_radix(N, 0) == 0 ? 0 : pow2_radix(N, 1) == 0 ? 1 : pow2_radix(N, 2) == 0 ? 2
+                     : pow2_radix(N, 3) == 0 ? 3 : 4;
+    bool same = pl.nst == ns;
+#pragma unroll
+    for (int st = 0; st < ns; ++st) same = same && pl.rad[st] == pow2_radix(N, st);
+    return same;
+}
+
 // Full 1-D transform of npen LDS-resident pencils.  Caller guarantees
 // npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.
-// POW2: lengths 32..256 use the compile-time stages (measured faster in the x and y passes;
-// in the z passes, which interleave the transform with the real/complex packing, the extra
-// registers cost an occupancy step and the runtime stages are faster).
+// POW2: lengths 32..256 with make_plan's radices use the compile-time stages (measured faster in
+// the x and y passes; in the z passes, which interleave the transform with the real/complex
+// packing, the extra registers cost an occupancy step and the runtime stages are faster).
 template <typename T, int MAXB, bool POW2 = true>
 __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign,
                         int pitch) {
     if constexpr (POW2) {
         switch (pl.n) {
-        case 32: lds_fft_pow2<T, MAXB, 32, 0, 1>(buf, npen, tw, sign, pitch); return;
-        case 64: lds_fft_pow2<T, MAXB, 64, 0, 1>(buf, npen, tw, sign, pitch); return;
-        case 128: lds_fft_pow2<T, MAXB, 128, 0, 1>(buf, npen, tw, sign, pitch); return;
-        case 256: lds_fft_pow2<T, MAXB, 256, 0, 1>(buf, npen, tw, sign, pitch); return;
+        case 32: if (pow2_default<32>(pl)) { lds_fft_pow2<T, MAXB, 32, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
+        case 64: if (pow2_default<64>(pl)) { lds_fft_pow2<T, MAXB, 64, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
+        case 128: if (pow2_default<128>(pl)) { lds_fft_pow2<T, MAXB, 128, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
+        case 256: if (pow2_default<256>(pl)) { lds_fft_pow2<T, MAXB, 256, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
         default: break;
         }
     }
@@ -979,6 +991,47 @@ bool make_plan(int n, FftPlan &p) {
     while (m % 5 == 0) { p.rad[p.nst++] = 5; m /= 5; }
     if (p.nst > MAX_RADIX_STAGES) return false;
     return m == 1;
+}
+
+// Radix plans of length n the tuner may time: make_plan's first, then every split of the power of
+// two into radix-8 and radix-4 stages with at most one radix-2 stage (more radix-2 stages only add
+// passes over LDS), each in make_plan's order (8s, 4s, the 2, then 3s and 5s) and in the reverse
+// order.  Returns the number written to out (at most cap); 0 when make_plan rejects n.
+int plan_candidates(int n, FftPlan *out, int cap) {
+    FftPlan d;
+    if (cap < 1 || !make_plan(n, d)) return 0;
+    int cnt = 0;
+    auto add = [&](const FftPlan &p) {
+        for (int i = 0; i < cnt; ++i) {
+            bool same = out[i].nst == p.nst;
+            for (int s = 0; same && s < p.nst; ++s) same = out[i].rad[s] == p.rad[s];
+            if (same) return;
+        }
+        if (cnt < cap) out[cnt++] = p;
+    };
+    add(d);
+    int m = n, k2 = 0, k3 = 0, k5 = 0;
+    while (m % 2 == 0) { ++k2; m /= 2; }
+    while (m % 3 == 0) { ++k3; m /= 3; }
+    while (m % 5 == 0) { ++k5; m /= 5; }
+    for (int n8 = k2 / 3; n8 >= 0; --n8)
+        for (int n4 = (k2 - 3 * n8) / 2; n4 >= 0; --n4) {
+            const int n2 = k2 - 3 * n8 - 2 * n4;
+            if (n2 > 1 || n8 + n4 + n2 + k3 + k5 > MAX_RADIX_STAGES) continue;
+            FftPlan p;
+            p.n = n;
+            p.nst = 0;
+            for (int i = 0; i < n8; ++i) p.rad[p.nst++] = 8;
+            for (int i = 0; i < n4; ++i) p.rad[p.nst++] = 4;
+            for (int i = 0; i < n2; ++i) p.rad[p.nst++] = 2;
+            for (int i = 0; i < k3; ++i) p.rad[p.nst++] = 3;
+            for (int i = 0; i < k5; ++i) p.rad[p.nst++] = 5;
+            add(p);
+            FftPlan r = p;
+            for (int s = 0; s < p.nst; ++s) r.rad[s] = p.rad[p.nst - 1 - s];
+            add(r);
+        }
+    return cnt;
 }
 
 // LDS budget of one FFT block (bytes)

@@ -12,11 +12,14 @@
 #include <omp.h>
 
 #include <algorithm>
+#include <array>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
 #include <atomic>
 #include <thread>
@@ -886,26 +889,132 @@ const char *srx_strerror(int code) {
     }
 }
 
-// FFT tile autotuner: each pass runs on the context's own buffers (the inverse z pass into a
-// scratch DisGrad) with every candidate tile (powers of two up to the largest valid one, and that
-// one), three timed launches after a warm-up, and keeps the fastest by a 3% margin over the
-// default.
-static int autotune_tiles(srx_ctx *c) {
+// Average time (us) of three launches of FFT pass `pass` (0 forward z, 1 forward y, 2 Green x,
+// 3 inverse y, 4 inverse z) at tile `t`, after one warm-up launch, on the context's own buffers with
+// its current plans (the inverse z pass writes the scratch DisGrad `scratch`).  With a `flush`
+// buffer (2 x FLUSH_BYTES) each launch is timed alone after a streaming copy through it, so the
+// pass reads its input from HBM as it does in the solver, where the Newton kernel's traffic has
+// evicted it from the last-level cache; else the three launches run back to back.
+constexpr size_t FLUSH_BYTES = (size_t)320 << 20;
+static int time_pass(srx_ctx *c, int pass, int t, double *scratch, hipEvent_t e0, hipEvent_t e1, double &us,
+                     char *flush = nullptr) {
     const int prec = c->opt.precision;
     hipStream_t st = c->stream;
+    auto run = [&]() {
+        switch (pass) {
+        case 0: launch_fft_fwd_z(st, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], nullptr, t); break;
+        case 1: launch_fft_y(st, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], nullptr, t); break;
+        case 2: launch_green_x(st, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, nullptr, t); break;
+        case 3: launch_fft_y(st, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], nullptr, t); break;
+        default:
+            launch_fft_inv_z_update(st, prec, c->specB, scratch, c->n, c->pl[2], c->tw[2], nullptr, (double)c->nvox,
+                                    t);
+        }
+    };
+    run();
+    float ms = 0.0f;
+    if (flush) {
+        double tot = 0.0;
+        for (int r = 0; r < 3; ++r) {
+            launch_stream_copy(st, flush, flush + FLUSH_BYTES, FLUSH_BYTES / 16, 1, 2048);
+            if (hipEventRecord(e0, st) != hipSuccess) return SRX_E_HIP;
+            run();
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess)
+                return SRX_E_HIP;
+            tot += ms;
+        }
+        us = 1000.0 * tot / 3.0;
+        return 0;
+    }
+    if (hipEventRecord(e0, st) != hipSuccess) return SRX_E_HIP;
+    for (int r = 0; r < 3; ++r) run();
+    if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+        hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess)
+        return SRX_E_HIP;
+    us = 1000.0 * ms / 3.0;
+    return 0;
+}
+
+static_assert(SRX_FFT_MAX_STAGES == MAX_RADIX_STAGES, "plan stage count of the ABI");
+
+// FFT passes of each plan axis (x: the Green pass; y and z: forward and inverse)
+static const int AXIS_PASSES[3][2] = {{2, -1}, {1, 3}, {0, 4}};
+
+// Radix plans chosen by timing (options.reserved[3] = 1), kept per process for the grid shape and
+// precision: every context of one shape, e.g. the members of an ensemble, runs the same plans.
+static std::mutex g_plan_mu;
+static std::map<std::array<int, 5>, FftPlan> g_plan_cache;
+
+// FFT autotuner, at srx_open on the context's own buffers.
+// Radix plans (opt-in, options.reserved[3] = 1): for each axis whose passes take a plan (not the
+// x axis where the wavefront Green pass runs), every candidate of plan_candidates that the passes
+// accept is timed over that axis's passes at their default tiles, each launch alone after a
+// streaming copy that evicts its input from the caches, in two rounds (the faster one counts);
+// another plan replaces make_plan's only when more than 5% faster, since the timings at
+// srx_open vary by a few percent from one context to the next.  Another radix order is another
+// rounding order, so results then differ from the default plans in the last bits (within the FFT's
+// rounding error), which is why the default keeps make_plan's plans.
+// Tiles (unless options.reserved[2] = 1): each pass is timed with every candidate tile (powers of
+// two up to the largest valid one, and that one), the default (the largest) first; another must be
+// more than 3% faster to replace it, so near-ties do not flip between runs.  Tiles change only how many pencils a block
+// transforms, which leaves every result bit-identical; the wavefront x pass (a different operation
+// order) is not timed against the LDS kernel but always used where it applies, so results never
+// depend on the tile timings (or on what else the GPU runs meanwhile).
+static int autotune_fft(srx_ctx *c) {
+    const int prec = c->opt.precision;
     double *scratch = nullptr;
     HIPCK(hipMalloc(&scratch, sizeof(double) * 9 * (size_t)c->nvox));
     hipEvent_t e0 = nullptr, e1 = nullptr;
     int rc = 0;
     if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) rc = SRX_E_HIP;
-    for (int pass = 0; pass < 5 && !rc; ++pass) {
+    char *flush = nullptr;
+    if (!rc && c->opt.reserved[3] == 1 && hipMalloc(&flush, 2 * FLUSH_BYTES) != hipSuccess) rc = SRX_E_NOMEM;
+    for (int ax = 0; ax < 3 && !rc && c->opt.reserved[3] == 1; ++ax) {
+        if (ax == 0 && green_wave_ok(c->n)) continue;   // the wavefront pass takes no plan
+        const std::array<int, 5> key = {ax, c->n[0], c->n[1], c->n[2], prec};
+        {
+            std::lock_guard<std::mutex> lk(g_plan_mu);
+            auto it = g_plan_cache.find(key);
+            if (it != g_plan_cache.end()) {
+                c->pl[ax] = it->second;
+                continue;
+            }
+        }
+        FftPlan cand[32];
+        const int nc = plan_candidates(c->n[ax], cand, 32);
+        const FftPlan dflt = c->pl[ax];
+        double t_best[32];
+        for (int i = 0; i < nc; ++i) t_best[i] = 1e300;
+        // two rounds over the candidates, each keeping its faster round, so that a slow spell of
+        // the GPU does not favour the candidates timed after it
+        for (int round = 0; round < 2 && !rc; ++round)
+            for (int i = 0; i < nc && !rc; ++i) {
+                FftPlan trial[3] = {c->pl[0], c->pl[1], c->pl[2]};
+                trial[ax] = cand[i];
+                if (!grid_supported(trial, c->n, prec)) continue;
+                c->pl[ax] = cand[i];
+                double tot = 0.0;
+                for (int k = 0; k < 2 && !rc; ++k) {
+                    double us = 0.0;
+                    if (AXIS_PASSES[ax][k] >= 0) rc = time_pass(c, AXIS_PASSES[ax][k], 0, scratch, e0, e1, us, flush);
+                    tot += us;
+                }
+                t_best[i] = std::min(t_best[i], tot);
+            }
+        c->pl[ax] = dflt;
+        if (rc) break;
+        // make_plan's plan (candidate 0) unless another is more than 5% faster: then the fastest
+        int bi = 0;
+        for (int i = 1; i < nc; ++i)
+            if (t_best[i] < 0.95 * t_best[0] && (bi == 0 || t_best[i] < t_best[bi])) bi = i;
+        const FftPlan best_plan = cand[bi];
+        std::lock_guard<std::mutex> lk(g_plan_mu);
+        c->pl[ax] = g_plan_cache.emplace(key, best_plan).first->second;   // a context that tuned first wins
+    }
+    for (int pass = 0; pass < 5 && !rc && c->opt.reserved[2] == 0; ++pass) {
         const FftPlan &p = c->pl[pass == 2 ? 0 : (pass == 1 || pass == 3) ? 1 : 2];
         const int tmax = fft_tile_max(pass, c->n, p, prec);
-        // the default first (the largest tile); another candidate must be more than 3% faster to
-        // replace it, so near-ties do not flip between runs.  Candidates differ only in pencils per
-        // block, which leaves every result bit-identical; the wavefront x pass (a different
-        // operation order) is not timed against the LDS kernel but always used where it applies, so
-        // results never depend on the timings (or on what else the GPU runs meanwhile).
         std::vector<int> cand;
         if (pass == 2 && green_wave_ok(c->n)) cand.push_back(-1);   // one wavefront per row
         else {
@@ -914,27 +1023,8 @@ static int autotune_tiles(srx_ctx *c) {
         }
         double best = 1e300;
         for (int t : cand) {
-            auto run = [&]() {
-                switch (pass) {
-                case 0: launch_fft_fwd_z(st, prec, c->f.sig, c->specB, c->n, c->pl[2], c->tw[2], nullptr, t); break;
-                case 1: launch_fft_y(st, prec, c->specB, c->specC, 6, -1, 1, c->n, c->pl[1], c->tw[1], nullptr, t); break;
-                case 2: launch_green_x(st, prec, c->specC, c->n, c->pl[0], c->tw[0], c->dc, nullptr, t); break;
-                case 3: launch_fft_y(st, prec, c->specC, c->specB, 9, +1, 0, c->n, c->pl[1], c->tw[1], nullptr, t); break;
-                default:
-                    launch_fft_inv_z_update(st, prec, c->specB, scratch, c->n, c->pl[2], c->tw[2], nullptr,
-                                            (double)c->nvox, t);
-                }
-            };
-            run();
-            float ms = 0.0f;
-            if (hipEventRecord(e0, st) != hipSuccess) { rc = SRX_E_HIP; break; }
-            for (int r = 0; r < 3; ++r) run();
-            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-                hipEventElapsedTime(&ms, e0, e1) != hipSuccess || hipGetLastError() != hipSuccess) {
-                rc = SRX_E_HIP;
-                break;
-            }
-            const double us = 1000.0 * ms / 3.0;
+            double us = 0.0;
+            if ((rc = time_pass(c, pass, t, scratch, e0, e1, us))) break;
             if (us < (best < 1e300 ? 0.97 * best : best)) {
                 best = us;
                 c->tiles[pass] = t;
@@ -945,10 +1035,17 @@ static int autotune_tiles(srx_ctx *c) {
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);
     (void)hipFree(scratch);
-    if (c->opt.verbose && !rc)
+    if (flush) (void)hipFree(flush);
+    if (c->opt.verbose && !rc) {
         std::printf("fft tiles (fwd z, fwd y, green x, inv y, inv z): %d %d %d %d %d  (%.1f %.1f %.1f %.1f %.1f us)\n",
                     c->tiles[0], c->tiles[1], c->tiles[2], c->tiles[3], c->tiles[4], c->tile_us[0], c->tile_us[1],
                     c->tile_us[2], c->tile_us[3], c->tile_us[4]);
+        for (int ax = 0; ax < 3; ++ax) {
+            std::printf("fft plan axis %d (n = %d):", ax, c->pl[ax].n);
+            for (int s = 0; s < c->pl[ax].nst; ++s) std::printf(" %d", c->pl[ax].rad[s]);
+            std::printf("\n");
+        }
+    }
     return rc;
 }
 
@@ -959,6 +1056,55 @@ int srx_fft_tiles(srx_ctx *c, int *tiles, double *us) {
         if (us) us[k] = c->tile_us[k];
     }
     return 0;
+}
+
+int srx_fft_plan(srx_ctx *c, int axis, int *radices, int *nstages) {
+    if (!c || !radices || !nstages || axis < 0 || axis > 2) return SRX_E_ARG;
+    *nstages = c->pl[axis].nst;
+    for (int s = 0; s < c->pl[axis].nst; ++s) radices[s] = c->pl[axis].rad[s];
+    return 0;
+}
+
+static void drop_graphs(srx_ctx *c);
+
+int srx_set_fft_plan(srx_ctx *c, int axis, const int *radices, int nstages) {
+    if (!c || !radices || axis < 0 || axis > 2 || nstages < 1 || nstages > SRX_FFT_MAX_STAGES) return SRX_E_ARG;
+    FftPlan p;
+    p.n = c->n[axis];
+    p.nst = nstages;
+    long prod = 1;
+    for (int s = 0; s < nstages; ++s) {
+        const int r = radices[s];
+        if (r != 2 && r != 3 && r != 4 && r != 5 && r != 8) return SRX_E_ARG;
+        p.rad[s] = r;
+        prod *= r;
+        if (prod > p.n) return SRX_E_ARG;
+    }
+    if (prod != p.n) return SRX_E_ARG;
+    FftPlan trial[3] = {c->pl[0], c->pl[1], c->pl[2]};
+    trial[axis] = p;
+    if (!grid_supported(trial, c->n, c->opt.precision)) return SRX_E_GRID;
+    HIPCK(hipStreamSynchronize(c->stream));
+    c->pl[axis] = p;
+    for (int k = 0; k < 2; ++k)
+        if (AXIS_PASSES[axis][k] >= 0) {
+            c->tiles[AXIS_PASSES[axis][k]] = 0;   // the default tile of the new plan
+            c->tile_us[AXIS_PASSES[axis][k]] = 0.0;
+        }
+    drop_graphs(c);   // captured batches hold the previous plan
+    return 0;
+}
+
+int srx_fft_plan_candidates(int n, int *radices, int max_plans, int *nplans) {
+    if (!nplans || max_plans < 0 || (max_plans > 0 && !radices)) return SRX_E_ARG;
+    FftPlan cand[32];
+    const int nc = plan_candidates(n, cand, 32);
+    if (nc == 0) return SRX_E_GRID;
+    *nplans = nc;
+    for (int i = 0; i < nc && i < max_plans; ++i)
+        for (int s = 0; s < SRX_FFT_MAX_STAGES; ++s)
+            radices[i * SRX_FFT_MAX_STAGES + s] = s < cand[i].nst ? cand[i].rad[s] : 0;
+    return nc > max_plans ? SRX_E_SIZE : 0;
 }
 
 int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_ctx **out) {
@@ -977,7 +1123,7 @@ int srx_open(const char *deck, const char *workdir, const srx_options *opt, srx_
     c->nkt = 8;
     int rc = read_deck(deck, c->dk);
     if (!rc) rc = setup(c);
-    if (!rc && c->opt.reserved[2] == 0) rc = autotune_tiles(c);
+    if (!rc && (c->opt.reserved[2] == 0 || c->opt.reserved[3] == 1)) rc = autotune_fft(c);
     if (rc) {
         srx_close(c);
         return rc;

@@ -257,6 +257,8 @@ constexpr int NR_NPART = 18;   // conv_s, conv_e, sig6[6], sig6_ph1[6], nr_sum, 
 constexpr int END_NPART = 24;  // eps6[6], edot6[6], rhoF, rhoP, rhos, rhom, rhodip, rhofor, rhoinp, rsl, rlc, nan, pad..
 
 bool make_plan(int n, FftPlan &p);
+// radix plans of length n the autotuner may time, make_plan's first (fft.hip)
+int plan_candidates(int n, FftPlan *out, int cap);
 // whether the FFT passes take an n0 x n1 x n2 grid in precision prec (host.cpp)
 bool srx_grid_supported(const FftPlan pl[3], const int n[3], int prec);
 void set_launch_events(hipEvent_t start, hipEvent_t stop);
