@@ -1279,7 +1279,7 @@ static int write_field_dumps(srx_ctx *c, int step) {
 // ensemble can run the outer iterations of all its members together (run_lockstep):
 // step_begin: the increment prologue and the control block of its outer iterations;
 // step_iterate: the outer iterations of this context alone; step_finish: everything after them.
-static int step_begin(srx_ctx *c) {
+static int step_begin(srx_ctx *c, EnsIncr *batch = nullptr) {
     const Deck &d = c->dk;
     const double dt = d.TimeStep;
     const int prec = c->opt.precision;
@@ -1293,7 +1293,16 @@ static int step_begin(srx_ctx *c) {
         add[k] = c->Udot[k] * dt;
         set[k] = c->DisGradAvg[k] + c->acum[k];
     }
-    launch_prologue(c->stream, c->f, N, add, c->creep, set, c->uniform);
+    if (batch) {   // a lockstep ensemble launches every member's prologue at once
+        for (int k = 0; k < 9; ++k) {
+            batch->add[k] = add[k];
+            batch->set[k] = set[k];
+            batch->uold[k] = c->uniform[k];
+        }
+        batch->creep = c->creep;
+    } else {
+        launch_prologue(c->stream, c->f, N, add, c->creep, set, c->uniform);
+    }
     if (c->creep == 1) for (int k = 0; k < 9; ++k) c->uniform[k] = set[k];
     else for (int k = 0; k < 9; ++k) c->uniform[k] += add[k];
     for (int k = 0; k < 9; ++k) c->dDisGradAvg[k] = c->acum[k] = 0.0;
@@ -1391,21 +1400,29 @@ static int step_iterate(srx_ctx *c) {
     return 0;
 }
 
-static int step_finish(srx_ctx *c, srx_step_report *rep) {
+// An increment's end in two halves around the end-of-increment kernel, so that a lockstep ensemble
+// can launch that kernel and its reduction once for all members: finish_begin takes the outer
+// iterations' results from the control block, finish_end everything after the kernel's reduced
+// values (c->hred).
+struct Finish {
+    srx_step_report r;
+    int iter, nrmax, do_orient;
+    long nrtot;
+    double nrwave;
+};
+
+static int finish_begin(srx_ctx *c, Finish &F) {
     const Deck &d = c->dk;
     const double dt = d.TimeStep;
-    const int prec = c->opt.precision;
-    const int N = (int)c->nvox;
-    (void)dt; (void)prec; (void)N; (void)d;
-    const int rotated = c->istep > 2 ? 1 : 0;
     IterCtl &ctl = *c->hctl;
-    srx_step_report r;
+    srx_step_report &r = F.r;
     std::memset(&r, 0, sizeof(r));
     if (ctl.nan) return SRX_E_NAN;
-    const int iter = ctl.iter, nrmax = ctl.nrmax;
-    const long nrtot = (long)ctl.nrtot;
-    const double nrwave = ctl.nrwave;
-    c->iters_last = iter;
+    F.iter = ctl.iter;
+    F.nrmax = ctl.nrmax;
+    F.nrtot = (long)ctl.nrtot;
+    F.nrwave = ctl.nrwave;
+    c->iters_last = F.iter;
     c->Err_s = ctl.Err_s;
     c->Err_e = ctl.Err_e;
     c->s_vm = ctl.s_vm;
@@ -1431,13 +1448,20 @@ static int step_finish(srx_ctx *c, srx_step_report *rep) {
     std::memcpy(c->DisGradAvg_t, act, sizeof(act));
     std::memcpy(r.disgrad_avg_actual, act, sizeof(act));
     std::memcpy(r.velgrad_avg, vga, sizeof(vga));
-    // Edot/Eps, update_orient (istep>1), DislocationEvolution, averages
-    const int do_orient = (d.Update_Flag == 1 && c->istep > 1) ? 1 : 0;
-    { Timer t(c, 6); launch_end(c->stream, c->dc, c->f, N, iter == 1 ? 1 : 0, rotated, do_orient, 1, c->istep, c->partials,
-                                      c->opt.gnd); }
-    { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->end_blocks, END_NPART, c->red); }
-    int rc = sync_read(c, END_NPART);
-    if (rc) return rc;
+    // Edot/Eps, update_orient (istep>1), DislocationEvolution, averages: the end-of-increment kernel
+    F.do_orient = (d.Update_Flag == 1 && c->istep > 1) ? 1 : 0;
+    return 0;
+}
+
+static int finish_end(srx_ctx *c, Finish &F, srx_step_report *rep) {
+    const Deck &d = c->dk;
+    const double dt = d.TimeStep;
+    const int N = (int)c->nvox;
+    srx_step_report &r = F.r;
+    const int iter = F.iter, nrmax = F.nrmax;
+    const long nrtot = F.nrtot;
+    const double nrwave = F.nrwave;
+    int rc = 0;
     const double *h = c->hred;
     if (h[21] > 0) return SRX_E_NAN;
     {
@@ -1508,6 +1532,18 @@ static int step_finish(srx_ctx *c, srx_step_report *rep) {
     return frc;
 }
 
+static int step_finish(srx_ctx *c, srx_step_report *rep) {
+    Finish F;
+    int rc = finish_begin(c, F);
+    if (rc) return rc;
+    const int rotated = c->istep > 2 ? 1 : 0;
+    { Timer t(c, 6); launch_end(c->stream, c->dc, c->f, (int)c->nvox, F.iter == 1 ? 1 : 0, rotated, F.do_orient, 1,
+                                c->istep, c->partials, c->opt.gnd); }
+    { Timer t(c, 7); launch_reduce(c->stream, c->partials, c->end_blocks, END_NPART, c->red); }
+    if ((rc = sync_read(c, END_NPART))) return rc;
+    return finish_end(c, F, rep);
+}
+
 int srx_step(srx_ctx *c, srx_step_report *rep) {
     if (!c) return SRX_E_ARG;
     if (c->istep >= c->nsteps) return SRX_E_DONE;
@@ -1529,11 +1565,12 @@ int srx_run(srx_ctx *c) {
     return 0;
 }
 
-// Lockstep ensemble run: every member advances one increment at a time together.  Each member's
-// prologue and end of increment run in turn, its FFT passes on one of up to eight side streams
-// (members spread over them), and the Newton solve, its reduction and the iteration control of all
-// members run as one launch each over the member table; the host reads back only the members' stop
-// flags, once per batch of ITER_BATCH iterations.
+// Lockstep ensemble run: every member advances one increment at a time together.  The increment
+// prologue, the Newton solve, its reduction and the iteration control, and the end-of-increment
+// kernel and its reduction of all members run as one launch each over member tables; each member's
+// FFT passes run on one of up to eight side streams (members spread over them).  The host reads
+// back only the members' stop flags, once per batch of ITER_BATCH iterations, and their control
+// blocks and reduced end-of-increment values once per increment.
 static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> &ctx, srx_ensemble_result *results,
                         int max_side) {
     const int M = (int)members.size();
@@ -1544,6 +1581,7 @@ static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> 
     std::vector<hipEvent_t> ev(NS, nullptr);
     hipEvent_t fork = nullptr;
     EnsMember *dtab = nullptr, *htab = nullptr;
+    EnsIncr *dinc = nullptr, *hinc = nullptr;
     int *ddone = nullptr, *hdone = nullptr;
     std::vector<hipStream_t> own(M);
     int rc = 0;
@@ -1556,6 +1594,8 @@ static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> 
     ok(hipEventCreateWithFlags(&fork, hipEventDisableTiming));
     ok(hipMalloc(&dtab, sizeof(EnsMember) * M));
     ok(hipHostMalloc((void **)&htab, sizeof(EnsMember) * M, hipHostMallocDefault));
+    ok(hipMalloc(&dinc, sizeof(EnsIncr) * M));
+    ok(hipHostMalloc((void **)&hinc, sizeof(EnsIncr) * M, hipHostMallocDefault));
     ok(hipMalloc(&ddone, sizeof(int) * M));
     ok(hipHostMalloc((void **)&hdone, sizeof(int) * M, hipHostMallocDefault));
     // the members order their own copies and kernels on the main stream during the run
@@ -1579,10 +1619,19 @@ static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> 
             if (ctx[k]->istep != istep) rc = SRX_E_ARG;   // members advance together from istep 0
         if (rc) break;
         const int rotated = istep > 2 ? 1 : 0;
-        int nb = 0, most = 0;
+        int nb = 0, most = 0, pb = 0;
         for (int j = 0; j < nact && !rc; ++j) {
             srx_ctx *c = ctx[act[j]];
-            if (int e = step_begin(c)) rc = e;
+            EnsIncr &inc = hinc[j];
+            if (int e = step_begin(c, &inc)) rc = e;
+            inc.cst = c->dc;
+            inc.f = c->f;
+            inc.nvox = (int)c->nvox;
+            inc.pblk0 = pb;
+            inc.pnblk = prologue_block_count((int)c->nvox);
+            pb += inc.pnblk;
+            inc.partials = c->partials;
+            inc.red = c->red;
             htab[j].cst = c->dc;
             htab[j].f = c->f;
             htab[j].nvox = (int)c->nvox;
@@ -1595,7 +1644,10 @@ static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> 
             nb += c->nr_blocks;
             most = std::max(most, std::max(c->itermax, c->opt.fixed_iters));
         }
-        if (rc || !ok(hipMemcpyAsync(dtab, htab, sizeof(EnsMember) * nact, hipMemcpyHostToDevice, S))) break;
+        if (rc || !ok(hipMemcpyAsync(dtab, htab, sizeof(EnsMember) * nact, hipMemcpyHostToDevice, S)) ||
+            !ok(hipMemcpyAsync(dinc, hinc, sizeof(EnsIncr) * nact, hipMemcpyHostToDevice, S)))
+            break;
+        launch_prologue_ens(S, dinc, nact, pb);
         const srx_options &o = ctx[act[0]]->opt;
         int ph2 = 0;
         for (int k : act) ph2 |= ctx[k]->hc.ph[1].xtal;
@@ -1634,10 +1686,40 @@ static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> 
         for (int j = 0; j < nact; ++j)
             ok(hipMemcpyAsync(ctx[act[j]]->hctl, ctx[act[j]]->dctl, sizeof(IterCtl), hipMemcpyDeviceToHost, S));
         if (!ok(hipStreamSynchronize(S))) break;
-        for (int k : act) {
-            const int e = step_finish(ctx[k], &results[k].last);
-            if (e) results[k].rc = e;
-            else ++results[k].increments;
+        // the end of the increment: every member's end-of-increment kernel and its reduction in one
+        // launch each (members whose iterations failed drop out of the table)
+        std::vector<Finish> fin(nact);
+        std::vector<int> fj;
+        int eb = 0;
+        for (int j = 0; j < nact; ++j) {
+            srx_ctx *c = ctx[act[j]];
+            if (const int e = finish_begin(c, fin[j])) {
+                results[act[j]].rc = e;
+                continue;
+            }
+            EnsIncr &inc = hinc[fj.size()];
+            if ((int)fj.size() != j) inc = hinc[j];
+            inc.blk0 = eb;
+            inc.nblk = c->end_blocks;
+            eb += c->end_blocks;
+            inc.lagged = fin[j].iter == 1 ? 1 : 0;
+            inc.orient = fin[j].do_orient;
+            fj.push_back(j);
+        }
+        const int nfin = (int)fj.size();
+        if (nfin) {
+            if (!ok(hipMemcpyAsync(dinc, hinc, sizeof(EnsIncr) * nfin, hipMemcpyHostToDevice, S))) break;
+            launch_end_ens(S, dinc, nfin, eb, istep, o.gnd);
+            launch_reduce_end_ens(S, dinc, nfin);
+            for (int j : fj)
+                ok(hipMemcpyAsync(ctx[act[j]]->hred, ctx[act[j]]->red, sizeof(double) * END_NPART,
+                                  hipMemcpyDeviceToHost, S));
+            if (!ok(hipStreamSynchronize(S))) break;
+        }
+        for (int j : fj) {
+            const int e = finish_end(ctx[act[j]], fin[j], &results[act[j]].last);
+            if (e) results[act[j]].rc = e;
+            else ++results[act[j]].increments;
         }
     }
     for (int j = 0; j < M; ++j) ctx[members[j]]->stream = own[j];
@@ -1647,8 +1729,10 @@ static int run_lockstep(const std::vector<int> &members, std::vector<srx_ctx *> 
     }
     if (fork) (void)hipEventDestroy(fork);
     if (dtab) (void)hipFree(dtab);
+    if (dinc) (void)hipFree(dinc);
     if (ddone) (void)hipFree(ddone);
     if (htab) (void)hipHostFree(htab);
+    if (hinc) (void)hipHostFree(hinc);
     if (hdone) (void)hipHostFree(hdone);
     return rc;
 }

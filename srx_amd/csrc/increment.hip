@@ -14,17 +14,32 @@ struct Vec9 {
 // evolution.c:1115-1128.  In a creep run DisGrad is reset to the macroscopic value; the
 // fluctuation it held (DisGrad minus the uniform part uold) is carried in fl0 so that the
 // accumulated fluctuation of evolution.c:1198 survives the reset.
-__global__ void k_prologue(Fields f, int nvox, Vec9 add, int creep, Vec9 set, Vec9 uold) {
-    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void prologue_voxel(const Fields &f, int nvox, unsigned blk, const double *add, int creep,
+                                               const double *set, const double *uold) {
+    const int v = blk * blockDim.x + threadIdx.x;
     if (v >= nvox) return;
     const size_t N = (size_t)nvox;
 #pragma unroll
     for (int k = 0; k < 9; ++k) {
         const double d = f.dg[k * N + v];
         f.dg0[k * N + v] = d;
-        f.dg[k * N + v] = creep ? set.v[k] : d + add.v[k];
-        if (creep && f.fl0) f.fl0[k * N + v] += d - uold.v[k];
+        f.dg[k * N + v] = creep ? set[k] : d + add[k];
+        if (creep && f.fl0) f.fl0[k * N + v] += d - uold[k];
     }
+}
+
+constexpr int PRO_BLOCK = 256;
+
+__global__ void k_prologue(Fields f, int nvox, Vec9 add, int creep, Vec9 set, Vec9 uold) {
+    prologue_voxel(f, nvox, blockIdx.x, add.v, creep, set.v, uold.v);
+}
+
+// every member of a lockstep ensemble in one launch: member m owns blocks [pblk0, pblk0 + pnblk)
+__global__ void __launch_bounds__(PRO_BLOCK) k_prologue_ens(const EnsIncr *__restrict__ tab, int nmem) {
+    int m = 0;
+    while (m + 1 < nmem && (int)blockIdx.x >= tab[m + 1].pblk0) ++m;
+    const EnsIncr &e = tab[m];
+    prologue_voxel(e.f, e.nvox, blockIdx.x - (unsigned)e.pblk0, e.add, e.creep, e.set, e.uold);
 }
 
 // ---------------------------------------------------------------------------
@@ -234,17 +249,19 @@ __device__ __forceinline__ void end_voxel(const DevConst *__restrict__ cst, cons
 #ifndef SRX_END_WAVES_PER_EU
 #define SRX_END_WAVES_PER_EU 2
 #endif
+// Block blk of nblk (0: the launch's grid size) of one grid's end-of-increment launch.  The kernel declares the block's LDS
+// (lv: END_NPART partial values per lane, lrs: NSYS densities per lane, both in columns of stride
+// END_BLOCK; red: one row of partials per wavefront), so that every address is a constant.
 template <bool GND>
-__global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(SRX_END_WAVES_PER_EU, 8))) k_end(const DevConst *__restrict__ cst, Fields f, int nvox,
-                                                   int lagged_possible, int do_orient, int istep,
-                                                   double *__restrict__ partials) {
-    const int v = blockIdx.x * END_BLOCK + threadIdx.x;
+__device__ __forceinline__ void end_block(const DevConst *__restrict__ cst, const Fields &f, int nvox,
+                                          int lagged_possible, int do_orient, int istep,
+                                          double *__restrict__ partials, unsigned blk, unsigned nblk, double *lv,
+                                          double *lrs, double (*red)[END_NPART]) {
+    const int v = blk * END_BLOCK + threadIdx.x;
     const size_t N = (size_t)nvox;
     const bool valid = v < nvox;
     const int ph = valid ? f.phase[v] : 0;
     // this lane's partial values, in an LDS column (stride END_BLOCK) rather than registers
-    __shared__ double lv[END_NPART * END_BLOCK];
-    __shared__ double lrs[NSYS * END_BLOCK];
     double *vals = lv + threadIdx.x;
 #pragma unroll
     for (int k = 0; k < END_NPART; ++k) vals[k * END_BLOCK] = 0.0;
@@ -271,7 +288,6 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
             vals[i * END_BLOCK] = ldc(f.eps + (size_t)(i) * N, N, 8u * (unsigned)v);
         }
     }
-    __shared__ double red[END_BLOCK / 64][END_NPART];
     const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
 #pragma unroll
     for (int k = 0; k < END_NPART; ++k) {
@@ -283,8 +299,32 @@ __global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(
         const int k = threadIdx.x;
         double acc = red[0][k];
         for (int w = 1; w < END_BLOCK / 64; ++w) acc += red[w][k];
-        partials[(size_t)k * gridDim.x + blockIdx.x] = acc;
+        partials[(size_t)k * (nblk ? nblk : gridDim.x) + blk] = acc;
     }
+}
+
+template <bool GND>
+__global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(SRX_END_WAVES_PER_EU, 8)))
+k_end(const DevConst *__restrict__ cst, Fields f, int nvox, int lagged_possible, int do_orient, int istep,
+      double *__restrict__ partials) {
+    __shared__ double lv[END_NPART * END_BLOCK];
+    __shared__ double lrs[NSYS * END_BLOCK];
+    __shared__ double red[END_BLOCK / 64][END_NPART];
+    end_block<GND>(cst, f, nvox, lagged_possible, do_orient, istep, partials, blockIdx.x, 0u, lv, lrs, red);
+}
+
+// every member of a lockstep ensemble in one launch: member m owns blocks [blk0, blk0 + nblk)
+template <bool GND>
+__global__ void __launch_bounds__(END_BLOCK) __attribute__((amdgpu_waves_per_eu(SRX_END_WAVES_PER_EU, 8)))
+k_end_ens(const EnsIncr *__restrict__ tab, int nmem, int istep) {
+    int m = 0;
+    while (m + 1 < nmem && (int)blockIdx.x >= tab[m + 1].blk0) ++m;
+    const EnsIncr &e = tab[m];
+    __shared__ double lv[END_NPART * END_BLOCK];
+    __shared__ double lrs[NSYS * END_BLOCK];
+    __shared__ double red[END_BLOCK / 64][END_NPART];
+    end_block<GND>(e.cst, e.f, e.nvox, e.lagged, e.orient, istep, e.partials, blockIdx.x - (unsigned)e.blk0,
+                   (unsigned)e.nblk, lv, lrs, red);
 }
 
 // ---------------------------------------------------------------------------
@@ -507,15 +547,29 @@ int end_block_count(int nvox) { return (nvox + END_BLOCK - 1) / END_BLOCK; }
 
 void launch_prologue(hipStream_t s, Fields f, int nvox, const double *a, int creep, const double *st,
                      const double *uo) {
-    const int B = 256;
     Vec9 add, set, uold;
     for (int k = 0; k < 9; ++k) {
         add.v[k] = a[k];
         set.v[k] = st[k];
         uold.v[k] = uo[k];
     }
-    SRX_LAUNCH(k_prologue, dim3((nvox + B - 1) / B), dim3(B), 0, s, f, nvox, add, creep, set, uold);
+    SRX_LAUNCH(k_prologue, dim3(prologue_block_count(nvox)), dim3(PRO_BLOCK), 0, s, f, nvox, add, creep, set, uold);
     check_launch("prologue");
+}
+
+int prologue_block_count(int nvox) { return (nvox + PRO_BLOCK - 1) / PRO_BLOCK; }
+
+void launch_prologue_ens(hipStream_t s, const EnsIncr *tab, int nmem, int pblocks) {
+    SRX_LAUNCH(k_prologue_ens, dim3(pblocks), dim3(PRO_BLOCK), 0, s, tab, nmem);
+    check_launch("prologue_ens");
+}
+
+void launch_end_ens(hipStream_t s, const EnsIncr *tab, int nmem, int eblocks, int istep, int gnd) {
+    if (gnd)
+        SRX_LAUNCH(k_end_ens<true>, dim3(eblocks), dim3(END_BLOCK), 0, s, tab, nmem, istep);
+    else
+        SRX_LAUNCH(k_end_ens<false>, dim3(eblocks), dim3(END_BLOCK), 0, s, tab, nmem, istep);
+    check_launch("end_ens");
 }
 
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int,

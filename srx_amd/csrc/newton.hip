@@ -666,12 +666,9 @@ k_nr_ens(const EnsMember *__restrict__ tab, int nmem) {
 // keeps eight independent accumulators over a fixed stride, combined in a fixed order, so the
 // result is the same on every run.
 constexpr int RED_BLOCK = 1024;
-__global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
-                                                      unsigned long long maxmask, double *__restrict__ out,
-                                                      const IterCtl *__restrict__ ctl) {
-    if (ctl && ctl->done) return;
-    __shared__ double sh[RED_BLOCK];
-    const int k = blockIdx.x;
+// value k of the block partials part[k][0 .. nblocks-1] into out[k]; sh: the kernel's RED_BLOCK doubles of LDS
+__device__ __forceinline__ void reduce_value(const double *__restrict__ part, int nblocks, unsigned long long maxmask,
+                                             double *__restrict__ out, unsigned k, double *sh) {
     const bool ismax = (maxmask >> k) & 1ull;
     const double *row = part + (size_t)k * nblocks;
     const double init = ismax ? -1e308 : 0.0;
@@ -696,7 +693,23 @@ __global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__
         __syncthreads();
     }
     if (threadIdx.x == 0) out[k] = sh[0];
+}
+
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce(const double *__restrict__ part, int nblocks, int nvals,
+                                                      unsigned long long maxmask, double *__restrict__ out,
+                                                      const IterCtl *__restrict__ ctl) {
+    if (ctl && ctl->done) return;
+    __shared__ double sh[RED_BLOCK];
+    reduce_value(part, nblocks, maxmask, out, blockIdx.x, sh);
     (void)nvals;
+}
+
+// the end-of-increment partials of every member of a lockstep ensemble: block (k, m) reduces value
+// k of member m into its `red`
+__global__ void __launch_bounds__(RED_BLOCK) k_reduce_end_ens(const EnsIncr *__restrict__ tab) {
+    const EnsIncr &e = tab[blockIdx.y];
+    __shared__ double sh[RED_BLOCK];
+    reduce_value(e.partials, e.nblk, 0ull, e.red, blockIdx.x, sh);
 }
 
 // First stage of the per-iteration reduction: block (k, sg) reduces value k over the sg-th of
@@ -932,6 +945,11 @@ void launch_reduce(hipStream_t s, const double *partials, int nblocks, int nvals
     const unsigned long long mask = nvals == NR_NPART ? (1ull << 15) : 0ull;
     SRX_LAUNCH(k_reduce, dim3(nvals), dim3(RED_BLOCK), 0, s, partials, nblocks, nvals, mask, out, ctl);
     check_launch("reduce");
+}
+
+void launch_reduce_end_ens(hipStream_t s, const EnsIncr *tab, int nmem) {
+    SRX_LAUNCH(k_reduce_end_ens, dim3(END_NPART, nmem), dim3(RED_BLOCK), 0, s, tab);
+    check_launch("reduce_end_ens");
 }
 
 void launch_reduce_nr(hipStream_t s, const double *partials, int nblocks, double *seg, const IterCtl *ctl) {

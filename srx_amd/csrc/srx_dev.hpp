@@ -244,6 +244,19 @@ struct EnsMember {
     IterCtl *ctl;
 };
 
+// One member of a lockstep ensemble's once-per-increment launches (k_prologue_ens, k_end_ens and
+// the reduction of the end-of-increment partials): the member owns the prologue blocks
+// [pblk0, pblk0 + pnblk) and the end blocks [blk0, blk0 + nblk) of those launches.
+struct EnsIncr {
+    const DevConst *cst;
+    Fields f;
+    int nvox, pblk0, pnblk, blk0, nblk;
+    int creep, lagged, orient;   // the prologue's creep flag; k_end's lagged_possible and do_orient
+    double add[9], set[9], uold[9];   // the prologue's arguments
+    double *partials;   // END_NPART x nblk block partials
+    double *red;        // END_NPART reduced values
+};
+
 // Radix plan of one 1-D FFT length.
 struct FftPlan {
     int n;
@@ -303,6 +316,13 @@ void launch_prologue(hipStream_t s, Fields f, int nvox, const double *add9, int 
                      const double *uold9);
 void launch_end(hipStream_t s, const DevConst *cst, const Fields &f, int nvox, int lagged_possible, int rotated,
                 int do_orient, int do_hardening, int istep, double *partials, int gnd = 0);
+// The increment prologue, the end-of-increment kernel and the reduction of its partials (into each
+// member's `red`) over every member of a lockstep ensemble in one launch each (tab: device array of
+// nmem members; pblocks, eblocks: their prologue and end blocks in total).
+int prologue_block_count(int nvox);
+void launch_prologue_ens(hipStream_t s, const EnsIncr *tab, int nmem, int pblocks);
+void launch_end_ens(hipStream_t s, const EnsIncr *tab, int nmem, int eblocks, int istep, int gnd);
+void launch_reduce_end_ens(hipStream_t s, const EnsIncr *tab, int nmem);
 // GND option: gradient of gamdot (grad = 1), GND density update, GND parts of rho_F / rho_P and one
 // partial per block (gnd_block_count) of sum sqrt(rho_g1^2 + rho_g2^2)
 int gnd_block_count(int nvox);
