@@ -54,23 +54,25 @@ def test_ensemble_matches_members_run_alone(tmp_path):
         assert a == b
 
 
-@pytest.mark.parametrize("concurrency", [0, -2])
-def test_lockstep_ensemble_matches_members_run_alone(tmp_path, concurrency):
-    """concurrency 0: all members advance together, their Newton solves, reductions and iteration
-    control in one launch each over the member table (-2: two such groups, concurrently); every
-    member's results equal its deck run alone, bit for bit (the same kernels per member, only
-    launched together)."""
+@pytest.mark.parametrize("concurrency,precision,gnd", [(0, srx_amd.PREC_FP64, 0), (-2, srx_amd.PREC_FP64, 0),
+                                                       (0, srx_amd.PREC_MIXED, 0), (0, srx_amd.PREC_FP64, 1)])
+def test_lockstep_ensemble_matches_members_run_alone(tmp_path, concurrency, precision, gnd):
+    """concurrency 0: all members advance together, their increment prologues, Newton solves,
+    reductions, iteration control and ends of increment in one launch each over the member table
+    (-2: two such groups, concurrently); every member's results equal its deck run alone, bit for
+    bit (the same per-voxel code per member, only launched together), in both precisions and with
+    the GND option."""
     decks = make_cases(str(tmp_path / "ens"))
-    res = srx_amd.run_ensemble(decks, concurrency=concurrency, write_files=1)
+    res = srx_amd.run_ensemble(decks, concurrency=concurrency, write_files=1, precision=precision, gnd=gnd)
     alone = make_cases(str(tmp_path / "alone"))
     for k, deck in enumerate(alone):
-        s = srx_amd.Solver(deck, write_files=1)
+        s = srx_amd.Solver(deck, write_files=1, precision=precision, gnd=gnd)
         for _ in range(3):
             last = s.step()
         s.close()
         r = res[k]
         assert r["rc"] == 0 and r["increments"] == 3, (k, r["rc"])
-        for key in ("iters", "nr_iters_total", "s_vm", "e_vm", "rho_s_avg"):
+        for key in ("iters", "nr_iters_total", "s_vm", "e_vm", "rho_s_avg", "rho_g_avg"):
             assert r["last"][key] == last[key], (k, key)
         assert r["last"]["sig_avg"] == last["sig_avg"]
         a = open(os.path.join(os.path.dirname(decks[k]), "vm.out")).read()
