@@ -245,8 +245,8 @@ def learned_bench(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2, help="timed increments")
-    ap.add_argument("--warmup", type=int, default=3,
+    ap.add_argument("--steps", type=int, default=20, help="timed increments")
+    ap.add_argument("--warmup", type=int, default=5,
                     help="untimed increments before the timed ones (the texture starts to rotate at increment 3)")
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--grains", type=int, default=1000)
