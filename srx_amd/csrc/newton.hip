@@ -83,6 +83,14 @@ struct NrPro {
 // An empty statement that takes x in a register: x is computed before it, and the statement stays
 // in program order with the memory waits around it.
 __device__ __forceinline__ void pin(double &x) { asm volatile("" : "+v"(x)); }
+// A field's base pointer as an opaque value at this point: the epilogue's buffer descriptors are
+// built after the Newton loop instead of being the prologue's identical ones, which the compiler
+// would keep in scalar registers (spilled to vector lanes) across the loop.
+template <typename T>
+__device__ __forceinline__ T *fresh(T *p) {
+    asm volatile("" : "+s"(p));
+    return p;
+}
 // The prologue is formed while the copies issued before it are in flight, not after the wait for
 // them (the compiler would otherwise sink the arithmetic to its first use).
 __device__ __forceinline__ void pin_prologue(NrPro &p) {
@@ -299,9 +307,10 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
         if (FIRST && it == 2) {
+            const double *nc = fresh(f.nrc);
 #pragma unroll
             for (int s = 0; s < NSYS; ++s)
-                tp[s * NR_BLOCK] = ldc(f.nrc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8);
+                tp[s * NR_BLOCK] = ldc(nc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8);
         }
         const double *tpu = tp;
         // SRX_NR_GROUP systems per trip, under one wavefront-uniform branch: their slip-rate
@@ -406,8 +415,9 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
         // the orientation again (the registers that held it were released for the Newton loop and
         // the LDS holds the per-system constants)
         double a[9];
+        const double *tr = fresh(f.tran);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) a[k] = ldc(f.tran + (size_t)(k) * N, N, o8);
+        for (int k = 0; k < 9; ++k) a[k] = ldc(tr + (size_t)(k) * N, N, o8);
         double Q[5][5];
         q5_from(a, Q);
         // eps_tot - eps_c at the pre-update iterate: q + edot dt + S' pre.  With the texture
@@ -463,7 +473,7 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     }
     res.sig[5] = sh;
 #pragma unroll
-    for (int k = 0; k < 5; ++k) stc(f.sig_pre + (size_t)(k) * N, N, o8, pre[k]);
+    for (int k = 0; k < 5; ++k) stc(fresh(f.sig_pre) + (size_t)(k) * N, N, o8, pre[k]);
     res.it = it;
     res.itot = itot;
     bool bad = false;
@@ -611,9 +621,10 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
     if (valid) {
         const unsigned o8 = 8u * (unsigned)v;
         // gas voxels: Sig = Edot = 0 (evolution.c:418-423)
+        double *sg = fresh(f.sig);
 #pragma unroll
-        for (int k = 0; k < 6; ++k) stc(f.sig + (size_t)(k) * N, N, o8, r.sig[k]);
-        f.nr_it[v] = r.it;
+        for (int k = 0; k < 6; ++k) stc(sg + (size_t)(k) * N, N, o8, r.sig[k]);
+        fresh(f.nr_it)[v] = r.it;
     }
     // block partials: conv_s, conv_e, sig6, sig6 of phase 1, nr_sum, nr_max, nan, wavefront nr_max
     const int lane = threadIdx.x & 63;
