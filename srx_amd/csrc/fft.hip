@@ -156,12 +156,23 @@ __device__ __forceinline__ void chunked_copy(int tot, Ld ld, St st) {
     }
 }
 
+// Position in LDS of element e of a pencil whose length is a multiple of 32: bits 0-4 of e XORed
+// with b and 8b, b = bits 5-6 of e (a permutation within each aligned run of 32 elements).  A
+// radix-8 first stage writes element 8j + r from butterfly j and a radix-4 second stage element
+// 32 jl + k + 8 r (j = 8 jl + k); unswizzled, four lanes of a pencil share each LDS bank in both,
+// swizzled the lanes of a pencil write distinct banks.  The passes' own LDS accesses use the same
+// positions.
+__device__ __forceinline__ int lds_swz(int e) {
+    const int b = (e >> 5) & 3;
+    return e ^ b ^ (b << 3);
+}
+
 // One Stockham stage (radix R, sub-transform span L) over `npen` pencils of
 // length N stored pencil-major in LDS.  Every butterfly of the stage is held
 // in registers between the load and the store, so the stage runs in place.
 template <typename T, int R, int MAXB>
 __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int L, const cx<T> *__restrict__ tw,
-                                               int sign, int pitch) {
+                                               int sign, int pitch, bool swz) {
     const int nb = N / R;
     const int total = npen * nb;
     const int tstep = N / (L * R);
@@ -172,9 +183,9 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
             const int p = dnb.div(t), j = t - p * nb;
-            const cx<T> *src = buf + p * pitch + j;
+            const cx<T> *src = buf + p * pitch + (swz ? 0 : j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
+            for (int r = 0; r < R; ++r) v[m][r] = src[swz ? lds_swz(j + r * nb) : r * nb];
         }
     }
     __syncthreads();
@@ -193,9 +204,10 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
                 }
             }
             Dft<T, R>::run(v[m], sign);
-            cx<T> *dst = buf + p * pitch + jl * L * R + k;
+            const int e0 = jl * L * R + k;
+            cx<T> *dst = buf + p * pitch + (swz ? 0 : e0);
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
+            for (int r = 0; r < R; ++r) dst[swz ? lds_swz(e0 + r * L) : r * L] = v[m][r];
         }
     }
     __syncthreads();
@@ -203,7 +215,7 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
 
 // The same stage with the pencil length and span known at compile time (powers of two):
 // the index arithmetic reduces to shifts and masks.
-template <typename T, int R, int MAXB, int N, int L>
+template <typename T, int R, int MAXB, int N, int L, bool SW>
 __device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign,
                                                   int pitch) {
     constexpr int nb = N / R;
@@ -215,9 +227,9 @@ __device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
             const int p = t / nb, j = t % nb;
-            const cx<T> *src = buf + p * pitch + j;
+            const cx<T> *src = buf + p * pitch + (SW ? 0 : j);
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
+            for (int r = 0; r < R; ++r) v[m][r] = src[SW ? lds_swz(j + r * nb) : r * nb];
         }
     }
     __syncthreads();
@@ -236,9 +248,10 @@ __device__ __forceinline__ void stockham_stage_ct(cx<T> *buf, int npen, const cx
                 }
             }
             Dft<T, R>::run(v[m], sign);
-            cx<T> *dst = buf + p * pitch + (j / L) * L * R + k;
+            const int e0 = (j / L) * L * R + k;
+            cx<T> *dst = buf + p * pitch + (SW ? 0 : e0);
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
+            for (int r = 0; r < R; ++r) dst[SW ? lds_swz(e0 + r * L) : r * L] = v[m][r];
         }
     }
     __syncthreads();
@@ -258,13 +271,13 @@ __host__ __device__ constexpr int pow2_radix(int N, int st) {
     return 0;
 }
 
-template <typename T, int MAXB, int N, int ST, int L>
+template <typename T, int MAXB, int N, int ST, int L, bool SW>
 __device__ __forceinline__ void lds_fft_pow2(cx<T> *buf, int npen, const cx<T> *__restrict__ tw, int sign,
                                              int pitch) {
     constexpr int R = pow2_radix(N, ST);
     if constexpr (R != 0) {
-        stockham_stage_ct<T, R, MAXB, N, L>(buf, npen, tw, sign, pitch);
-        lds_fft_pow2<T, MAXB, N, ST + 1, L * R>(buf, npen, tw, sign, pitch);
+        stockham_stage_ct<T, R, MAXB, N, L, SW>(buf, npen, tw, sign, pitch);
+        lds_fft_pow2<T, MAXB, N, ST + 1, L * R, SW>(buf, npen, tw, sign, pitch);
     }
 }
 
@@ -281,31 +294,33 @@ __device__ __forceinline__ bool pow2_default(const FftPlan &pl) {
 }
 
 // Full 1-D transform of npen LDS-resident pencils.  Caller guarantees
-// npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.
+// npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.  SWZ: pencils whose
+// length is a multiple of 32 are stored swizzled (lds_swz), and the caller's own accesses use it too.
 // POW2: lengths 32..256 with make_plan's radices use the compile-time stages (measured faster in
 // the x and y passes; in the z passes, which interleave the transform with the real/complex
 // packing, the extra registers cost an occupancy step and the runtime stages are faster).
-template <typename T, int MAXB, bool POW2 = true>
+template <typename T, int MAXB, bool POW2 = true, bool SWZ = true>
 __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign,
                         int pitch) {
     if constexpr (POW2) {
         switch (pl.n) {
-        case 32: if (pow2_default<32>(pl)) { lds_fft_pow2<T, MAXB, 32, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
-        case 64: if (pow2_default<64>(pl)) { lds_fft_pow2<T, MAXB, 64, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
-        case 128: if (pow2_default<128>(pl)) { lds_fft_pow2<T, MAXB, 128, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
-        case 256: if (pow2_default<256>(pl)) { lds_fft_pow2<T, MAXB, 256, 0, 1>(buf, npen, tw, sign, pitch); return; } break;
+        case 32: if (pow2_default<32>(pl)) { lds_fft_pow2<T, MAXB, 32, 0, 1, SWZ>(buf, npen, tw, sign, pitch); return; } break;
+        case 64: if (pow2_default<64>(pl)) { lds_fft_pow2<T, MAXB, 64, 0, 1, SWZ>(buf, npen, tw, sign, pitch); return; } break;
+        case 128: if (pow2_default<128>(pl)) { lds_fft_pow2<T, MAXB, 128, 0, 1, SWZ>(buf, npen, tw, sign, pitch); return; } break;
+        case 256: if (pow2_default<256>(pl)) { lds_fft_pow2<T, MAXB, 256, 0, 1, SWZ>(buf, npen, tw, sign, pitch); return; } break;
         default: break;
         }
     }
+    const bool swz = SWZ && pl.n % 32 == 0;
     int L = 1;
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
         switch (R) {
-        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
-        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
-        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
-        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
-        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
+        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
+        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
+        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
+        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
         }
         L *= R;
     }
@@ -360,7 +375,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
                 buf[yy * P + z] = cx<T>{T(v.x - ma), T(two ? v.y - mb : 0.0)};
             });
         __syncthreads();
-        lds_fft<T, 2, false>(buf, ty, pz, stw, -1, P);
+        lds_fft<T, 2, false, false>(buf, ty, pz, stw, -1, P);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
             const int kz = dty.div(i), yy = i - kz * ty;
             const cx<T> z1 = buf[yy * P + kz];
@@ -386,8 +401,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int P = n1 + 1;   // padded pencil pitch
+    const int P = lds_pitch(n1);   // padded pencil pitch
     cx<T> *stw = buf + TX * P;
+    const bool sw = n1 % 32 == 0;   // pencils stored swizzled (lds_swz), as lds_fft transforms them
     for (int i = threadIdx.x; i < n1; i += FFT_BLOCK) stw[i] = tw[i];
     const int H = n2 / 2 + 1;
     const int x0 = blockIdx.x * TX;
@@ -403,7 +419,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
         chunked_copy<cx<T>>(tx * n1, [&](int i) { return src[i]; },
                             [&](int i, cx<T> v) {
                                 const int xx = dn1.div(i), y = i - xx * n1;
-                                buf[xx * P + y] = v;
+                                buf[xx * P + (sw ? lds_swz(y) : y)] = v;
                             });
     } else {
         chunked_copy<cx<T>>(
@@ -414,7 +430,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
             },
             [&](int i, cx<T> v) {
                 const int y = dtx.div(i), xx = i - y * tx;
-                buf[xx * P + y] = v;
+                buf[xx * P + (sw ? lds_swz(y) : y)] = v;
             });
     }
     __syncthreads();
@@ -422,12 +438,12 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int y = dtx.div(i), xx = i - y * tx;
-            out[base + (size_t)y * n0 + x0 + xx] = buf[xx * P + y];
+            out[base + (size_t)y * n0 + x0 + xx] = buf[xx * P + (sw ? lds_swz(y) : y)];
         }
     } else {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int xx = dn1.div(i), y = i - xx * n1;
-            out[base + (size_t)(x0 + xx) * n1 + y] = buf[xx * P + y];
+            out[base + (size_t)(x0 + xx) * n1 + y] = buf[xx * P + (sw ? lds_swz(y) : y)];
         }
     }
 }
@@ -667,7 +683,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
         },
         [&](int i, cx<T> v) { buf[i] = v; });
     __syncthreads();
-    if (mode != 2) lds_fft<T, MAXB>(buf, cin * ty, px, stw, -1, n0);
+    if (mode != 2) lds_fft<T, MAXB, true, false>(buf, cin * ty, px, stw, -1, n0);
     if (mode == 1) {
         double dc6[6] = {dc0, dc1, dc2, dc3, dc4, dc5};
         if (ctl)
@@ -683,7 +699,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_green_x(cx<T> *__restrict__ spec,
         }
         __syncthreads();
     }
-    if (mode != 0) lds_fft<T, MAXB>(buf, cout * ty, px, stw, +1, n0);
+    if (mode != 0) lds_fft<T, MAXB, true, false>(buf, cout * ty, px, stw, +1, n0);
     for (int i = threadIdx.x; i < cout * slot; i += FFT_BLOCK) {
         const int c = dslot.div(i), r = i - c * slot, yy = dn0.div(r), x = r - yy * n0;
         spec[((size_t)c * H + kz) * plane + (size_t)(y0 + yy) * n0 + x] = buf[c * slot + yy * n0 + x];
@@ -933,7 +949,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
             }
         }
         __syncthreads();
-        lds_fft<T, 2, false>(buf, ty, pz, stw, +1, P);
+        lds_fft<T, 2, false, false>(buf, ty, pz, stw, +1, P);
         for (int b = 0; b < tot; b += FFT_CHUNK * FFT_BLOCK) {
             if (accumulate && b > 0) {
 #pragma unroll
@@ -1037,16 +1053,19 @@ int plan_candidates(int n, FftPlan *out, int cap) {
 // LDS budget of one FFT block (bytes)
 constexpr int FFT_LDS_MAX = 48 * 1024;
 
-// LDS of a y or z pass: t pencils at the padded pitch n + 1, then the twiddle table
-static size_t padded_lds(int t, int n, size_t csz) { return ((size_t)t * (n + 1) + n) * csz; }
+// LDS of a y pass (sw) or z pass: t pencils at the padded pitch (lds_pitch(n), or n + 1), then the
+// twiddle table
+static size_t padded_lds(int t, int n, size_t csz, bool sw) {
+    return ((size_t)t * (sw ? lds_pitch(n) : n + 1) + n) * csz;
+}
 
-// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*(n+1) complex values within
+// largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*lds_pitch(n) complex values within
 // FFT_LDS_MAX and t <= cap
 static int tile_for(const FftPlan &p, int per_tile_comps, int maxb, int cap, int prec) {
     const int mr = min_radix(p);
     int t = (maxb * FFT_BLOCK * mr) / (p.n * per_tile_comps);
     const int csz = prec == 0 ? 16 : 8;
-    const int tl = (FFT_LDS_MAX - p.n * csz) / (per_tile_comps * (p.n + 1) * csz);
+    const int tl = (FFT_LDS_MAX - p.n * csz) / (per_tile_comps * lds_pitch(p.n) * csz);
     if (t > tl) t = tl;
     if (t > cap) t = cap;
     if (t < 1) t = 1;
@@ -1081,11 +1100,11 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
     const int TY = zy_tile(pz, n[1], tile, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, 3);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     }
@@ -1099,11 +1118,11 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
-        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>));
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>), true);
         SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX, ctl);
     } else {
-        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>));
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>), true);
         SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX, ctl);
     }
@@ -1180,12 +1199,12 @@ void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double 
     dim3 grid(n[0], (n[1] + TY - 1) / TY, 5);
     const double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // dDisGradAvg comes from ctl
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
                            a[6], a[7], a[8], ctl);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
                            a[7], a[8], ctl);
@@ -1199,11 +1218,11 @@ void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     }
@@ -1220,12 +1239,12 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
                            0., 0., nullptr);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
                            0., nullptr);

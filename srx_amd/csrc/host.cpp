@@ -387,7 +387,7 @@ std::string resolve(const std::string &dir, const char *name) {
 }
 
 // The passes' limits, as the launchers size them: a y or z pass holds one pencil (padded pitch
-// n + 1) and the twiddle table in 48 KB of LDS with at most 2 butterflies per thread; the Green x
+// lds_pitch(n)) and the twiddle table in 48 KB of LDS with at most 2 butterflies per thread; the Green x
 // pass (LDS kernel, one row per block) holds 9 component rows and the twiddle table, 10 rows of n0
 // complex values (16 B FP64, 8 B mixed), with at most 4 butterflies per thread.
 bool grid_supported(const FftPlan pl[3], const int n[3], int prec) {
@@ -400,7 +400,7 @@ bool grid_supported(const FftPlan pl[3], const int n[3], int prec) {
         const int slots = ax == 0 ? 9 : 1;
         const int maxb = ax == 0 ? 4 : 2;   // butterflies per thread of the x pass and of the y/z passes
         if ((long)slots * p.n / std::max(mr, 1) > (long)maxb * 256) return false;
-        const long lds = ax == 0 ? 10L * p.n * csz : (2L * p.n + 1) * csz;
+        const long lds = ax == 0 ? 10L * p.n * csz : ((long)lds_pitch(p.n) + p.n) * csz;
         if (lds > 48 * 1024) return false;
     }
     if (n[0] < 1 || n[1] < 1 || n[2] < 1) return false;
