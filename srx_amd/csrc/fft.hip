@@ -297,8 +297,8 @@ __device__ __forceinline__ bool pow2_default(const FftPlan &pl) {
 // npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.  SWZ: pencils whose
 // length is a multiple of 32 are stored swizzled (lds_swz), and the caller's own accesses use it too.
 // POW2: lengths 32..256 with make_plan's radices use the compile-time stages (measured faster in
-// the x and y passes; in the z passes, which interleave the transform with the real/complex
-// packing, the extra registers cost an occupancy step and the runtime stages are faster).
+// the x, y and z passes; in the z passes they cost an occupancy step, which the swizzled, conflict
+// free stages more than repay).
 template <typename T, int MAXB, bool POW2 = true, bool SWZ = true>
 __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__restrict__ tw, int sign,
                         int pitch) {
@@ -348,8 +348,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int P = n2 + 1;           // padded pencil pitch: rows fall on different LDS banks
+    const int P = lds_pitch(n2);    // padded pencil pitch: rows fall on different LDS banks
     cx<T> *stw = buf + TY * P;      // twiddle table staged in LDS
+    const bool sw = n2 % 32 == 0;   // pencils stored swizzled (lds_swz), as lds_fft transforms them
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
     const int ty = min(TY, n1 - y0);
@@ -372,14 +373,15 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
             ty * n2, [&](int i) { return cx<double>{ia[g0 + i], ib[g0 + i]}; },
             [&](int i, cx<double> v) {
                 const int yy = dn2.div(i), z = i - yy * n2;
-                buf[yy * P + z] = cx<T>{T(v.x - ma), T(two ? v.y - mb : 0.0)};
+                buf[yy * P + (sw ? lds_swz(z) : z)] = cx<T>{T(v.x - ma), T(two ? v.y - mb : 0.0)};
             });
         __syncthreads();
-        lds_fft<T, 2, false, false>(buf, ty, pz, stw, -1, P);
+        lds_fft<T, 2, true, true>(buf, ty, pz, stw, -1, P);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
             const int kz = dty.div(i), yy = i - kz * ty;
-            const cx<T> z1 = buf[yy * P + kz];
-            const cx<T> z2 = buf[yy * P + (kz == 0 ? 0 : n2 - kz)];
+            const int e2 = kz == 0 ? 0 : n2 - kz;
+            const cx<T> z1 = buf[yy * P + (sw ? lds_swz(kz) : kz)];
+            const cx<T> z2 = buf[yy * P + (sw ? lds_swz(e2) : e2)];
             const size_t o = ((size_t)kz * n0 + x) * n1 + y0 + yy;
             specB[ca * HS + o] = cx<T>{T(0.5) * (z1.x + z2.x), T(0.5) * (z1.y - z2.y)};
             if (two) specB[cb * HS + o] = cx<T>{T(0.5) * (z1.y + z2.y), T(-0.5) * (z1.x - z2.x)};
@@ -901,8 +903,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int P = n2 + 1;   // padded pencil pitch
-    cx<T> *stw = buf + TY * P;
+    const int P = lds_pitch(n2);    // padded pencil pitch: rows fall on different LDS banks
+    cx<T> *stw = buf + TY * P;      // twiddle table staged in LDS
+    const bool sw = n2 % 32 == 0;   // pencils stored swizzled (lds_swz), as lds_fft transforms them
     for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
@@ -932,8 +935,9 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
                 const int kz = dty.div(i), yy = i - kz * ty;
                 const cx<T> A = v.a;
                 const cx<T> B = two ? v.b : cx<T>{T(0), T(0)};
-                buf[yy * P + kz] = cx<T>{A.x - B.y, A.y + B.x};
-                if (kz >= 1 && kz <= n2 - H) buf[yy * P + (n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
+                buf[yy * P + (sw ? lds_swz(kz) : kz)] = cx<T>{A.x - B.y, A.y + B.x};
+                if (kz >= 1 && kz <= n2 - H)
+                    buf[yy * P + (sw ? lds_swz(n2 - kz) : n2 - kz)] = cx<T>{A.x + B.y, B.x - A.y};
             });
         // DisGrad of the block's rows (contiguous): the first chunk of each thread's values is
         // loaded before the transform, so its round trip to HBM overlaps the LDS work
@@ -949,7 +953,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
             }
         }
         __syncthreads();
-        lds_fft<T, 2, false, false>(buf, ty, pz, stw, +1, P);
+        lds_fft<T, 2, true, true>(buf, ty, pz, stw, +1, P);
         for (int b = 0; b < tot; b += FFT_CHUNK * FFT_BLOCK) {
             if (accumulate && b > 0) {
 #pragma unroll
@@ -963,7 +967,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
                 const int i = b + e * FFT_BLOCK + (int)threadIdx.x;
                 if (i < tot) {
                     const int yy = dn2.div(i), z = i - yy * n2;
-                    const cx<T> v = buf[yy * P + z];
+                    const cx<T> v = buf[yy * P + (sw ? lds_swz(z) : z)];
                     if (accumulate) {
                         da[g0 + i] = pre[e].x + (add[ca] + (double)v.x * rdiv);
                         if (two) db[g0 + i] = pre[e].y + (add[cb] + (double)v.y * rdiv);
@@ -1053,11 +1057,8 @@ int plan_candidates(int n, FftPlan *out, int cap) {
 // LDS budget of one FFT block (bytes)
 constexpr int FFT_LDS_MAX = 48 * 1024;
 
-// LDS of a y pass (sw) or z pass: t pencils at the padded pitch (lds_pitch(n), or n + 1), then the
-// twiddle table
-static size_t padded_lds(int t, int n, size_t csz, bool sw) {
-    return ((size_t)t * (sw ? lds_pitch(n) : n + 1) + n) * csz;
-}
+// LDS of a y or z pass: t pencils at the padded pitch lds_pitch(n), then the twiddle table
+static size_t padded_lds(int t, int n, size_t csz) { return ((size_t)t * lds_pitch(n) + n) * csz; }
 
 // largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*lds_pitch(n) complex values within
 // FFT_LDS_MAX and t <= cap
@@ -1100,11 +1101,11 @@ void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, c
     const int TY = zy_tile(pz, n[1], tile, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, 3);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     }
@@ -1118,11 +1119,11 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
-        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>), true);
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>));
         SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX, ctl);
     } else {
-        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>), true);
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>));
         SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX, ctl);
     }
@@ -1199,12 +1200,12 @@ void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double 
     dim3 grid(n[0], (n[1] + TY - 1) / TY, 5);
     const double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // dDisGradAvg comes from ctl
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
                            a[6], a[7], a[8], ctl);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
                            a[7], a[8], ctl);
@@ -1218,11 +1219,11 @@ void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     }
@@ -1239,12 +1240,12 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
         SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
                            0., 0., nullptr);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), false);
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
         SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
                            0., nullptr);

@@ -206,9 +206,9 @@ __host__ __device__ inline int ctl_stop(const IterCtl &c) {
     return !(c.iter < c.itermax && e > c.err);
 }
 
-// Pitch (complex values) of one LDS pencil of length n in the y passes: a pencil whose length is a
-// multiple of 32 is stored swizzled (lds_swz) at pitch n + 2, so that the pencils a wavefront covers
-// start on different LDS banks; other lengths at pitch n + 1 (the z passes always use n + 1).
+// Pitch (complex values) of one LDS pencil of length n in the z and y passes: a pencil whose length
+// is a multiple of 32 is stored swizzled (lds_swz) at pitch n + 2, so that the pencils a wavefront
+// covers start on different LDS banks; other lengths at pitch n + 1.
 __host__ __device__ inline int lds_pitch(int n) { return n % 32 == 0 ? n + 2 : n + 1; }
 
 // Pointers to the per-voxel device arrays (component-major).
