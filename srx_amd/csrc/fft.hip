@@ -172,7 +172,7 @@ __device__ __forceinline__ int lds_swz(int e) {
 // in registers between the load and the store, so the stage runs in place.
 template <typename T, int R, int MAXB>
 __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int L, const cx<T> *__restrict__ tw,
-                                               int sign, int pitch, bool swz) {
+                                               int sign, int pitch) {
     const int nb = N / R;
     const int total = npen * nb;
     const int tstep = N / (L * R);
@@ -183,9 +183,9 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
         const int t = threadIdx.x + m * FFT_BLOCK;
         if (t < total) {
             const int p = dnb.div(t), j = t - p * nb;
-            const cx<T> *src = buf + p * pitch + (swz ? 0 : j);
+            const cx<T> *src = buf + p * pitch + j;
 #pragma unroll
-            for (int r = 0; r < R; ++r) v[m][r] = src[swz ? lds_swz(j + r * nb) : r * nb];
+            for (int r = 0; r < R; ++r) v[m][r] = src[r * nb];
         }
     }
     __syncthreads();
@@ -204,10 +204,9 @@ __device__ __forceinline__ void stockham_stage(cx<T> *buf, int npen, int N, int 
                 }
             }
             Dft<T, R>::run(v[m], sign);
-            const int e0 = jl * L * R + k;
-            cx<T> *dst = buf + p * pitch + (swz ? 0 : e0);
+            cx<T> *dst = buf + p * pitch + jl * L * R + k;
 #pragma unroll
-            for (int r = 0; r < R; ++r) dst[swz ? lds_swz(e0 + r * L) : r * L] = v[m][r];
+            for (int r = 0; r < R; ++r) dst[r * L] = v[m][r];
         }
     }
     __syncthreads();
@@ -294,8 +293,8 @@ __device__ __forceinline__ bool pow2_default(const FftPlan &pl) {
 }
 
 // Full 1-D transform of npen LDS-resident pencils.  Caller guarantees
-// npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.  SWZ: pencils whose
-// length is a multiple of 32 are stored swizzled (lds_swz), and the caller's own accesses use it too.
+// npen*N/min_radix <= MAXB*FFT_BLOCK and a __syncthreads() before the call.  SWZ: the compile-time
+// stages store the pencils swizzled (lds_swz), and the caller's own accesses must do the same.
 // POW2: lengths 32..256 with make_plan's radices use the compile-time stages (measured faster in
 // the x, y and z passes; in the z passes they cost an occupancy step, which the swizzled, conflict
 // free stages more than repay).
@@ -311,16 +310,15 @@ __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__
         default: break;
         }
     }
-    const bool swz = SWZ && pl.n % 32 == 0;
     int L = 1;
     for (int st = 0; st < pl.nst; ++st) {
         const int R = pl.rad[st];
         switch (R) {
-        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
-        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
-        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
-        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
-        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign, pitch, swz); break;
+        case 8: stockham_stage<T, 8, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 4: stockham_stage<T, 4, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 2: stockham_stage<T, 2, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        case 3: stockham_stage<T, 3, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
+        default: stockham_stage<T, 5, MAXB>(buf, npen, pl.n, L, tw, sign, pitch); break;
         }
         L *= R;
     }
@@ -339,7 +337,7 @@ __device__ void lds_fft(cx<T> *buf, int npen, const FftPlan &pl, const cx<T> *__
 // ---------------------------------------------------------------------------
 // forward z: real comps -> half spectrum, layout B [c][kz][x][y]
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool SW>
 __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double *__restrict__ in, cx<T> *__restrict__ specB,
                                                      int ncomp, int n0, int n1, int n2, FftPlan pz,
                                                      const cx<T> *__restrict__ tw, int TY, double m0, double m1,
@@ -348,9 +346,11 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int P = lds_pitch(n2);    // padded pencil pitch: rows fall on different LDS banks
-    cx<T> *stw = buf + TY * P;      // twiddle table staged in LDS
-    const bool sw = n2 % 32 == 0;   // pencils stored swizzled (lds_swz), as lds_fft transforms them
+    // SW: the plan runs the compile-time stages, which store the pencils swizzled (lds_swz) at pitch
+    // n2 + 2, and the accesses here do the same; else the runtime stages, plain, at pitch n2 + 1
+    constexpr bool sw = SW;
+    const int P = SW ? n2 + 2 : n2 + 1;   // padded pencil pitch: rows fall on different LDS banks
+    cx<T> *stw = buf + TY * P;            // twiddle table staged in LDS
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
     const int ty = min(TY, n1 - y0);
@@ -376,7 +376,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
                 buf[yy * P + (sw ? lds_swz(z) : z)] = cx<T>{T(v.x - ma), T(two ? v.y - mb : 0.0)};
             });
         __syncthreads();
-        lds_fft<T, 2, true, true>(buf, ty, pz, stw, -1, P);
+        if constexpr (SW) lds_fft<T, 2, true, true>(buf, ty, pz, stw, -1, P);
+        else lds_fft<T, 2, false, false>(buf, ty, pz, stw, -1, P);
         for (int i = threadIdx.x; i < H * ty; i += FFT_BLOCK) {
             const int kz = dty.div(i), yy = i - kz * ty;
             const int e2 = kz == 0 ? 0 : n2 - kz;
@@ -395,7 +396,7 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_fwd_z(const double
 // (forward, sign -1); otherwise C -> B (inverse, sign +1).
 // grid: (ceil(n0/TX), H, ncomp)
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool SW>
 __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ in, cx<T> *__restrict__ out,
                                                      int n0, int n1, int n2, int b_to_c, FftPlan py,
                                                      const cx<T> *__restrict__ tw, int TX,
@@ -403,9 +404,10 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int P = lds_pitch(n1);   // padded pencil pitch
+    // SW: compile-time stages, swizzled pencils at pitch n1 + 2; else runtime stages, plain, n1 + 1
+    constexpr bool sw = SW;
+    const int P = SW ? n1 + 2 : n1 + 1;   // padded pencil pitch
     cx<T> *stw = buf + TX * P;
-    const bool sw = n1 % 32 == 0;   // pencils stored swizzled (lds_swz), as lds_fft transforms them
     for (int i = threadIdx.x; i < n1; i += FFT_BLOCK) stw[i] = tw[i];
     const int H = n2 / 2 + 1;
     const int x0 = blockIdx.x * TX;
@@ -436,7 +438,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) k_fft_y(const cx<T> *__restrict__ i
             });
     }
     __syncthreads();
-    lds_fft<T, 2>(buf, tx, py, stw, b_to_c ? -1 : +1, P);
+    if constexpr (SW) lds_fft<T, 2, true, true>(buf, tx, py, stw, b_to_c ? -1 : +1, P);
+    else lds_fft<T, 2, false, false>(buf, tx, py, stw, b_to_c ? -1 : +1, P);
     if (b_to_c) {
         for (int i = threadIdx.x; i < tx * n1; i += FFT_BLOCK) {
             const int y = dtx.div(i), xx = i - y * tx;
@@ -893,7 +896,7 @@ __global__ void __launch_bounds__(256, SRX_GREEN_MIN_BLOCKS) k_green_wave(cx<T> 
 // inverse z (c2r) fused with Eq. 15: dg[c] += ddg[c] + Re(IFFT)/Nxyz
 // grid: (n0, ceil(n1/TY)); `dg` may be an output (scale mode) when accumulate=0
 // ---------------------------------------------------------------------------
-template <typename T>
+template <typename T, bool SW>
 __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> *__restrict__ specB, double *__restrict__ dg,
                                                      int ncomp, int n0, int n1, int n2, FftPlan pz,
                                                      const cx<T> *__restrict__ tw, int TY, int accumulate,
@@ -903,9 +906,10 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
     if (ctl && ctl->done) return;
     extern __shared__ __align__(16) unsigned char smem_raw[];
     cx<T> *buf = reinterpret_cast<cx<T> *>(smem_raw);
-    const int P = lds_pitch(n2);    // padded pencil pitch: rows fall on different LDS banks
-    cx<T> *stw = buf + TY * P;      // twiddle table staged in LDS
-    const bool sw = n2 % 32 == 0;   // pencils stored swizzled (lds_swz), as lds_fft transforms them
+    // SW: compile-time stages, swizzled pencils at pitch n2 + 2; else runtime stages, plain, n2 + 1
+    constexpr bool sw = SW;
+    const int P = SW ? n2 + 2 : n2 + 1;   // padded pencil pitch
+    cx<T> *stw = buf + TY * P;
     for (int i = threadIdx.x; i < n2; i += FFT_BLOCK) stw[i] = tw[i];
     const int x = blockIdx.x;
     const int y0 = blockIdx.y * TY;
@@ -953,7 +957,8 @@ __global__ void __launch_bounds__(FFT_BLOCK) SRX_ZPASS_ATTR k_inv_z(const cx<T> 
             }
         }
         __syncthreads();
-        lds_fft<T, 2, true, true>(buf, ty, pz, stw, +1, P);
+        if constexpr (SW) lds_fft<T, 2, true, true>(buf, ty, pz, stw, +1, P);
+        else lds_fft<T, 2, false, false>(buf, ty, pz, stw, +1, P);
         for (int b = 0; b < tot; b += FFT_CHUNK * FFT_BLOCK) {
             if (accumulate && b > 0) {
 #pragma unroll
@@ -1057,8 +1062,23 @@ int plan_candidates(int n, FftPlan *out, int cap) {
 // LDS budget of one FFT block (bytes)
 constexpr int FFT_LDS_MAX = 48 * 1024;
 
-// LDS of a y or z pass: t pencils at the padded pitch lds_pitch(n), then the twiddle table
-static size_t padded_lds(int t, int n, size_t csz) { return ((size_t)t * lds_pitch(n) + n) * csz; }
+// LDS of a y or z pass: t pencils at the padded pitch (n + 2 swizzled, else n + 1), then the
+// twiddle table
+static size_t padded_lds(int t, int n, size_t csz, bool sw) { return ((size_t)t * (sw ? n + 2 : n + 1) + n) * csz; }
+
+// Whether a z or y pass runs plan p with the compile-time stages and swizzled pencils (the kernels'
+// SW = true instantiation): lengths 32..256 with make_plan's radices.  Other plans run the runtime
+// stages on plain pencils (SW = false), a separate instantiation so that neither pays for the
+// other's registers.
+static bool plan_swizzled(const FftPlan &p) {
+    if (p.n != 32 && p.n != 64 && p.n != 128 && p.n != 256) return false;
+    FftPlan d;
+    make_plan(p.n, d);
+    if (d.nst != p.nst) return false;
+    for (int s = 0; s < p.nst; ++s)
+        if (d.rad[s] != p.rad[s]) return false;
+    return true;
+}
 
 // largest tile count t with t*n/minrad <= maxb*FFT_BLOCK, t*comps*lds_pitch(n) complex values within
 // FFT_LDS_MAX and t <= cap
@@ -1099,14 +1119,23 @@ int fft_tile_max(int pass, const int n[3], const FftPlan &p, int prec) {
 void launch_fft_fwd_z(hipStream_t s, int prec, const double *sig, void *specB, const int n[3], const FftPlan &pz,
                       const void *twz, const IterCtl *ctl, int tile) {
     const int TY = zy_tile(pz, n[1], tile, prec);
+    const bool sw = plan_swizzled(pz);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, 3);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
-        SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), sw);
+        if (sw)
+            SRX_LAUNCH((k_fwd_z<double, true>), grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
+                           n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
+        else
+            SRX_LAUNCH((k_fwd_z<double, false>), grid, dim3(FFT_BLOCK), sm, s, sig, (cx<double> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
-        SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), sw);
+        if (sw)
+            SRX_LAUNCH((k_fwd_z<float, true>), grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
+                           n[2], pz, (const cx<float> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
+        else
+            SRX_LAUNCH((k_fwd_z<float, false>), grid, dim3(FFT_BLOCK), sm, s, sig, (cx<float> *)specB, 6, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, ctl);
     }
     check_launch("fwd_z");
@@ -1116,15 +1145,24 @@ void launch_fft_y(hipStream_t s, int prec, const void *in, void *out, int ncomp,
                   const FftPlan &py, const void *twy, const IterCtl *ctl, int tile) {
     (void)sign;
     const int TX = zy_tile(py, n[0], tile, prec);
+    const bool sw = plan_swizzled(py);
     const int H = n[2] / 2 + 1;
     dim3 grid((n[0] + TX - 1) / TX, H, ncomp);
     if (prec == 0) {
-        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>));
-        SRX_LAUNCH(k_fft_y<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<double>), sw);
+        if (sw)
+            SRX_LAUNCH((k_fft_y<double, true>), grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
+                           n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX, ctl);
+        else
+            SRX_LAUNCH((k_fft_y<double, false>), grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)in, (cx<double> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<double> *)twy, TX, ctl);
     } else {
-        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>));
-        SRX_LAUNCH(k_fft_y<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
+        size_t sm = padded_lds(TX, n[1], sizeof(cx<float>), sw);
+        if (sw)
+            SRX_LAUNCH((k_fft_y<float, true>), grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
+                           n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX, ctl);
+        else
+            SRX_LAUNCH((k_fft_y<float, false>), grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)in, (cx<float> *)out,
                            n[0], n[1], n[2], b_to_c, py, (const cx<float> *)twy, TX, ctl);
     }
     check_launch("fft_y");
@@ -1197,16 +1235,27 @@ void launch_green_x(hipStream_t s, int prec, void *specC, const int n[3], const 
 void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double *dg, const int n[3], const FftPlan &pz,
                              const void *twz, const IterCtl *ctl, double divisor, int tile) {
     const int TY = zy_tile(pz, n[1], tile, prec);
+    const bool sw = plan_swizzled(pz);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, 5);
     const double a[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};   // dDisGradAvg comes from ctl
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
-        SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), sw);
+        if (sw)
+            SRX_LAUNCH((k_inv_z<double, true>), grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
+                           n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
+                           a[6], a[7], a[8], ctl);
+        else
+            SRX_LAUNCH((k_inv_z<double, false>), grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<double> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5],
                            a[6], a[7], a[8], ctl);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
-        SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), sw);
+        if (sw)
+            SRX_LAUNCH((k_inv_z<float, true>), grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
+                           n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
+                           a[7], a[8], ctl);
+        else
+            SRX_LAUNCH((k_inv_z<float, false>), grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, dg, 9, n[0], n[1],
                            n[2], pz, (const cx<float> *)twz, TY, 1, divisor, a[0], a[1], a[2], a[3], a[4], a[5], a[6],
                            a[7], a[8], ctl);
     }
@@ -1217,14 +1266,23 @@ void launch_fft_inv_z_update(hipStream_t s, int prec, const void *specB, double 
 void launch_fft_real_forward_test(hipStream_t s, int prec, const double *in, int ncomp, void *specB, void *specC,
                                   const int n[3], const FftPlan *pl, void *const *tw) {
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
+    const bool sw = plan_swizzled(pl[2]);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
-        SRX_LAUNCH(k_fwd_z<double>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), sw);
+        if (sw)
+            SRX_LAUNCH((k_fwd_z<double, true>), grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
+                           n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
+        else
+            SRX_LAUNCH((k_fwd_z<double, false>), grid, dim3(FFT_BLOCK), sm, s, in, (cx<double> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<double> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
-        SRX_LAUNCH(k_fwd_z<float>, grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), sw);
+        if (sw)
+            SRX_LAUNCH((k_fwd_z<float, true>), grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
+                           n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
+        else
+            SRX_LAUNCH((k_fwd_z<float, false>), grid, dim3(FFT_BLOCK), sm, s, in, (cx<float> *)specB, ncomp, n[0], n[1],
                            n[2], pl[2], (const cx<float> *)tw[2], TY, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, nullptr);
     }
     check_launch("fwd_z(test)");
@@ -1238,15 +1296,26 @@ void launch_fft_real_inverse_test(hipStream_t s, int prec, void *specC, void *sp
     launch_green_generic(s, prec, specC, n, pl[0], tw[0], nullptr, nullptr, 2, ncomp, nullptr);
     launch_fft_y(s, prec, specC, specB, ncomp, +1, 0, n, pl[1], tw[1], nullptr);
     const int TY = tile_for(pl[2], 1, 2, n[1] < 16 ? n[1] : 16, prec);
+    const bool sw = plan_swizzled(pl[2]);
     dim3 grid(n[0], (n[1] + TY - 1) / TY, (ncomp + 1) / 2);
     if (prec == 0) {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>));
-        SRX_LAUNCH(k_inv_z<double>, grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<double>), sw);
+        if (sw)
+            SRX_LAUNCH((k_inv_z<double, true>), grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
+                           n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
+                           0., 0., nullptr);
+        else
+            SRX_LAUNCH((k_inv_z<double, false>), grid, dim3(FFT_BLOCK), sm, s, (const cx<double> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<double> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0.,
                            0., 0., nullptr);
     } else {
-        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>));
-        SRX_LAUNCH(k_inv_z<float>, grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
+        size_t sm = padded_lds(TY, n[2], sizeof(cx<float>), sw);
+        if (sw)
+            SRX_LAUNCH((k_inv_z<float, true>), grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
+                           n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
+                           0., nullptr);
+        else
+            SRX_LAUNCH((k_inv_z<float, false>), grid, dim3(FFT_BLOCK), sm, s, (const cx<float> *)specB, out, ncomp, n[0],
                            n[1], n[2], pl[2], (const cx<float> *)tw[2], TY, 0, divisor, 0., 0., 0., 0., 0., 0., 0., 0.,
                            0., nullptr);
     }

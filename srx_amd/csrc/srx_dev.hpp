@@ -206,9 +206,9 @@ __host__ __device__ inline int ctl_stop(const IterCtl &c) {
     return !(c.iter < c.itermax && e > c.err);
 }
 
-// Pitch (complex values) of one LDS pencil of length n in the z and y passes: a pencil whose length
-// is a multiple of 32 is stored swizzled (lds_swz) at pitch n + 2, so that the pencils a wavefront
-// covers start on different LDS banks; other lengths at pitch n + 1.
+// Largest pitch (complex values) of one LDS pencil of length n in the z and y passes, for sizing:
+// a pencil the compile-time stages transform is stored swizzled at pitch n + 2 (a multiple of 32
+// long), so that the pencils a wavefront covers start on different LDS banks; others at n + 1.
 __host__ __device__ inline int lds_pitch(int n) { return n % 32 == 0 ? n + 2 : n + 1; }
 
 // Pointers to the per-voxel device arrays (component-major).
