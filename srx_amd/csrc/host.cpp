@@ -669,6 +669,8 @@ int setup(srx_ctx *c) {
         const size_t bytes = sizeof(double) * 36 * nrc_pitch(N);
         HIPCK(hipMalloc(&F.nrc, bytes));
         if ((rc = ctx_zero(st, F.nrc, bytes))) return rc;
+        // mixed precision keeps them in FP32, in the same memory (36 nrcf_pitch(N) floats fit)
+        F.nrcf = c->opt.precision == SRX_PREC_MIXED ? reinterpret_cast<float *>(F.nrc) : nullptr;
     }
     if ((rc = dalloc(&F.sig, 6, 0.0)) || (rc = dalloc(&F.sig_pre, 5, 0.0)) || (rc = dalloc(&F.eps, 5, 0.0)) ||
         (rc = dalloc(&F.edot, 5, 0.0)) || (rc = dalloc(&F.dg, 9, 0.0)) || (rc = dalloc(&F.dg0, 9, 0.0)) ||
@@ -681,6 +683,7 @@ int setup(srx_ctx *c) {
         (rc = dalloc(&F.acr, 15, 0.0)) ||
         (c->creep == 1 && (rc = dalloc(&F.fl0, 9, 0.0))))
         return rc;
+    F.acrf = c->opt.precision == SRX_PREC_MIXED ? reinterpret_cast<float *>(F.acr) : nullptr;
     if (c->opt.gnd) {
         // GND densities start at zero (init.c:1077-1087); same-grain flags of Gradient_ExchangeGrainID
         // (constitutive.c:731-880): +x, -x, +y, -y, +z and, as the reference compares, the y- neighbour

@@ -115,8 +115,13 @@ __device__ __forceinline__ void nr_load(const Fields &f, size_t N, unsigned o8, 
 #pragma unroll
     for (int k = 0; k < 9; ++k) in.d[k] = ldc(f.dg + (size_t)k * N, N, o8);
     if (LOAD_A) {
+        if (f.acrf) {
 #pragma unroll
-        for (int k = 0; k < 15; ++k) in.A[k] = ldc(f.acr + (size_t)k * N, N, o8);
+            for (int k = 0; k < 15; ++k) in.A[k] = (double)ldcf(f.acrf + (size_t)k * N, N, o8 / 2);
+        } else {
+#pragma unroll
+            for (int k = 0; k < 15; ++k) in.A[k] = ldc(f.acr + (size_t)k * N, N, o8);
+        }
     }
 }
 
@@ -196,8 +201,17 @@ __device__ __forceinline__ void nr_prologue(const DevConst *__restrict__ cst, co
 #pragma unroll
             for (int i = 0; i < 5; ++i) pro.A[ps(i, i)] += P.sinv[i];
         }
+        if (f.acrf) {
 #pragma unroll
-        for (int k = 0; k < 15; ++k) stc(f.acr + (size_t)(k) * N, N, o8, pro.A[k]);
+            for (int k = 0; k < 15; ++k) {
+                const float a32 = (float)pro.A[k];
+                pro.A[k] = (double)a32;
+                stcf(f.acrf + (size_t)(k) * N, N, 4u * (unsigned)v, a32);
+            }
+        } else {
+#pragma unroll
+            for (int k = 0; k < 15; ++k) stc(f.acr + (size_t)(k) * N, N, o8, pro.A[k]);
+        }
     }
 }
 
@@ -236,8 +250,12 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
     // increment's from f.nrc (this lane's own stores).
     double *tp = lds + NRC_TP * NR_BLOCK + threadIdx.x, *inv = lds + NRC_INV * NR_BLOCK + threadIdx.x;
     double *g0 = lds + NRC_G0 * NR_BLOCK + threadIdx.x;
+    // mixed precision: the constants are stored in FP32 (f.nrcf), and this call iterates with the
+    // rounded values too, so that every call of the increment solves with the same constants
+    const size_t fpitch = nrcf_pitch(N);
     if (FIRST) {
         const int nsys = P.nsys;
+        const bool f32 = f.nrcf != nullptr;
         double rs[NSYS];
 #pragma unroll
         for (int s = 0; s < NSYS; ++s) rs[s] = ldc(f.rho_s + (size_t)(s) * N, N, o8);
@@ -259,14 +277,26 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
                 stc(f.rho_m_old + (size_t)(s) * N, N, o8, rmo);
                 stc(f.rho_m + (size_t)(s) * N, N, o8, rmn);
             } else {
+                // an inactive system: e = |rss| ci - ct stays negative (also as FP32's largest value)
                 ct = 1e300; ci = 0.0; cg = 0.0;
                 tp[s * NR_BLOCK] = 1e300;
             }
+            if (f32) {
+                ct = (double)(float)fmin(ct, 3.0e38);
+                ci = (double)(float)ci;
+                cg = (double)(float)cg;
+            }
             inv[s * NR_BLOCK] = ci;
             g0[s * NR_BLOCK] = cg;
-            stc(f.nrc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8, ct);
-            stc(f.nrc + (size_t)(NRC_INV + s) * nrc_pitch(N), nrc_pitch(N), o8, ci);
-            stc(f.nrc + (size_t)(NRC_G0 + s) * nrc_pitch(N), nrc_pitch(N), o8, cg);
+            if (f32) {
+                stcf(f.nrcf + (size_t)(NRC_TP + s) * fpitch, fpitch, 4u * (unsigned)v, (float)ct);
+                stcf(f.nrcf + (size_t)(NRC_INV + s) * fpitch, fpitch, 4u * (unsigned)v, (float)ci);
+                stcf(f.nrcf + (size_t)(NRC_G0 + s) * fpitch, fpitch, 4u * (unsigned)v, (float)cg);
+            } else {
+                stc(f.nrc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8, ct);
+                stc(f.nrc + (size_t)(NRC_INV + s) * nrc_pitch(N), nrc_pitch(N), o8, ci);
+                stc(f.nrc + (size_t)(NRC_G0 + s) * nrc_pitch(N), nrc_pitch(N), o8, cg);
+            }
         }
     }
 
@@ -307,10 +337,17 @@ __device__ __forceinline__ void nr_voxel(const DevConst *__restrict__ cst, const
 #pragma unroll
         for (int k = 0; k < 5; ++k) ed[k] = 0.0;
         if (FIRST && it == 2) {
-            const double *nc = fresh(f.nrc);
+            if (f.nrcf) {
+                const float *nc = fresh(f.nrcf);
 #pragma unroll
-            for (int s = 0; s < NSYS; ++s)
-                tp[s * NR_BLOCK] = ldc(nc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8);
+                for (int s = 0; s < NSYS; ++s)
+                    tp[s * NR_BLOCK] = (double)ldcf(nc + (size_t)(NRC_TP + s) * fpitch, fpitch, 4u * (unsigned)v);
+            } else {
+                const double *nc = fresh(f.nrc);
+#pragma unroll
+                for (int s = 0; s < NSYS; ++s)
+                    tp[s * NR_BLOCK] = ldc(nc + (size_t)(NRC_TP + s) * nrc_pitch(N), nrc_pitch(N), o8);
+            }
         }
         const double *tpu = tp;
         // SRX_NR_GROUP systems per trip, under one wavefront-uniform branch: their slip-rate
@@ -518,6 +555,42 @@ __device__ __forceinline__ void copy_field(const double *base, size_t P, double 
     }
 #endif
 }
+// The same for constants stored in FP32 at pitch P (a multiple of 4): one 16-byte load per lane moves
+// four components (lanes 16 q .. 16 q + 15 component c + q, four voxels each) into the upper half of
+// the NC slots, FP32 slot k at float offset (NC / 2 + k) 64 ... (NC a multiple of 4); widen_slots
+// then converts them in place.
+template <int NC>
+__device__ __forceinline__ void copy_field_f32(const float *base, size_t P, double *lds, int blk) {
+    static_assert(NC % 4 == 0, "four components per load");
+#if __HIP_DEVICE_COMPILE__
+    const int lane = threadIdx.x;
+    const int v0 = blk * NR_BLOCK;
+    const unsigned pb = 4u * (unsigned)min((size_t)(v0 + 4 * (lane & 15)), P - 4);
+    const unsigned q = 4u * (unsigned)P * (unsigned)(lane >> 4);
+    float *half = reinterpret_cast<float *>(lds + NC / 2 * NR_BLOCK);
+#pragma unroll
+    for (int c = 0; c < NC; c += 4)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(
+            __builtin_amdgcn_make_buffer_rsrc(const_cast<float *>(base + (size_t)c * P), (short)0,
+                                              rsrc_bytes(4 * P * 4u), 0x00020000),
+            (lds_void *)(half + c * NR_BLOCK), 16, pb + q, 0, 0, 0);
+#endif
+}
+// After copy_field_f32 and copy_wait: FP32 slot k -> FP64 slot k of the lane's voxel.  Every FP32
+// value of the lane is read before the first FP64 slot is written (the FP64 slots overlay the FP32
+// ones; a block is one wavefront, whose LDS accesses complete in program order).
+typedef float __attribute__((may_alias)) float_alias;
+template <int NC>
+__device__ __forceinline__ void widen_slots(double *lds) {
+    static_assert(NR_BLOCK == 64, "one wavefront per block");
+    const float_alias *half = reinterpret_cast<const float_alias *>(lds + NC / 2 * NR_BLOCK) + threadIdx.x;
+    float x[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) x[c] = half[c * NR_BLOCK];
+    asm volatile("" ::: "memory");
+#pragma unroll
+    for (int c = 0; c < NC; ++c) lds[c * NR_BLOCK + threadIdx.x] = (double)x[c];
+}
 __device__ __forceinline__ void copy_wait() {
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -594,10 +667,15 @@ __device__ __forceinline__ void nr_block(const DevConst *__restrict__ cst, const
         }
         if constexpr (WS == 2)
             nr_learned_start(cst, f, N, v, valid && ph >= 1 && ph <= NPH && cst->ph[ph - 1].xtal, in, wsp);
-        copy_field<NRC_COMPS>(f.nrc, nrc_pitch(N), lds, blk);
+        const bool f32 = f.nrcf != nullptr;
+        if (f32)
+            copy_field_f32<NRC_COMPS>(f.nrcf, nrcf_pitch(N), lds, blk);
+        else
+            copy_field<NRC_COMPS>(f.nrc, nrc_pitch(N), lds, blk);
         nr_prologue<false, ROT, 0, (WS != 0)>(cst, f, vv, N, in, pro, valid);
         pin_prologue(pro);
         copy_wait();
+        if (f32) widen_slots<NRC_COMPS>(lds);
     }
     NrResult r;
     r.cs = 0.0; r.ce = 0.0; r.it = 0; r.itot = 0; r.bad = 0;

@@ -17,6 +17,10 @@ constexpr int MAX_RADIX_STAGES = 12;
 // Component pitch of Fields::nrc: the voxel count rounded up to even, so that every component
 // starts 16-byte aligned.
 __host__ __device__ inline size_t nrc_pitch(size_t n) { return (n + 1) & ~(size_t)1; }
+// The same for the FP32 copy of the constants (mixed precision): a multiple of four voxels, so that
+// one 16-byte load moves four of a component's values.  36 nrcf_pitch(n) floats fit in the
+// 36 nrc_pitch(n) doubles of the FP64 array.
+__host__ __device__ inline size_t nrcf_pitch(size_t n) { return (n + 3) & ~(size_t)3; }
 // Largest grid (voxels): the per-voxel kernels address element v of a component, or of the
 // component after it, by a 32-bit byte offset (8 v, 8 (N + v)) from the component's base.
 constexpr long long SRX_MAX_VOXELS = (1ll << 28) - 1;
@@ -231,6 +235,9 @@ struct Fields {
                        //   increment, written by its first Newton call, read by the later ones
     double *nrc;       // [36][nrc_pitch(N)] per-system Newton constants of the increment, written by its first call:
                        //   tau_pass/(tau_0+tau_cut) [12], 1/(tau_0+tau_cut) [12], gamma_0 dt [12]
+    float *acrf;       // mixed precision: A rounded to FP32, [15][N] in acr's memory; null in FP64
+    float *nrcf;       // mixed precision: the same constants rounded to FP32, [36][nrcf_pitch(N)] in nrc's memory
+                       //   (every Newton call of the increment uses the rounded values); null in FP64
     float *dsig;       // [6][N] last Newton call's stress change, B basis, crystal frame (warm start only)
     float *ecp;        // [6][N] eps_c = sym(DisGrad) (B basis, sample frame) of the last Newton call (warm start only)
     const MlpPacked *mlp;   // learned Newton start (non-null: the start comes from the network)
