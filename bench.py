@@ -38,9 +38,11 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 FP64_PEAK_TFLOPS = 78.6        # MI355X FP64 (vector and matrix), TFLOP/s
 F_NR = 2300.0                  # flops per voxel per Newton iteration (algorithmic count)
-# HBM bytes per voxel per launch of the Newton kernel, from rocprofv3 FETCH_SIZE (x2.0) and
-# WRITE_SIZE (x1.0) runs calibrated on a 1 GiB device copy (profiles/evp_fft_128cubed_pmc.md)
-NR_TRAFFIC_BYTES_PER_VOXEL = 817.2
+# HBM bytes per voxel per launch of the Newton kernel (after the first of an increment), from
+# rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs calibrated on a 1 GiB device copy
+# (profiles/evp_fft_128cubed_pmc.md); mixed precision reads the increment's Newton matrix and
+# per-system constants as FP32
+NR_TRAFFIC_BYTES_PER_VOXEL = {"fp64": 817.2, "mixed": 611.5}
 METRIC = "converged voxel-increments/s at 128³, plain FP64 vs mixed-precision at equal stress error"
 
 
@@ -375,7 +377,7 @@ def main() -> int:
         ach = F_NR * E["nr_total"] / (kt["nr"][0] * 1e-3) / 1e12
         out["roofline"] = {"kernel": "nr", "bound": "fp64_valu", "peak_kind": "FP64 vector ALU (no MFMA in this kernel)",
                            "achieved": ach, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
-                           "frac": ach / FP64_PEAK_TFLOPS, "traffic": NR_TRAFFIC_BYTES_PER_VOXEL * nvox,
+                           "frac": ach / FP64_PEAK_TFLOPS, "traffic": NR_TRAFFIC_BYTES_PER_VOXEL[head] * nvox,
                            "traffic_unit": "HBM bytes per launch (rocprofv3 PMC, profiles/)",
                            "algorithmic": "%.0f flop per voxel per Newton iteration x %.3f Newton iterations per voxel "
                                           "per launch" % (F_NR, E["nr_total"] / nvox / max(E["iters"], 1))}
