@@ -52,7 +52,9 @@ enum {
 };
 
 typedef struct {
-    int    precision;        /* SRX_PREC_FP64 (default) or SRX_PREC_MIXED */
+    int    precision;        /* SRX_PREC_FP64 (default) or SRX_PREC_MIXED: FP32 FFT/Green passes and
+                                FP32 storage of each increment's Newton matrix and per-system
+                                constants, FP64 Newton arithmetic, hardening and residual */
     int    fixed_iters;      /* >0: exactly this many outer iterations per increment (parity runs) */
     double err_override;     /* >0: replaces the deck's Err */
     int    itermax_override; /* >0: replaces the deck's IterMax */
