@@ -656,6 +656,58 @@ __device__ __forceinline__ void green_point(const DevConst *__restrict__ cst, co
     }
 }
 
+// The same operator in factored form along an x row, for rows with ky and kz both off their
+// Nyquist indices and not both 0.  With t the unnormalised wave vector (unit_k), n = t / |t| and
+// du_ij = -w_i n_j, the y and z columns are t_y Y_i and t_z Y_i with Y_i = -w_i / |t|, and t_y, t_z
+// are constant along the row, so the row's inverse x transforms of those six components are
+// t_y IFFT(Y_i) and t_z IFFT(Y_i).  X_i is the x column.  At the x Nyquist index the Hermitian
+// average of green_point keeps this form: Y_i = -(w_i + w'_i) / (2 |t|), X_i = -(w_i - w'_i) n_x / 2.
+template <typename T>
+__device__ __forceinline__ void green_point_xy(const DevConst *__restrict__ cst, int kx, int ky, int kz, int n0,
+                                               int n1, int n2, const cx<T> q[6], cx<T> X[3], cx<T> Y[3]) {
+    using A = T;
+    const A *G6;
+    if constexpr (sizeof(T) == 4) G6 = cst->G6f;
+    else G6 = cst->G6;
+    A tzv, tyv, txv;
+    unit_k(kz, n2, tzv);
+    unit_k(ky, n1, tyv);
+    unit_k(kx, n0, txv);
+    const bool nyx = (n0 % 2 == 0) && (kx == n0 / 2);
+    A sr[6], si[6];
+    {
+        A vr[6], vi[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) {
+            vr[c] = (A)q[c].x;
+            vi[c] = (A)q[c].y;
+        }
+        b6_to_sym(vr, sr);
+        b6_to_sym(vi, si);
+    }
+    A nv[3] = {txv, tyv, tzv};
+    const A rg = A(1) / sqrt(nv[0] * nv[0] + nv[1] * nv[1] + nv[2] * nv[2]);
+    nv[0] *= rg; nv[1] *= rg; nv[2] *= rg;
+    A wr[3], wi[3];
+    green_w6(G6, nv, sr, si, wr, wi);
+    if (!nyx) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            X[a] = cx<T>{-wr[a] * nv[0], -wi[a] * nv[0]};
+            Y[a] = cx<T>{-wr[a] * rg, -wi[a] * rg};
+        }
+    } else {
+        const A np[3] = {-nv[0], nv[1], nv[2]};
+        A wr2[3], wi2[3];
+        green_w6(G6, np, sr, si, wr2, wi2);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            X[a] = cx<T>{A(0.5) * (-wr[a] + wr2[a]) * nv[0], A(0.5) * (-wi[a] + wi2[a]) * nv[0]};
+            Y[a] = cx<T>{A(-0.5) * (wr[a] + wr2[a]) * rg, A(-0.5) * (wi[a] + wi2[a]) * rg};
+        }
+    }
+}
+
 // x pass with the Green operator.  grid: (ceil(n1/TY), H).  LDS: 9 slots of TY*n0.
 // mode 1: forward x of 6 comps, Green, inverse x of 9 comps (the solver pass)
 // mode 0: forward x only (ncomp comps, testing)   mode 2: inverse x only (testing)
@@ -874,8 +926,37 @@ __global__ void __launch_bounds__(256, SRX_GREEN_MIN_BLOCKS) k_green_wave(cx<T> 
 #pragma unroll
         for (int r = 0; r < NPL; ++r) q[c][r] = spec[c * cstride + row + 64 * r];
     wave_fft_fwd<T, NPL, 6>(q, wl, ws, lane);
-    cx<T> o[9][NPL];
     const int kb = NPL * brev6(lane);
+    // the row's Green operator in factored form (6 inverse transforms) unless ky or kz is at its
+    // Nyquist index or the row holds k = 0 (wavefront-uniform)
+    const bool factored = !((n1 % 2 == 0 && ky == n1 / 2) || (n2 % 2 == 0 && kz == n2 / 2) || (ky == 0 && kz == 0));
+    if (factored) {
+        cx<T> o[6][NPL];   // X_0..2, Y_0..2
+#pragma unroll
+        for (int r = 0; r < NPL; ++r) {
+            cx<T> qi[6], xi[3], yi[3];
+#pragma unroll
+            for (int c = 0; c < 6; ++c) qi[c] = q[c][r];
+            green_point_xy(cst, r + kb, ky, kz, N, n1, n2, qi, xi, yi);
+#pragma unroll
+            for (int a = 0; a < 3; ++a) { o[a][r] = xi[a]; o[3 + a][r] = yi[a]; }
+        }
+        wave_fft_inv<T, NPL, 6>(o, wl, ws, lane);
+        T ty, tz;
+        unit_k(ky, n1, ty);
+        unit_k(kz, n2, tz);
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int r = 0; r < NPL; ++r) {
+                const cx<T> y = o[3 + a][r];
+                spec[(3 * a) * cstride + row + 64 * r] = o[a][r];
+                spec[(3 * a + 1) * cstride + row + 64 * r] = cx<T>{ty * y.x, ty * y.y};
+                spec[(3 * a + 2) * cstride + row + 64 * r] = cx<T>{tz * y.x, tz * y.y};
+            }
+        return;
+    }
+    cx<T> o[9][NPL];
 #pragma unroll
     for (int r = 0; r < NPL; ++r) {
         cx<T> qi[6], oi[9];
