@@ -42,7 +42,7 @@ F_NR = 2300.0                  # flops per voxel per Newton iteration (algorithm
 # rocprofv3 FETCH_SIZE (x2.0) and WRITE_SIZE (x1.0) runs calibrated on a 1 GiB device copy
 # (profiles/evp_fft_128cubed_pmc.md); mixed precision reads the increment's Newton matrix and
 # per-system constants as FP32
-NR_TRAFFIC_BYTES_PER_VOXEL = {"fp64": 817.2, "mixed": 611.5}
+NR_TRAFFIC_BYTES_PER_VOXEL = {"fp64": 817.2, "mixed": 609.4}
 METRIC = "converged voxel-increments/s at 128³, plain FP64 vs mixed-precision at equal stress error"
 
 
