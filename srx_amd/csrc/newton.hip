@@ -557,8 +557,8 @@ __device__ __forceinline__ void copy_field(const double *base, size_t P, double 
 }
 // The same for constants stored in FP32 at pitch P (a multiple of 4): one 16-byte load per lane moves
 // four components (lanes 16 q .. 16 q + 15 component c + q, four voxels each) into the upper half of
-// the NC slots, FP32 slot k at float offset (NC / 2 + k) 64 ... (NC a multiple of 4); widen_slots
-// then converts them in place.
+// the NC FP64 slots: FP32 slot k at float offset (NC + k) 64 from their start (NC a multiple of 4);
+// widen_slots then converts them in place.
 template <int NC>
 __device__ __forceinline__ void copy_field_f32(const float *base, size_t P, double *lds, int blk) {
     static_assert(NC % 4 == 0, "four components per load");
