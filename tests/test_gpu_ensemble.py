@@ -15,6 +15,7 @@ CASES = [  # (grid, grains, seed, strain rate, texture)
     ((8, 12, 10), 5, 22, 3e-3, "cube"),
     ((12, 12, 12), 7, 23, 5e-4, "random"),
     ((8, 8, 8), 3, 24, 1e-3, "cube"),
+    ((9, 6, 5), 3, 25, 2e-3, "random"),   # 270 voxels: padded pitches, a ragged last Newton block
 ]
 
 
